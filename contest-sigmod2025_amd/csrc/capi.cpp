@@ -1,0 +1,328 @@
+// C ABI (include/gpujoin.h) and the C++ contest surface (include/contest/plan.h).
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/contest/plan.h"
+#include "../../include/gpujoin.h"
+#include "engine.hpp"
+#include "gen.hpp"
+
+struct cgj_context {
+    std::unique_ptr<cgj::Context> impl;
+    std::string                   error;
+};
+struct cgj_dtable {
+    std::unique_ptr<cgj::DevTable> impl;
+};
+struct cgj_dresult {
+    std::unique_ptr<cgj::DevResult> impl;
+};
+
+namespace {
+
+template <class F>
+int guarded(cgj_context* ctx, F&& f) {
+    try {
+        if (ctx) ctx->error.clear();
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->error = e.what();
+        return -1;
+    } catch (...) {
+        if (ctx) ctx->error = "unknown error";
+        return -1;
+    }
+}
+
+void free_column(cgj_result_column& c) {
+    for (uint64_t i = 0; i < c.num_pages; ++i)
+        if (c.pages && c.pages[i]) ::operator delete(c.pages[i]);
+    delete[] c.pages;
+    c.pages     = nullptr;
+    c.num_pages = 0;
+}
+
+} // namespace
+
+extern "C" {
+
+cgj_context* cgj_context_create(int device) {
+    try {
+        auto* c = new cgj_context;
+        c->impl = std::make_unique<cgj::Context>(device);
+        return c;
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void cgj_context_destroy(cgj_context* ctx) { delete ctx; }
+
+const char* cgj_last_error(const cgj_context* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+
+int cgj_execute(cgj_context* ctx, const cgj_plan* plan, const cgj_host_table* inputs, uint32_t num_inputs,
+                cgj_result* out) {
+    return guarded(ctx, [&] {
+        std::memset(out, 0, sizeof(*out));
+        std::vector<std::unique_ptr<cgj::DevTable>> owned;
+        std::vector<const cgj::DevTable*>           views;
+        // upload only the columns that the plan scans
+        std::vector<std::vector<bool>> used(num_inputs);
+        for (uint32_t i = 0; i < num_inputs; ++i) used[i].assign(inputs[i].num_columns, false);
+        for (uint32_t n = 0; n < plan->num_nodes; ++n) {
+            const cgj_node& nd = plan->nodes[n];
+            if (nd.kind != CGJ_NODE_SCAN || nd.base_table_id >= num_inputs) continue;
+            for (uint32_t a = 0; a < nd.num_output_attrs; ++a)
+                if (nd.output_attrs[a].index < used[nd.base_table_id].size())
+                    used[nd.base_table_id][nd.output_attrs[a].index] = true;
+        }
+        for (uint32_t i = 0; i < num_inputs; ++i) {
+            auto dt = ctx->impl->upload(inputs[i], &used[i]);
+            views.push_back(dt.get());
+            owned.push_back(std::move(dt));
+        }
+        auto res = ctx->impl->execute(*plan, views);
+        ctx->impl->download(*res, *out);
+    });
+}
+
+void cgj_result_release(cgj_result* res) {
+    if (!res || !res->columns) return;
+    for (uint32_t c = 0; c < res->num_columns; ++c) free_column(res->columns[c]);
+    delete[] res->columns;
+    res->columns     = nullptr;
+    res->num_columns = 0;
+}
+
+cgj_dtable* cgj_dtable_upload(cgj_context* ctx, const cgj_host_table* table) {
+    cgj_dtable* out = nullptr;
+    guarded(ctx, [&] {
+        auto* t = new cgj_dtable;
+        t->impl = ctx->impl->upload(*table);
+        out     = t;
+    });
+    return out;
+}
+
+void cgj_dtable_free(cgj_context*, cgj_dtable* t) { delete t; }
+
+uint64_t cgj_dtable_num_rows(const cgj_dtable* t) { return t ? t->impl->nrows : 0; }
+
+uint64_t cgj_dtable_num_pages(const cgj_dtable* t, uint32_t column) {
+    if (!t || column >= t->impl->cols.size()) return 0;
+    return t->impl->cols[column].npages;
+}
+
+int cgj_execute_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                       cgj_dresult** out) {
+    return guarded(ctx, [&] {
+        *out = nullptr;
+        std::vector<const cgj::DevTable*> views;
+        for (uint32_t i = 0; i < num_inputs; ++i) views.push_back(inputs[i]->impl.get());
+        auto* r = new cgj_dresult;
+        try {
+            r->impl = ctx->impl->execute(*plan, views);
+        } catch (...) {
+            delete r;
+            throw;
+        }
+        *out = r;
+    });
+}
+
+uint64_t cgj_dresult_num_rows(const cgj_dresult* r) { return r ? r->impl->nrows : 0; }
+uint64_t cgj_dresult_num_pages(const cgj_dresult* r) { return r ? r->impl->total_pages : 0; }
+
+int cgj_dresult_download(cgj_context* ctx, const cgj_dresult* r, cgj_result* out) {
+    return guarded(ctx, [&] {
+        std::memset(out, 0, sizeof(*out));
+        ctx->impl->download(*r->impl, *out);
+    });
+}
+
+void cgj_dresult_free(cgj_context*, cgj_dresult* r) { delete r; }
+
+int cgj_last_stats(const cgj_context* ctx, cgj_stats* out) {
+    if (!ctx || !out) return -1;
+    *out = ctx->impl->stats;
+    return 0;
+}
+
+int cgj_scan_decode(cgj_context* ctx, const cgj_dtable* t, uint32_t column, uint64_t* d_slots) {
+    return guarded(ctx, [&] {
+        if (column >= t->impl->cols.size()) throw std::runtime_error("column out of range");
+        const cgj::DevColumn& c = t->impl->cols[column];
+        CGJ_HIP(cgj::launch_decode(c.view(0), c.nrows, d_slots, ctx->impl->stream()));
+        CGJ_HIP(hipStreamSynchronize(ctx->impl->stream()));
+    });
+}
+
+int cgj_encode_column(int32_t type, const void* values, const char* chars, const uint8_t* valid, uint64_t n,
+                      cgj_result_column* out) {
+    try {
+        std::memset(out, 0, sizeof(*out));
+        Column col(static_cast<DataType>(type));
+        switch (type) {
+        case CGJ_INT32: {
+            ColumnInserter<int32_t> ins(col);
+            auto*                   v = static_cast<const int32_t*>(values);
+            for (uint64_t i = 0; i < n; ++i) (valid && !valid[i]) ? ins.insert_null() : ins.insert(v[i]);
+            ins.finalize();
+            break;
+        }
+        case CGJ_INT64: {
+            ColumnInserter<int64_t> ins(col);
+            auto*                   v = static_cast<const int64_t*>(values);
+            for (uint64_t i = 0; i < n; ++i) (valid && !valid[i]) ? ins.insert_null() : ins.insert(v[i]);
+            ins.finalize();
+            break;
+        }
+        case CGJ_FP64: {
+            ColumnInserter<double> ins(col);
+            auto*                  v = static_cast<const double*>(values);
+            for (uint64_t i = 0; i < n; ++i) (valid && !valid[i]) ? ins.insert_null() : ins.insert(v[i]);
+            ins.finalize();
+            break;
+        }
+        case CGJ_VARCHAR: {
+            ColumnInserter<std::string> ins(col);
+            auto*                       off = static_cast<const uint64_t*>(values);
+            for (uint64_t i = 0; i < n; ++i) {
+                if (valid && !valid[i])
+                    ins.insert_null();
+                else
+                    ins.insert(std::string_view(chars + off[i], off[i + 1] - off[i]));
+            }
+            ins.finalize();
+            break;
+        }
+        default: return -1;
+        }
+        out->type      = type;
+        out->num_pages = col.pages.size();
+        out->pages     = new void*[col.pages.size() ? col.pages.size() : 1];
+        for (size_t i = 0; i < col.pages.size(); ++i) out->pages[i] = col.pages[i];
+        col.pages.clear(); // ownership moves to `out`
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+void cgj_column_release(cgj_result_column* col) {
+    if (col) free_column(*col);
+}
+
+cgj_dtable* cgj_dtable_generate(cgj_context* ctx, uint64_t seed, uint64_t num_rows, const cgj_gen_column* columns,
+                                uint32_t num_columns) {
+    cgj_dtable* out = nullptr;
+    guarded(ctx, [&] {
+        auto* t = new cgj_dtable;
+        t->impl = ctx->impl->generate(seed, num_rows, columns, num_columns);
+        out     = t;
+    });
+    return out;
+}
+
+int cgj_host_generate(uint64_t seed, uint64_t first_row, uint64_t num_rows, const cgj_gen_column* g,
+                      cgj_result_column* out) {
+    try {
+        std::memset(out, 0, sizeof(*out));
+        if (g->type == CGJ_VARCHAR) return -1;
+        std::vector<uint8_t>  valid(num_rows);
+        std::vector<uint64_t> vals(num_rows);
+        for (uint64_t i = 0; i < num_rows; ++i) {
+            uint64_t r = first_row + i;
+            valid[i]   = !cgj::gen_is_null(seed, g->null_stream, g->null_frac, r);
+            vals[i]    = cgj::gen_value(g->type, g->kind, seed, g->stream, g->domain ? g->domain : 1, g->zipf_s,
+                                        g->miss_frac, g->lo, g->hi, r);
+        }
+        if (g->type == CGJ_INT32) {
+            std::vector<int32_t> v32(num_rows);
+            for (uint64_t i = 0; i < num_rows; ++i) v32[i] = int32_t(uint32_t(vals[i]));
+            return cgj_encode_column(g->type, v32.data(), nullptr, valid.data(), num_rows, out);
+        }
+        return cgj_encode_column(g->type, vals.data(), nullptr, valid.data(), num_rows, out);
+    } catch (...) {
+        return -1;
+    }
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// contest surface: Contest::build_context / execute / destroy_context
+// (reference engine/Execute.cpp:28-49, include/plan.h:337-344)
+// ---------------------------------------------------------------------------
+namespace Contest {
+
+void* build_context() {
+    cgj_context* ctx = cgj_context_create(0);
+    if (!ctx) throw std::runtime_error("cannot create the GPU context");
+    return ctx;
+}
+
+void destroy_context(void* context) { cgj_context_destroy(static_cast<cgj_context*>(context)); }
+
+ColumnarTable execute(const Plan& plan, void* context) {
+    auto* ctx = static_cast<cgj_context*>(context);
+    std::unique_ptr<cgj_context, void (*)(cgj_context*)> own(nullptr, cgj_context_destroy);
+    if (!ctx) {
+        ctx = cgj_context_create(0);
+        if (!ctx) throw std::runtime_error("cannot create the GPU context");
+        own.reset(ctx);
+    }
+    // POD view of the plan
+    std::vector<std::vector<cgj_attr>> attrs(plan.nodes.size());
+    std::vector<cgj_node>              nodes(plan.nodes.size());
+    for (size_t i = 0; i < plan.nodes.size(); ++i) {
+        const PlanNode& pn = plan.nodes[i];
+        for (auto& [idx, dt] : pn.output_attrs) attrs[i].push_back(cgj_attr{idx, int32_t(dt)});
+        cgj_node& n        = nodes[i];
+        n                  = cgj_node{};
+        n.output_attrs     = attrs[i].data();
+        n.num_output_attrs = uint32_t(attrs[i].size());
+        if (auto* s = std::get_if<ScanNode>(&pn.data)) {
+            n.kind          = CGJ_NODE_SCAN;
+            n.base_table_id = s->base_table_id;
+        } else {
+            const JoinNode& j = std::get<JoinNode>(pn.data);
+            n.kind            = CGJ_NODE_JOIN;
+            n.build_left      = j.build_left ? 1 : 0;
+            n.left            = j.left;
+            n.right           = j.right;
+            n.left_attr       = j.left_attr;
+            n.right_attr      = j.right_attr;
+        }
+    }
+    std::vector<std::vector<cgj_host_column>> cols(plan.inputs.size());
+    std::vector<cgj_host_table>               tables(plan.inputs.size());
+    for (size_t t = 0; t < plan.inputs.size(); ++t) {
+        for (const Column& c : plan.inputs[t].columns)
+            cols[t].push_back(cgj_host_column{reinterpret_cast<const void* const*>(c.pages.data()), c.pages.size(),
+                                              int32_t(c.type)});
+        tables[t] = cgj_host_table{plan.inputs[t].num_rows, cols[t].data(), uint32_t(cols[t].size())};
+    }
+    cgj_plan   p{nodes.data(), uint32_t(nodes.size()), plan.root};
+    cgj_result r{};
+    if (cgj_execute(ctx, &p, tables.data(), uint32_t(tables.size()), &r) != 0)
+        throw std::runtime_error(std::string("execute: ") + cgj_last_error(ctx));
+    ColumnarTable out;
+    out.num_rows = r.num_rows;
+    for (uint32_t c = 0; c < r.num_columns; ++c) {
+        Column col(static_cast<DataType>(r.columns[c].type));
+        for (uint64_t i = 0; i < r.columns[c].num_pages; ++i) {
+            col.pages.push_back(static_cast<Page*>(r.columns[c].pages[i]));
+            r.columns[c].pages[i] = nullptr;
+        }
+        out.columns.push_back(std::move(col));
+    }
+    cgj_result_release(&r);
+    return out;
+}
+
+} // namespace Contest

@@ -1,0 +1,776 @@
+// Host engine: device tables, plan import, join ordering and pipeline
+// execution over the HIP kernels.
+//
+// Plan import follows the reference's semantics (engine/query/PlanImport.cpp:24-89):
+// every scan-node output column is an attribute, join conditions merge
+// attributes into equivalence classes, and the query result is the inner join
+// of all scanned relations over those classes, projected on the root's
+// output attributes. The given join order and build sides are advisory, as in
+// the reference (QueryPlan.cpp:748-782 re-plans from scratch): the planner
+// below orders joins itself with a DP over connected sub-graphs.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <numeric>
+
+#include "gen.hpp"
+#include "planner.hpp"
+
+namespace cgj {
+
+// ---------------------------------------------------------------------------
+// device heap
+// ---------------------------------------------------------------------------
+size_t DeviceHeap::round(size_t b) {
+    if (b <= (1u << 20)) return (b + 4095) & ~size_t(4095);
+    return (b + (2u << 20) - 1) & ~((size_t(2) << 20) - 1);
+}
+
+void* DeviceHeap::alloc(size_t bytes) {
+    size_t r  = round(bytes);
+    auto   it = free_.lower_bound(r);
+    if (it != free_.end() && it->first <= r + r / 4) {
+        void* p = it->second;
+        live_[p] = it->first;
+        free_.erase(it);
+        return p;
+    }
+    void*      p = nullptr;
+    hipError_t e = hipMalloc(&p, r);
+    if (e != hipSuccess) {
+        trim();
+        e = hipMalloc(&p, r);
+    }
+    if (e != hipSuccess) throw HipError("hipMalloc(" + std::to_string(r) + "): " + hipGetErrorString(e));
+    live_[p] = r;
+    return p;
+}
+
+void DeviceHeap::free(void* p) {
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    free_.emplace(it->second, p);
+    live_.erase(it);
+}
+
+void DeviceHeap::trim() {
+    for (auto& kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+}
+
+DeviceHeap::~DeviceHeap() {
+    trim();
+    for (auto& kv : live_) (void)hipFree(kv.first);
+    live_.clear();
+}
+
+ScanCol DevColumn::view(uint32_t col_id) const {
+    ScanCol c;
+    c.pages     = pages.get<uint8_t>();
+    c.row_start = row_start.get<uint32_t>();
+    c.idx_page  = idx_page.get<uint32_t>();
+    c.idx_nn    = idx_nn.get<uint32_t>();
+    c.type      = type;
+    c.col_id    = col_id;
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// context, tables
+// ---------------------------------------------------------------------------
+Context::Context(int device) : device_(device) {
+    CGJ_HIP(hipSetDevice(device));
+    CGJ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        cus_ = uint32_t(prop.multiProcessorCount);
+}
+
+Context::~Context() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    heap_.trim();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Context::prepare_column(DevColumn& c) {
+    uint64_t nidx = c.nrows / kIndexRows + 2;
+    c.row_start   = DevBuf(&heap_, (c.npages + 1) * sizeof(uint32_t));
+    c.idx_page    = DevBuf(&heap_, nidx * sizeof(uint32_t));
+    c.idx_nn      = DevBuf(&heap_, nidx * sizeof(uint32_t));
+    if (c.npages == 0) {
+        CGJ_HIP(hipMemsetAsync(c.row_start.get(), 0, sizeof(uint32_t), stream_));
+        return;
+    }
+    DevBuf scratch(&heap_, prepare_scratch_words(c.npages) * sizeof(uint32_t) + 16);
+    DevBuf total(&heap_, sizeof(uint64_t));
+    CGJ_HIP(launch_prepare_column(c.pages.get<uint8_t>(), c.npages, c.type, c.row_start.get<uint32_t>(),
+                                  c.idx_page.get<uint32_t>(), c.idx_nn.get<uint32_t>(), scratch.get<uint32_t>(),
+                                  total.get<uint64_t>(), stream_));
+    uint64_t rows = 0;
+    CGJ_HIP(hipMemcpyAsync(&rows, total.get(), sizeof(rows), hipMemcpyDeviceToHost, stream_));
+    CGJ_HIP(hipStreamSynchronize(stream_));
+    if (rows != c.nrows)
+        throw std::runtime_error("column pages hold " + std::to_string(rows) + " rows, table has " +
+                                 std::to_string(c.nrows));
+}
+
+std::unique_ptr<DevTable> Context::upload(const cgj_host_table& t, const std::vector<bool>* used) {
+    auto dt   = std::make_unique<DevTable>();
+    dt->nrows = t.num_rows;
+    const size_t kChunk = size_t(64) << 20;
+    void*        pinned = nullptr;
+    CGJ_HIP(hipHostMalloc(&pinned, kChunk, hipHostMallocDefault));
+    try {
+        for (uint32_t ci = 0; ci < t.num_columns; ++ci) {
+            const cgj_host_column& hc = t.columns[ci];
+            DevColumn              c;
+            c.type   = hc.type;
+            c.nrows  = t.num_rows;
+            if (used && !(*used)[ci]) { // never scanned: keep an empty placeholder
+                dt->cols.push_back(std::move(c));
+                continue;
+            }
+            c.npages = hc.num_pages;
+            c.pages  = DevBuf(&heap_, c.npages * kPageSize + 64);
+            uint64_t per = kChunk / kPageSize;
+            for (uint64_t p0 = 0; p0 < c.npages; p0 += per) {
+                uint64_t n = std::min(per, c.npages - p0);
+                CGJ_HIP(hipStreamSynchronize(stream_));
+                for (uint64_t i = 0; i < n; ++i)
+                    std::memcpy(static_cast<uint8_t*>(pinned) + i * kPageSize, hc.pages[p0 + i], kPageSize);
+                CGJ_HIP(hipMemcpyAsync(c.pages.get<uint8_t>() + p0 * kPageSize, pinned, n * kPageSize,
+                                       hipMemcpyHostToDevice, stream_));
+            }
+            prepare_column(c);
+            dt->cols.push_back(std::move(c));
+        }
+        CGJ_HIP(hipStreamSynchronize(stream_));
+    } catch (...) {
+        (void)hipHostFree(pinned);
+        throw;
+    }
+    CGJ_HIP(hipHostFree(pinned));
+    return dt;
+}
+
+// Page boundaries of a generated fixed-width column, by the contest writer's rule.
+static std::vector<uint32_t> gen_row_starts(const cgj_gen_column& g, uint64_t seed, uint64_t nrows) {
+    std::vector<uint32_t> rs;
+    uint32_t              w     = g.type == CGJ_INT32 ? 4 : 8;
+    uint32_t              first = w;
+    rs.push_back(0);
+    if (g.null_frac <= 0.0) {
+        uint32_t cap = 0;
+        {
+            uint32_t rows = 0, end = first;
+            while (!page_full(rows, end, true)) {
+                ++rows;
+                end += w;
+            }
+            cap = rows;
+        }
+        for (uint64_t r = cap; r < nrows; r += cap) rs.push_back(uint32_t(r));
+        rs.push_back(uint32_t(nrows));
+        return rs;
+    }
+    uint64_t base = stream_base(seed, g.null_stream);
+    uint32_t rows = 0, end = first;
+    for (uint64_t r = 0; r < nrows; ++r) {
+        bool valid = !(unit(draw(base, r)) < g.null_frac);
+        if (page_full(rows, end, valid)) {
+            rs.push_back(uint32_t(r));
+            rows = 0;
+            end  = first;
+        }
+        ++rows;
+        if (valid) end += w;
+    }
+    if (rows) rs.push_back(uint32_t(nrows));
+    return rs;
+}
+
+static GenSpec to_spec(const cgj_gen_column& g, uint64_t seed) {
+    GenSpec s;
+    s.type        = g.type;
+    s.kind        = g.kind;
+    s.seed        = seed;
+    s.stream      = g.stream;
+    s.null_stream = g.null_stream;
+    s.null_frac   = g.null_frac;
+    s.domain      = g.domain ? g.domain : 1;
+    s.zipf_s      = g.zipf_s;
+    s.miss_frac   = g.miss_frac;
+    s.lo          = g.lo;
+    s.hi          = g.hi;
+    return s;
+}
+
+std::unique_ptr<DevTable> Context::generate(uint64_t seed, uint64_t nrows, const cgj_gen_column* cols,
+                                            uint32_t ncols) {
+    auto dt   = std::make_unique<DevTable>();
+    dt->nrows = nrows;
+    for (uint32_t ci = 0; ci < ncols; ++ci) {
+        const cgj_gen_column& g = cols[ci];
+        if (g.type == CGJ_VARCHAR) throw std::runtime_error("generator: VARCHAR columns are not generated on device");
+        std::vector<uint32_t> rs = gen_row_starts(g, seed, nrows);
+        DevColumn             c;
+        c.type   = g.type;
+        c.nrows  = nrows;
+        c.npages = nrows ? rs.size() - 1 : 0;
+        c.pages  = DevBuf(&heap_, c.npages * kPageSize + 64);
+        if (c.npages) {
+            DevBuf drs(&heap_, rs.size() * sizeof(uint32_t));
+            CGJ_HIP(hipMemcpyAsync(drs.get(), rs.data(), rs.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+            CGJ_HIP(hipMemsetAsync(c.pages.get(), 0, c.npages * kPageSize, stream_));
+            CGJ_HIP(launch_gen_fill(to_spec(g, seed), drs.get<uint32_t>(), c.npages, c.pages.get<uint8_t>(), stream_));
+            CGJ_HIP(hipStreamSynchronize(stream_));
+        }
+        prepare_column(c);
+        dt->cols.push_back(std::move(c));
+    }
+    return dt;
+}
+
+JoinTuning Context::tune(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes) const {
+    return autotune_join(build_rows, probe_rows, key_bytes);
+}
+
+
+// ---------------------------------------------------------------------------
+// query execution
+// ---------------------------------------------------------------------------
+namespace {
+
+uint32_t log2_ceil(uint64_t x) {
+    uint32_t l = 0;
+    while ((1ull << l) < x) ++l;
+    return l;
+}
+
+// A hash table built on device: entries [key, attrs...], one word per class.
+struct HashTable {
+    DevBuf                slots;
+    DevBuf                bloom;
+    uint64_t              cap       = 0;
+    uint32_t              shift     = 64;
+    uint32_t              words     = 1;
+    uint32_t              bloom_log = 0;
+    bool                  has_bloom = false;
+    bool                  unique    = true;
+    uint64_t              tuples    = 0;
+    std::vector<uint32_t> eqs; // class of each word (word 0 = key)
+};
+
+// Materialised tuples: the driver of a pipeline that continues a long chain.
+struct Tuples {
+    DevBuf                buf;
+    uint64_t              n     = 0;
+    std::vector<uint32_t> eqs; // class of each word
+};
+
+struct Driver {
+    int           rel    = -1; // base relation, or -1 when `tuples` drives
+    const Tuples* tuples = nullptr;
+    uint64_t      set    = 0;  // relations covered
+};
+
+struct ProbeStep {
+    const HashTable* ht;
+    uint32_t         key_eq;
+    uint64_t         set; // relations of the build side
+};
+
+struct Target {
+    uint32_t              kind = TARGET_STAGE;
+    std::vector<uint32_t> eqs;       // TARGET_STAGE: class of each staged word
+    double                est  = 0;  // estimated tuples
+    DevBuf                stage;     // TARGET_STAGE output
+    uint64_t              count = 0; // tuples produced
+    DevResult*            result = nullptr;
+};
+
+class Executor {
+public:
+    Executor(Context& ctx, const Query& q, const PlanTree& t, const std::vector<const DevTable*>& inputs)
+        : ctx_(ctx), q_(q), t_(t), inputs_(inputs) {
+        counters_ = DevBuf(&ctx.heap(), sizeof(PipeCounters));
+        CGJ_HIP(hipEventCreate(&ev0_));
+        CGJ_HIP(hipEventCreate(&ev1_));
+    }
+    ~Executor() {
+        (void)hipEventDestroy(ev0_);
+        (void)hipEventDestroy(ev1_);
+    }
+
+    std::unique_ptr<DevResult> run();
+
+    double   ms_build = 0, ms_probe = 0;
+    uint32_t launches = 0, retries = 0;
+    uint64_t build_tuples = 0, probe_rows = 0;
+
+private:
+    Context&                                ctx_;
+    const Query&                            q_;
+    const PlanTree&                         t_;
+    const std::vector<const DevTable*>&     inputs_;
+    std::vector<std::unique_ptr<HashTable>> tables_;
+    std::vector<std::unique_ptr<Tuples>>    tuples_;
+    std::vector<const DevColumn*>           colreg_;
+    std::map<std::pair<uint32_t, uint64_t>, uint32_t> colid_;
+    DevBuf                                  counters_;
+    DevBuf                                  col_base_, col_npages_;
+    hipEvent_t                              ev0_ = nullptr, ev1_ = nullptr;
+
+    const DevColumn& column(uint32_t rel, uint64_t col) const {
+        return inputs_[q_.rels[rel].table]->cols[col];
+    }
+    uint32_t col_id(uint32_t rel, uint64_t col) {
+        uint64_t table = q_.rels[rel].table;
+        auto     key   = std::make_pair(uint32_t(table), col);
+        auto     it    = colid_.find(key);
+        if (it != colid_.end()) return it->second;
+        uint32_t id = uint32_t(colreg_.size());
+        if (id >= 4095) throw std::runtime_error("too many scanned columns");
+        colreg_.push_back(&column(rel, col));
+        colid_[key] = id;
+        return id;
+    }
+    void register_all_columns();
+    void walk(int node, std::vector<int>& joins, int& leaf) const;
+    const HashTable* build(int node, uint32_t key_eq);
+    const HashTable* make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est);
+    void pipeline(Driver d, std::vector<ProbeStep> probes, Target& t);
+    uint64_t launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms);
+};
+
+void Executor::walk(int node, std::vector<int>& joins, int& leaf) const {
+    // node = Join(build, probe); the probe side continues the pipeline down to a leaf
+    std::vector<int> stack;
+    int              n = node;
+    while (!t_.leaf(n)) {
+        stack.push_back(n);
+        n = t_.nodes[n].probe;
+    }
+    leaf = t_.nodes[n].rel;
+    joins.assign(stack.rbegin(), stack.rend()); // innermost join first
+}
+
+void Executor::register_all_columns() {
+    for (uint32_t r = 0; r < q_.rels.size(); ++r)
+        for (uint32_t e : q_.needed(1ull << r)) col_id(r, q_.attrs[q_.attr_of(r, e)].col);
+    for (uint32_t a : q_.out) col_id(q_.attrs[a].rel, q_.attrs[a].col);
+    std::vector<const uint8_t*> base;
+    std::vector<uint32_t>       np;
+    for (auto* c : colreg_) {
+        base.push_back(c->pages.get<uint8_t>());
+        np.push_back(uint32_t(c->npages));
+    }
+    if (base.empty()) {
+        base.push_back(nullptr);
+        np.push_back(0);
+    }
+    col_base_   = DevBuf(&ctx_.heap(), base.size() * sizeof(void*));
+    col_npages_ = DevBuf(&ctx_.heap(), np.size() * sizeof(uint32_t));
+    CGJ_HIP(hipMemcpyAsync(col_base_.get(), base.data(), base.size() * sizeof(void*), hipMemcpyHostToDevice,
+                           ctx_.stream()));
+    CGJ_HIP(hipMemcpyAsync(col_npages_.get(), np.data(), np.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           ctx_.stream()));
+}
+
+uint64_t Executor::launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms) {
+    a.counters    = counters_.get<PipeCounters>();
+    a.morsel_rows = 4096;
+    a.nrows       = nrows;
+    a.nmorsels    = (nrows + a.morsel_rows - 1) / a.morsel_rows;
+    a.col_base    = col_base_.get<const uint8_t* const>();
+    a.col_npages  = col_npages_.get<uint32_t>();
+    CGJ_HIP(hipMemsetAsync(a.counters, 0, sizeof(PipeCounters), ctx_.stream()));
+    uint64_t waves = std::min<uint64_t>(std::max<uint64_t>(a.nmorsels, 1), uint64_t(ctx_.cu_count()) * 16);
+    if (a.target == TARGET_PAGES && a.nvarchar) {
+        // staging is allocated by the caller for `waves`
+    }
+    CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
+    if (a.nmorsels) CGJ_HIP(launch_pipeline(a, nslots, uint32_t(waves), ctx_.stream()));
+    CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
+    CGJ_HIP(hipEventSynchronize(ev1_));
+    float f = 0;
+    CGJ_HIP(hipEventElapsedTime(&f, ev0_, ev1_));
+    ms += f;
+    ++launches;
+    return waves;
+}
+
+const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est) {
+    auto ht    = std::make_unique<HashTable>();
+    ht->eqs    = eqs;
+    ht->words  = uint32_t(eqs.size());
+    ht->tuples = staged.count;
+    JoinTuning tn = ctx_.tune(staged.count, probe_est, 8);
+    uint32_t   lg = log2_ceil(std::max<uint64_t>(staged.count, 1)) + tn.load_log;
+    if (lg < 6) lg = 6;
+    ht->cap   = 1ull << lg;
+    ht->shift = 64 - lg;
+    ht->slots = DevBuf(&ctx_.heap(), ht->cap * ht->words * sizeof(uint64_t));
+    DevBuf dup(&ctx_.heap(), sizeof(uint32_t));
+    CGJ_HIP(hipMemsetAsync(dup.get(), 0, sizeof(uint32_t), ctx_.stream()));
+    CGJ_HIP(launch_fill_u64(ht->slots.get<uint64_t>(), ht->cap * ht->words, kEmpty, ctx_.stream()));
+    if (tn.bloom_bits) {
+        uint64_t words = std::max<uint64_t>(1, staged.count * tn.bloom_bits / 64);
+        ht->bloom_log  = log2_ceil(words);
+        ht->bloom      = DevBuf(&ctx_.heap(), (1ull << ht->bloom_log) * sizeof(uint64_t));
+        ht->has_bloom  = true;
+        CGJ_HIP(launch_fill_u64(ht->bloom.get<uint64_t>(), 1ull << ht->bloom_log, 0, ctx_.stream()));
+    }
+    CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
+    CGJ_HIP(launch_ht_insert(staged.stage.get<uint64_t>(), staged.count, ht->words, ht->slots.get<uint64_t>(),
+                             ht->shift, ht->cap - 1, ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr,
+                             ht->bloom_log, dup.get<uint32_t>(), ctx_.stream()));
+    CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
+    uint32_t dflag = 0;
+    CGJ_HIP(hipMemcpyAsync(&dflag, dup.get(), sizeof(dflag), hipMemcpyDeviceToHost, ctx_.stream()));
+    CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
+    float f = 0;
+    CGJ_HIP(hipEventElapsedTime(&f, ev0_, ev1_));
+    ms_build += f;
+    ht->unique = dflag == 0;
+    build_tuples += staged.count;
+    staged.stage.reset();
+    tables_.push_back(std::move(ht));
+    return tables_.back().get();
+}
+
+const HashTable* Executor::build(int node, uint32_t key_eq) {
+    const TreeNode&       tn = t_.nodes[node];
+    std::vector<uint32_t> eqs{key_eq};
+    for (uint32_t e : q_.needed(tn.set))
+        if (e != key_eq) eqs.push_back(e);
+    if (eqs.size() > kMaxWords) throw std::runtime_error("hash-table entry wider than supported");
+    Target t;
+    t.kind = TARGET_STAGE;
+    t.eqs  = eqs;
+    t.est  = tn.card;
+    std::vector<int> joins;
+    int              leaf;
+    walk(node, joins, leaf);
+    std::vector<ProbeStep> probes;
+    uint64_t               cur = 1ull << leaf;
+    for (int j : joins) {
+        const TreeNode& jn  = t_.nodes[j];
+        const TreeNode& bn  = t_.nodes[jn.build];
+        auto            sh  = q_.shared(bn.set, cur);
+        if (sh.empty()) throw std::runtime_error("join without a predicate");
+        const HashTable* ht = build(jn.build, sh.front());
+        if (!ht) return nullptr;
+        probes.push_back(ProbeStep{ht, sh.front(), bn.set});
+        cur |= bn.set;
+    }
+    Driver d;
+    d.rel = leaf;
+    d.set = 1ull << leaf;
+    pipeline(d, probes, t);
+    if (t.count == 0) return nullptr; // empty build side: the inner join is empty
+    return make_table(t, eqs, uint64_t(t_.nodes[t_.root].card));
+}
+
+void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
+    // Chains longer than the kernel's probe limit are cut: the first part is
+    // materialised as tuples that drive the rest.
+    while (probes.size() > size_t(kMaxProbes)) {
+        std::vector<ProbeStep> head(probes.begin(), probes.begin() + kMaxProbes);
+        uint64_t               set = d.set;
+        for (auto& p : head) set |= p.set;
+        Target mid;
+        mid.kind = TARGET_STAGE;
+        mid.eqs  = q_.needed(set);
+        mid.est  = t.est;
+        pipeline(d, head, mid);
+        auto tp = std::make_unique<Tuples>();
+        tp->buf = std::move(mid.stage);
+        tp->n   = mid.count;
+        tp->eqs = mid.eqs;
+        tuples_.push_back(std::move(tp));
+        d.rel    = -1;
+        d.tuples = tuples_.back().get();
+        d.set    = set;
+        probes.erase(probes.begin(), probes.begin() + kMaxProbes);
+    }
+
+    PipeArgs a;
+    std::memset(&a, 0, sizeof(a));
+    std::map<uint32_t, uint32_t> slot; // class -> tuple slot
+    uint32_t                     nslots = 0;
+    uint64_t                     nrows  = 0;
+    std::vector<uint32_t>        scan_eqs;
+    if (d.rel >= 0) {
+        a.driver = DRIVER_PAGES;
+        scan_eqs = q_.needed(1ull << d.rel);
+        nrows    = q_.rels[d.rel].rows;
+        if (scan_eqs.size() > size_t(kMaxScan)) throw std::runtime_error("too many scanned columns in a pipeline");
+        for (uint32_t e : scan_eqs) {
+            uint32_t attr = q_.attr_of(uint32_t(d.rel), e);
+            uint64_t col  = q_.attrs[attr].col;
+            a.scan[a.nscan++] = column(uint32_t(d.rel), col).view(col_id(uint32_t(d.rel), col));
+            slot[e]           = nslots++;
+        }
+    } else {
+        a.driver      = DRIVER_TUPLES;
+        a.tuples      = d.tuples->buf.get<uint64_t>();
+        a.tuple_words = uint32_t(d.tuples->eqs.size());
+        nrows         = d.tuples->n;
+        if (d.tuples->eqs.size() > size_t(kMaxScan)) throw std::runtime_error("too many columns in a pipeline");
+        for (uint32_t w = 0; w < d.tuples->eqs.size(); ++w) {
+            a.tuple_word[a.nscan++] = w;
+            slot[d.tuples->eqs[w]]  = nslots++;
+        }
+    }
+    uint32_t scanned = nslots;
+    a.nprobes        = uint32_t(probes.size());
+    for (uint32_t i = 0; i < probes.size(); ++i) {
+        const HashTable* ht = probes[i].ht;
+        ProbeSpec&       P  = a.probes[i];
+        auto             ks = slot.find(probes[i].key_eq);
+        if (ks == slot.end()) throw std::runtime_error("probe key is not available in the pipeline");
+        P.ht        = ht->slots.get<uint64_t>();
+        P.shift     = ht->shift;
+        P.words     = ht->words;
+        P.mask      = ht->cap - 1;
+        P.key_slot  = ks->second;
+        P.unique    = ht->unique ? 1 : 0;
+        P.bloom     = ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr;
+        P.bloom_log = ht->bloom_log;
+        P.early     = ks->second < scanned ? 1 : 0;
+        for (uint32_t w = 1; w < ht->words; ++w) {
+            uint32_t e  = ht->eqs[w];
+            auto     it = slot.find(e);
+            if (it != slot.end()) {
+                if (P.nchecks >= uint32_t(kMaxChecks)) throw std::runtime_error("too many join predicates");
+                P.check_word[P.nchecks] = w;
+                P.check_slot[P.nchecks] = it->second;
+                ++P.nchecks;
+            } else {
+                P.attr_word[P.nattrs]   = w;
+                P.attr_slot[P.nattrs++] = nslots;
+                slot[e]                 = nslots++;
+            }
+        }
+    }
+    if (nslots > 24) throw std::runtime_error("pipeline carries too many columns");
+    for (auto& kv : slot)
+        if (q_.is_join_eq(kv.first)) a.not_null |= 1u << kv.second;
+
+    a.target = t.kind;
+    DevResult* res = t.result;
+    double*    ms  = t.kind == TARGET_PAGES ? &ms_probe : &ms_build;
+    if (t.kind == TARGET_STAGE) {
+        a.stage_words = uint32_t(t.eqs.size());
+        for (uint32_t w = 0; w < t.eqs.size(); ++w) {
+            auto it = slot.find(t.eqs[w]);
+            if (it == slot.end()) throw std::runtime_error("staged class is not available in the pipeline");
+            a.stage_slot[w] = it->second;
+        }
+        uint64_t cap = probes.empty() ? nrows : uint64_t(std::min(std::max(t.est * 1.25, 1024.0), 4e9));
+        cap          = std::max<uint64_t>(cap, 1);
+        for (int attempt = 0;; ++attempt) {
+            t.stage      = DevBuf(&ctx_.heap(), cap * a.stage_words * sizeof(uint64_t));
+            a.stage      = t.stage.get<uint64_t>();
+            a.stage_cap  = cap;
+            launch(a, nslots, nrows, *ms);
+            PipeCounters c;
+            CGJ_HIP(hipMemcpy(&c, counters_.get(), sizeof(c), hipMemcpyDeviceToHost));
+            if (!c.overflow) {
+                t.count = c.tuples;
+                break;
+            }
+            if (attempt > 2) throw std::runtime_error("staging overflow persists");
+            ++retries;
+            cap = c.tuples + 1;
+        }
+        return;
+    }
+    if (t.kind == TARGET_COUNT) {
+        launch(a, nslots, nrows, *ms);
+        PipeCounters c;
+        CGJ_HIP(hipMemcpy(&c, counters_.get(), sizeof(c), hipMemcpyDeviceToHost));
+        t.count = c.tuples;
+        return;
+    }
+    // TARGET_PAGES
+    a.nout = uint32_t(q_.out.size());
+    if (a.nout > uint32_t(kMaxOut)) throw std::runtime_error("too many output columns");
+    double bytes_per_row = 0;
+    for (uint32_t i = 0; i < a.nout; ++i) {
+        uint32_t attr = q_.out[i];
+        auto     it   = slot.find(q_.attrs[attr].eq);
+        if (it == slot.end()) throw std::runtime_error("output class is not available in the pipeline");
+        a.out[i].slot = it->second;
+        a.out[i].type = q_.attrs[attr].type;
+        a.out[i].vc   = a.out[i].type == CGJ_VARCHAR ? a.nvarchar++ : 0;
+        bytes_per_row += a.out[i].type == CGJ_INT32 ? 4.2 : (a.out[i].type == CGJ_VARCHAR ? 48.0 : 8.2);
+    }
+    uint64_t waves   = std::min<uint64_t>(std::max<uint64_t>((nrows + 4095) / 4096, 1), uint64_t(ctx_.cu_count()) * 16);
+    waves            = (waves + 3) / 4 * 4;
+    DevBuf   stage_s = DevBuf(&ctx_.heap(), std::max<uint64_t>(waves * a.nvarchar * kPageSize, 64));
+    a.str_stage      = stage_s.get<uint8_t>();
+    uint64_t pages   = uint64_t(std::min(t.est, 4e9) * bytes_per_row / 7800.0) + waves * a.nout + 64;
+    for (int attempt = 0;; ++attempt) {
+        res->pool    = DevBuf(&ctx_.heap(), pages * kPageSize + pages * 16 + 64);
+        a.pool       = res->pool.get<uint8_t>();
+        a.pool_pages = pages;
+        a.dir        = reinterpret_cast<uint32_t*>(a.pool + pages * kPageSize);
+        a.dir_cap    = pages;
+        launch(a, nslots, nrows, *ms);
+        PipeCounters c;
+        CGJ_HIP(hipMemcpy(&c, counters_.get(), sizeof(c), hipMemcpyDeviceToHost));
+        if (!c.overflow) {
+            res->nrows       = c.tuples;
+            res->total_pages = c.dir;
+            break;
+        }
+        if (attempt > 2) throw std::runtime_error("output page pool overflow persists");
+        ++retries;
+        pages = std::max<uint64_t>(c.pages, c.dir) + 64;
+    }
+    probe_rows += nrows;
+    // order pages: by column, then wave stream, then sequence
+    std::vector<uint32_t> dir(res->total_pages * 4);
+    if (!dir.empty())
+        CGJ_HIP(hipMemcpy(dir.data(), a.dir, dir.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint64_t> order(res->total_pages);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
+        const uint32_t* a0 = &dir[4 * x];
+        const uint32_t* b0 = &dir[4 * y];
+        if (a0[0] != b0[0]) return a0[0] < b0[0];
+        if (a0[1] != b0[1]) return a0[1] < b0[1];
+        return a0[2] < b0[2];
+    });
+    res->pages.assign(a.nout, {});
+    for (uint64_t i : order) res->pages[dir[4 * i]].push_back(dir[4 * i + 3]);
+}
+
+std::unique_ptr<DevResult> Executor::run() {
+    auto res = std::make_unique<DevResult>();
+    for (uint32_t a : q_.out) res->types.push_back(q_.attrs[a].type);
+    res->pages.assign(q_.out.size(), {});
+    for (auto& r : q_.rels)
+        if (r.rows == 0) return res; // an empty input empties the inner join
+    register_all_columns();
+    Target t;
+    t.kind   = TARGET_PAGES;
+    t.est    = t_.nodes[t_.root].card;
+    t.result = res.get();
+    std::vector<int> joins;
+    int              leaf;
+    walk(t_.root, joins, leaf);
+    std::vector<ProbeStep> probes;
+    uint64_t               cur = 1ull << leaf;
+    for (int j : joins) {
+        const TreeNode& jn = t_.nodes[j];
+        const TreeNode& bn = t_.nodes[jn.build];
+        auto            sh = q_.shared(bn.set, cur);
+        if (sh.empty()) throw std::runtime_error("join without a predicate");
+        const HashTable* ht = build(jn.build, sh.front());
+        if (!ht) {
+            res->pool.reset();
+            return res; // empty build side
+        }
+        probes.push_back(ProbeStep{ht, sh.front(), bn.set});
+        cur |= bn.set;
+    }
+    Driver d;
+    d.rel = leaf;
+    d.set = 1ull << leaf;
+    pipeline(d, probes, t);
+    return res;
+}
+
+} // namespace
+
+std::unique_ptr<DevResult> Context::execute(const cgj_plan& plan, const std::vector<const DevTable*>& inputs) {
+    std::vector<uint64_t> rows;
+    std::vector<uint32_t> cols;
+    for (auto* t : inputs) {
+        rows.push_back(t->nrows);
+        cols.push_back(uint32_t(t->cols.size()));
+    }
+    Query               q = import_plan(plan, rows, cols);
+    std::vector<double> base;
+    for (auto& r : q.rels) base.push_back(double(r.rows));
+    PlanTree tree = plan_joins(q, max_card_model(q, base));
+    for (auto& r : q.rels)
+        for (uint32_t a : r.attrs)
+            if (r.rows && inputs[r.table]->cols[q.attrs[a].col].type != q.attrs[a].type)
+                throw std::runtime_error("plan column type differs from the input column type");
+    Executor ex(*this, q, tree, inputs);
+    auto     res = ex.run();
+    stats               = cgj_stats{};
+    stats.ms_build      = ex.ms_build;
+    stats.ms_probe      = ex.ms_probe;
+    stats.ms_total      = ex.ms_build + ex.ms_probe;
+    stats.build_tuples  = ex.build_tuples;
+    stats.probe_rows    = ex.probe_rows;
+    stats.output_rows   = res->nrows;
+    stats.output_pages  = res->total_pages;
+    stats.pipelines     = ex.launches;
+    stats.retries       = ex.retries;
+    uint64_t in_pages   = 0;
+    for (auto& r : q.rels)
+        for (uint32_t e : q.needed(1ull << (&r - &q.rels[0]))) {
+            uint32_t a = q.attr_of(uint32_t(&r - &q.rels[0]), e);
+            if (r.rows) in_pages += inputs[r.table]->cols[q.attrs[a].col].npages;
+        }
+    stats.input_pages = in_pages;
+    return res;
+}
+
+void Context::download(const DevResult& r, cgj_result& out) {
+    out.num_rows    = r.nrows;
+    out.num_columns = uint32_t(r.types.size());
+    out.columns     = new cgj_result_column[r.types.size()];
+    for (size_t c = 0; c < r.types.size(); ++c) {
+        out.columns[c].type      = r.types[c];
+        out.columns[c].num_pages = 0;
+        out.columns[c].pages     = nullptr;
+    }
+    if (r.total_pages == 0) return;
+    uint64_t used = 0;
+    for (auto& col : r.pages)
+        for (uint32_t p : col) used = std::max<uint64_t>(used, uint64_t(p) + 1);
+    const uint64_t kChunkPages = 8192; // 64 MiB staging
+    void*          pinned      = nullptr;
+    CGJ_HIP(hipHostMalloc(&pinned, kChunkPages * kPageSize, hipHostMallocDefault));
+    try {
+        for (size_t c = 0; c < r.types.size(); ++c) {
+            out.columns[c].num_pages = r.pages[c].size();
+            out.columns[c].pages     = new void*[r.pages[c].size() ? r.pages[c].size() : 1];
+            for (size_t i = 0; i < r.pages[c].size(); ++i) out.columns[c].pages[i] = nullptr;
+        }
+        // copy the pool in chunks and scatter pages to their columns
+        std::vector<std::pair<uint32_t, uint32_t>> where(used, {~0u, ~0u});
+        for (size_t c = 0; c < r.types.size(); ++c)
+            for (size_t i = 0; i < r.pages[c].size(); ++i) where[r.pages[c][i]] = {uint32_t(c), uint32_t(i)};
+        for (uint64_t p0 = 0; p0 < used; p0 += kChunkPages) {
+            uint64_t n = std::min(kChunkPages, used - p0);
+            CGJ_HIP(hipMemcpyAsync(pinned, r.pool.get<uint8_t>() + p0 * kPageSize, n * kPageSize,
+                                   hipMemcpyDeviceToHost, stream_));
+            CGJ_HIP(hipStreamSynchronize(stream_));
+            for (uint64_t i = 0; i < n; ++i) {
+                auto w = where[p0 + i];
+                if (w.first == ~0u) continue;
+                void* pg = ::operator new(kPageSize);
+                std::memcpy(pg, static_cast<uint8_t*>(pinned) + i * kPageSize, kPageSize);
+                out.columns[w.first].pages[w.second] = pg;
+            }
+        }
+    } catch (...) {
+        (void)hipHostFree(pinned);
+        throw;
+    }
+    CGJ_HIP(hipHostFree(pinned));
+}
+
+} // namespace cgj

@@ -1,0 +1,146 @@
+// Host side of the join pipeline: device memory, device tables, plan import,
+// join ordering and pipeline execution. Everything here is host C++; device
+// work goes through kernels.hpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gpujoin.h"
+#include "kernels.hpp"
+
+namespace cgj {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define CGJ_HIP(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::cgj::HipError(std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+// Size-bucketed caching allocator over hipMalloc: repeated queries of the same
+// shape reuse their buffers instead of paying for hipMalloc/hipFree.
+class DeviceHeap {
+public:
+    ~DeviceHeap();
+    void* alloc(size_t bytes);
+    void  free(void* p);
+    void  trim();
+
+private:
+    static size_t round(size_t bytes);
+    std::multimap<size_t, void*>      free_;
+    std::unordered_map<void*, size_t> live_;
+};
+
+// RAII device buffer from a DeviceHeap.
+class DevBuf {
+public:
+    DevBuf() = default;
+    DevBuf(DeviceHeap* h, size_t bytes) : heap_(h), bytes_(bytes), ptr_(bytes ? h->alloc(bytes) : nullptr) {}
+    ~DevBuf() { reset(); }
+    DevBuf(DevBuf&& o) noexcept : heap_(o.heap_), bytes_(o.bytes_), ptr_(o.ptr_) { o.ptr_ = nullptr; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            heap_  = o.heap_;
+            bytes_ = o.bytes_;
+            ptr_   = o.ptr_;
+            o.ptr_ = nullptr;
+        }
+        return *this;
+    }
+    DevBuf(const DevBuf&)            = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void reset() {
+        if (ptr_) heap_->free(ptr_);
+        ptr_ = nullptr;
+    }
+    template <class T = void>
+    T*     get() const { return static_cast<T*>(ptr_); }
+    size_t bytes() const { return bytes_; }
+
+private:
+    DeviceHeap* heap_  = nullptr;
+    size_t      bytes_ = 0;
+    void*       ptr_   = nullptr;
+};
+
+// One column of a device-resident table.
+struct DevColumn {
+    int32_t  type   = 0;
+    uint64_t npages = 0;
+    uint64_t nrows  = 0;
+    DevBuf   pages;     // npages x 8 KiB
+    DevBuf   row_start; // u32 [npages + 1]
+    DevBuf   idx_page;  // u32 [nrows / 1024 + 1]
+    DevBuf   idx_nn;    // u32 [nrows / 1024 + 1]
+
+    ScanCol view(uint32_t col_id) const;
+};
+
+struct DevTable {
+    uint64_t               nrows = 0;
+    std::vector<DevColumn> cols;
+};
+
+// Output pages of a query, still in HBM: `pool` holds the pages, `pages`
+// lists per column the pool page indices in row order.
+struct DevResult {
+    uint64_t                           nrows = 0;
+    std::vector<int32_t>               types;
+    std::vector<std::vector<uint32_t>> pages;
+    uint64_t                           total_pages = 0;
+    DevBuf                             pool;
+};
+
+class Context;
+
+// Tunables of one hash join, chosen per (build rows, probe rows, key width).
+struct JoinTuning {
+    uint32_t load_log   = 1; // capacity = next_pow2(build) << load_log
+    uint32_t bloom_bits = 0; // bloom bits per key (0 = no bloom filter)
+};
+
+class Context {
+public:
+    explicit Context(int device);
+    ~Context();
+
+    int         device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+    DeviceHeap& heap() { return heap_; }
+
+    // Uploads the table's pages; with `used`, only the flagged columns.
+    std::unique_ptr<DevTable> upload(const cgj_host_table& t, const std::vector<bool>* used = nullptr);
+    std::unique_ptr<DevTable> generate(uint64_t seed, uint64_t nrows, const cgj_gen_column* cols, uint32_t ncols);
+    void prepare_column(DevColumn& c);
+
+    std::unique_ptr<DevResult> execute(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
+    void download(const DevResult& r, cgj_result& out);
+
+    cgj_stats   stats{};
+    std::string error;
+
+    JoinTuning tune(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes) const;
+    uint32_t   cu_count() const { return cus_; }
+
+private:
+    int         device_;
+    uint32_t    cus_    = 256;
+    hipStream_t stream_ = nullptr;
+    DeviceHeap  heap_;
+};
+
+} // namespace cgj

@@ -1,0 +1,705 @@
+// HIP kernels of the join pipeline for gfx950 (wave64).
+//
+//  * column preparation: rows per page -> prefix (reference TableScan.cpp:546-582
+//    makeSearchVector) plus a morsel index (page and non-NULL offset of every
+//    1024th row) so that any wave can start decoding at any morsel;
+//  * page decode into 64-bit slots (operator entry point and tests);
+//  * hash-table insert: open addressing with linear probing over
+//    [key, attrs...] entries, plus a blocked bloom filter;
+//  * the fused pipeline: scan pages -> up to four probes -> target, where the
+//    target is a staged tuple array (hash-table build side), the contest output
+//    pages, or a tuple count.
+#include <hip/hip_runtime.h>
+
+#include "gen.hpp"
+#include "kernels.hpp"
+
+namespace cgj {
+
+// ---------------------------------------------------------------------------
+// exclusive scan of u32 counts (block-local scan, block sums, add-back)
+// ---------------------------------------------------------------------------
+constexpr int kScanBlock = 1024;
+
+__global__ void k_page_counts(const uint8_t* pages, uint64_t npages, int32_t type, uint32_t* counts) {
+    uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= npages) return;
+    uint32_t nr = *reinterpret_cast<const uint16_t*>(pages + p * kPageSize);
+    uint32_t c  = nr;
+    if (type == T_VARCHAR && nr >= 0xfffe) c = nr - 0xfffe; // long string: 1 row on its first page
+    counts[p] = c;
+}
+
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+    // blockDim.x == kScanBlock, 16 waves
+    uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    uint32_t inc  = wave_incl_scan(v);
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t x = lane < kScanBlock / 64 ? sh[lane] : 0;
+        uint32_t y = wave_incl_scan(x);
+        if (lane < kScanBlock / 64) sh[lane] = y - x;
+        if (lane == kScanBlock / 64 - 1) sh[16] = y;
+    }
+    __syncthreads();
+    uint32_t r = sh[w] + inc - v;
+    *total     = sh[16];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_local(uint32_t* data, uint64_t n, uint32_t* sums) {
+    __shared__ uint32_t sh[17];
+    uint64_t i = uint64_t(blockIdx.x) * kScanBlock + threadIdx.x;
+    uint32_t v = i < n ? data[i] : 0;
+    uint32_t tot;
+    uint32_t e = block_excl_scan(v, sh, &tot);
+    if (i < n) data[i] = e;
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// One block scans all block sums in chunks; writes the grand total.
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, uint64_t nblocks, uint64_t* total) {
+    __shared__ uint32_t sh[17];
+    uint32_t carry = 0;
+    for (uint64_t base = 0; base < nblocks; base += kScanBlock) {
+        uint64_t i = base + threadIdx.x;
+        uint32_t v = i < nblocks ? sums[i] : 0;
+        uint32_t tot;
+        uint32_t e = block_excl_scan(v, sh, &tot);
+        if (i < nblocks) sums[i] = e + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_scan_add(uint32_t* data, uint64_t n, const uint32_t* sums, const uint64_t* total,
+                           uint32_t* out, uint64_t out_n) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = data[i] + sums[i / kScanBlock];
+    if (i == 0) out[out_n] = uint32_t(*total);
+}
+
+// Morsel index: for every row k*kIndexRows, its page and the count of non-NULL
+// values before it in that page.
+__global__ void k_morsel_index(const uint8_t* pages, uint64_t npages, int32_t type, const uint32_t* row_start,
+                               uint32_t* idx_page, uint32_t* idx_nn) {
+    uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= npages) return;
+    uint32_t r0 = row_start[p], r1 = row_start[p + 1];
+    if (r0 == r1) return;
+    const uint8_t* page = pages + p * kPageSize;
+    uint32_t       nr   = *reinterpret_cast<const uint16_t*>(page);
+    uint32_t       nv   = *reinterpret_cast<const uint16_t*>(page + 2);
+    bool           dense = (type == T_VARCHAR && nr >= 0xfffe) || nv == nr;
+    uint32_t       k     = (r0 + kIndexRows - 1) / kIndexRows;
+    for (uint32_t row = k * kIndexRows; row < r1; row += kIndexRows, ++k) {
+        uint32_t j  = row - r0;
+        uint32_t nn = j;
+        if (!dense) {
+            const uint8_t* bm = page + kPageSize - (nr + 7) / 8;
+            nn                = 0;
+            for (uint32_t b = 0; b < j / 8; ++b) nn += __popc(bm[b]);
+            if (j & 7) nn += __popc(bm[j / 8] & ((1u << (j & 7)) - 1));
+        }
+        idx_page[k] = uint32_t(p);
+        idx_nn[k]   = nn;
+    }
+}
+
+size_t prepare_scratch_words(uint64_t npages) { return npages + (npages + kScanBlock - 1) / kScanBlock + 2; }
+
+hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t type, uint32_t* row_start,
+                                 uint32_t* idx_page, uint32_t* idx_nn, uint32_t* scratch,
+                                 uint64_t* total_rows_dev, hipStream_t s) {
+    if (npages == 0) return hipSuccess;
+    uint32_t* counts  = scratch;
+    uint32_t* sums    = scratch + npages;
+    uint64_t  nblocks = (npages + kScanBlock - 1) / kScanBlock;
+    k_page_counts<<<dim3((npages + 255) / 256), dim3(256), 0, s>>>(pages, npages, type, counts);
+    k_scan_local<<<dim3(nblocks), dim3(kScanBlock), 0, s>>>(counts, npages, sums);
+    k_scan_sums<<<dim3(1), dim3(kScanBlock), 0, s>>>(sums, nblocks, total_rows_dev);
+    k_scan_add<<<dim3((npages + 255) / 256), dim3(256), 0, s>>>(counts, npages, sums, total_rows_dev, row_start,
+                                                                npages);
+    k_morsel_index<<<dim3((npages + 255) / 256), dim3(256), 0, s>>>(pages, npages, type, row_start, idx_page,
+                                                                   idx_nn);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// standalone decode: one wave per 1024-row morsel
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_decode(ScanCol c, uint64_t nrows, uint64_t* out) {
+    uint64_t wave   = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+    uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / 64;
+    uint64_t nm     = (nrows + kIndexRows - 1) / kIndexRows;
+    for (uint64_t m = wave; m < nm; m += nwaves) {
+        uint64_t r0  = m * kIndexRows;
+        uint64_t end = r0 + kIndexRows < nrows ? r0 + kIndexRows : nrows;
+        Cursor   cur = cursor_at(c, r0);
+        for (uint64_t r = r0; r < end; r += 64) {
+            uint64_t row    = r + lane_id();
+            bool     active = row < end;
+            uint64_t v      = decode_step(c, cur, uint32_t(row), active);
+            if (active) out[row] = v;
+        }
+    }
+}
+
+hipError_t launch_decode(const ScanCol& c, uint64_t nrows, uint64_t* out, hipStream_t s) {
+    if (nrows == 0) return hipSuccess;
+    uint64_t nm     = (nrows + kIndexRows - 1) / kIndexRows;
+    uint64_t blocks = (nm + 3) / 4;
+    if (blocks > 8192) blocks = 8192;
+    k_decode<<<dim3(blocks), dim3(256), 0, s>>>(c, nrows, out);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+    uint64_t i      = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    k_fill_u64<<<dim3(blocks), dim3(256), 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// hash-table insert (open addressing, linear probing, entries [key, attrs...])
+// ---------------------------------------------------------------------------
+__global__ void k_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,
+                            uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* dup_flag) {
+    uint64_t i      = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) {
+        const uint64_t* t   = stage + i * words;
+        uint64_t        key = t[0];
+        uint64_t        h   = hash_key(key);
+        uint64_t        pos = h >> shift;
+        bool            dup = false;
+        for (;;) {
+            unsigned long long prev =
+                atomicCAS(reinterpret_cast<unsigned long long*>(ht + pos * words), kEmpty, key);
+            if (prev == kEmpty) break;
+            dup |= prev == key;
+            pos = (pos + 1) & mask;
+        }
+        for (uint32_t w = 1; w < words; ++w) ht[pos * words + w] = t[w];
+        if (bloom) {
+            uint64_t h2 = hash_bloom(h);
+            atomicOr(reinterpret_cast<unsigned long long*>(bloom + bloom_word(h2, bloom_log)), bloom_bits(h2));
+        }
+        if (dup) *dup_flag = 1;
+    }
+}
+
+hipError_t launch_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,
+                            uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* dup_flag,
+                            hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 32768) blocks = 32768;
+    k_ht_insert<<<dim3(blocks), dim3(256), 0, s>>>(stage, n, words, ht, shift, mask, bloom, bloom_log, dup_flag);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// fused pipeline
+// ---------------------------------------------------------------------------
+constexpr uint32_t kWavesPerBlock = 4;
+uint32_t           pipeline_waves_per_block() { return kWavesPerBlock; }
+
+template <int NA>
+__device__ inline uint64_t sget(const uint64_t (&s)[NA], uint32_t i) {
+    uint64_t r = 0;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) r = (uint32_t(k) == i) ? s[k] : r;
+    return r;
+}
+template <int NA>
+__device__ inline void sset(uint64_t (&s)[NA], uint32_t i, uint64_t v) {
+#pragma unroll
+    for (int k = 0; k < NA; ++k) s[k] = (uint32_t(k) == i) ? v : s[k];
+}
+
+// Per-wave state of one output column's open page (wave-uniform values).
+struct OutState {
+    uint32_t page;  // pool index of the open page, kNoPage if none
+    uint32_t nr;    // rows in the page
+    uint32_t nv;    // non-NULL values in the page
+    int32_t  avail; // remaining bit budget
+    uint32_t chars; // VARCHAR: characters staged for the page
+    uint32_t seq;   // pages emitted by this wave for this column
+    uint32_t bad;   // pool exhausted: keep counting, write nothing
+    uint32_t pad;
+};
+constexpr uint32_t kNoPage = 0xffffffffu;
+
+__device__ inline int32_t fixed_budget(int32_t type) { return 8 * (int32_t(kPageSize) - 4 - (type == T_INT32 ? 0 : 4)); }
+
+__device__ inline void page_open(const PipeArgs& a, OutState& st, uint64_t* bits, int32_t budget) {
+    uint32_t idx = 0;
+    if (lane_id() == 0) idx = uint32_t(atomicAdd(&a.counters->pages, 1ull));
+    idx      = __shfl(idx, 0, 64);
+    st.bad   = idx >= a.pool_pages ? 1u : 0u;
+    if (st.bad && lane_id() == 0) atomicExch(&a.counters->overflow, 1ull);
+    st.page  = idx;
+    st.nr    = 0;
+    st.nv    = 0;
+    st.chars = 0;
+    st.avail = budget;
+    bits[lane_id()] = 0;
+}
+
+// Writes header, (staged characters) and bitmap of the open page and records it in the directory.
+__device__ inline void page_close(const PipeArgs& a, OutState& st, uint64_t* bits, uint32_t col, uint32_t stream,
+                                  const uint8_t* chars) {
+    if (!st.bad) {
+        uint8_t* page = a.pool + uint64_t(st.page) * kPageSize;
+        if (chars) {
+            uint8_t* dst = page + 4 + 2 * st.nv;
+            for (uint32_t i = lane_id(); i < st.chars; i += 64) dst[i] = chars[i];
+        }
+        uint32_t nb   = (st.nr + 7) / 8;
+        uint8_t* bm   = page + kPageSize - nb;
+        uint64_t word = bits[lane_id()];
+        uint32_t b0   = lane_id() * 8;
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b)
+            if (b0 + b < nb) bm[b0 + b] = uint8_t(word >> (8 * b));
+        if (lane_id() == 0) {
+            *reinterpret_cast<uint32_t*>(page) = st.nr | (st.nv << 16);
+            unsigned long long d = atomicAdd(&a.counters->dir, 1ull);
+            if (d < a.dir_cap) {
+                uint32_t* e = a.dir + 4 * d;
+                e[0]        = col;
+                e[1]        = stream;
+                e[2]        = st.seq;
+                e[3]        = st.page;
+            } else {
+                atomicExch(&a.counters->overflow, 1ull);
+            }
+        }
+    }
+    st.seq++;
+    st.page = kNoPage;
+}
+
+// Appends the rows of lanes with `have` (rank = position in the batch) to a
+// fixed-width column. Page packing follows the reference's bit budget
+// (TableTarget.cpp:111-182): each row costs 1 bit, a non-NULL row the value
+// width on top; the values are written straight to the page, the bitmap is
+// held by the lanes (lane i holds rows 64i..64i+63) until the page closes.
+__device__ void append_fixed(const PipeArgs& a, OutState* stp, uint64_t* bits, int32_t type, uint64_t v,
+                             bool have, uint32_t rank, uint32_t cnt, uint32_t col, uint32_t stream) {
+    const uint32_t w      = type == T_INT32 ? 4 : 8;
+    const uint32_t off    = type == T_INT32 ? 4 : 8;
+    const int32_t  budget = fixed_budget(type);
+    const bool     valid  = have && v != kNull;
+    uint32_t       start  = 0;
+    OutState       st     = *stp;
+    while (start < cnt) {
+        if (st.page == kNoPage) page_open(a, st, bits, budget);
+        bool     part = have && rank >= start;
+        uint32_t cost = part ? 1u + (valid ? w * 8 : 0u) : 0u;
+        uint32_t cum  = wave_incl_scan(cost);
+        bool     fits = part && int32_t(cum) <= st.avail && (st.nr + (rank - start)) < kMaxPageRows;
+        uint64_t fm   = __ballot(fits);
+        uint32_t k    = __popcll(fm);
+        uint64_t vm   = __ballot(fits && valid);
+        if (fits && valid && !st.bad) {
+            uint32_t vr  = __popcll(vm & mask_lt());
+            uint8_t* dst = a.pool + uint64_t(st.page) * kPageSize + off + uint64_t(st.nv + vr) * w;
+            if (w == 4)
+                *reinterpret_cast<uint32_t*>(dst) = uint32_t(v);
+            else
+                *reinterpret_cast<uint64_t*>(dst) = v;
+        }
+        uint64_t pat = wave_or((fits && valid) ? (1ull << (rank - start)) : 0ull);
+        uint32_t wi = st.nr >> 6, sh = st.nr & 63;
+        if (lane_id() == wi) bits[wi] |= pat << sh;
+        if (sh && lane_id() == wi + 1) bits[wi + 1] |= pat >> (64 - sh);
+        uint32_t used = k ? __shfl(cum, highest_lane(fm), 64) : 0;
+        st.nr += k;
+        st.nv += __popcll(vm);
+        st.avail -= int32_t(used);
+        start += k;
+        if (start < cnt || st.nr >= kMaxPageRows) page_close(a, st, bits, col, stream, nullptr);
+    }
+    *stp = st;
+}
+
+// Resolves a string handle to its bytes (short strings) or flags a long string.
+__device__ inline const uint8_t* resolve_string(const PipeArgs& a, uint64_t h, uint32_t& len, bool& is_long) {
+    const uint8_t* page = a.col_base[handle_col(h)] + handle_page(h) * kPageSize;
+    uint32_t       idx  = handle_idx(h);
+    if (idx == 0xffff) {
+        is_long = true;
+        len     = 0;
+        return page;
+    }
+    is_long         = false;
+    uint32_t nv     = *reinterpret_cast<const uint16_t*>(page + 2);
+    const uint16_t* offs  = reinterpret_cast<const uint16_t*>(page + 4);
+    uint32_t        end   = offs[idx];
+    uint32_t        begin = idx ? offs[idx - 1] : 0;
+    len                   = end - begin;
+    return page + 4 + 2 * nv + begin;
+}
+
+// VARCHAR counterpart of append_fixed (reference TableTarget.cpp:183-391):
+// NULL costs 1 bit, a short string 17 + 8*len bits; the end offsets go
+// straight to the page, the characters to the wave's 8 KiB staging area until
+// the page closes; a long string closes the page and is copied page by page.
+__device__ void append_varchar(const PipeArgs& a, OutState* stp, uint64_t* bits, uint8_t* stage, uint64_t v,
+                               bool have, uint32_t rank, uint32_t cnt, uint32_t col, uint32_t stream) {
+    const int32_t budget = 8 * (int32_t(kPageSize) - 4);
+    const bool    valid  = have && v != kNull;
+    uint32_t      len    = 0;
+    bool          is_long = false;
+    const uint8_t* src   = nullptr;
+    if (valid) src = resolve_string(a, v, len, is_long);
+    OutState st    = *stp;
+    uint32_t start = 0;
+    while (start < cnt) {
+        uint64_t lm      = __ballot(have && is_long && rank >= start);
+        uint32_t seg_end = cnt;
+        if (lm) seg_end = __shfl(rank, __ffsll((unsigned long long)lm) - 1, 64);
+        while (start < seg_end) {
+            if (st.page == kNoPage) page_open(a, st, bits, budget);
+            bool     part = have && rank >= start && rank < seg_end;
+            uint32_t cost = part ? (valid ? 17u + 8u * len : 1u) : 0u;
+            uint32_t cum  = wave_incl_scan(cost);
+            bool     fits = part && int32_t(cum) <= st.avail && (st.nr + (rank - start)) < kMaxPageRows;
+            uint64_t fm   = __ballot(fits);
+            uint32_t k    = __popcll(fm);
+            uint64_t vm   = __ballot(fits && valid);
+            uint32_t clen = (fits && valid) ? len : 0u;
+            uint32_t cinc = wave_incl_scan(clen);
+            if (fits && valid && !st.bad) {
+                uint32_t vr  = __popcll(vm & mask_lt());
+                uint8_t* pg  = a.pool + uint64_t(st.page) * kPageSize;
+                uint32_t endoff = st.chars + cinc;
+                *reinterpret_cast<uint16_t*>(pg + 4 + 2 * (st.nv + vr)) = uint16_t(endoff);
+                uint8_t* dst = stage + (endoff - len);
+                for (uint32_t i = 0; i < len; ++i) dst[i] = src[i];
+            }
+            uint64_t pat = wave_or((fits && valid) ? (1ull << (rank - start)) : 0ull);
+            uint32_t wi = st.nr >> 6, sh = st.nr & 63;
+            if (lane_id() == wi) bits[wi] |= pat << sh;
+            if (sh && lane_id() == wi + 1) bits[wi + 1] |= pat >> (64 - sh);
+            uint32_t used  = k ? __shfl(cum, highest_lane(fm), 64) : 0;
+            uint32_t added = k ? __shfl(cinc, highest_lane(fm), 64) : 0;
+            st.nr += k;
+            st.nv += __popcll(vm);
+            st.avail -= int32_t(used);
+            st.chars += added;
+            start += k;
+            if (start < seg_end || st.nr >= kMaxPageRows) page_close(a, st, bits, col, stream, stage);
+        }
+        if (start < cnt) {
+            // the row of rank `start` is a long string: copy its pages
+            if (st.page != kNoPage) page_close(a, st, bits, col, stream, stage);
+            uint32_t       src_lane = __ffsll((unsigned long long)lm) - 1;
+            uint64_t       h        = __shfl(v, src_lane, 64);
+            uint32_t       cid      = handle_col(h);
+            uint64_t       p0       = handle_page(h);
+            const uint8_t* base     = a.col_base[cid];
+            uint32_t       np       = a.col_npages[cid];
+            uint64_t       p        = p0;
+            do {
+                page_open(a, st, bits, budget);
+                if (!st.bad) {
+                    const uint64_t* s64 = reinterpret_cast<const uint64_t*>(base + p * kPageSize);
+                    uint64_t*       d64 = reinterpret_cast<uint64_t*>(a.pool + uint64_t(st.page) * kPageSize);
+                    for (uint32_t i = lane_id(); i < kPageSize / 8; i += 64) d64[i] = s64[i];
+                }
+                // record the copied page as-is (header already in place)
+                if (!st.bad && lane_id() == 0) {
+                    unsigned long long d = atomicAdd(&a.counters->dir, 1ull);
+                    if (d < a.dir_cap) {
+                        uint32_t* e = a.dir + 4 * d;
+                        e[0]        = col;
+                        e[1]        = stream;
+                        e[2]        = st.seq;
+                        e[3]        = st.page;
+                    } else {
+                        atomicExch(&a.counters->overflow, 1ull);
+                    }
+                }
+                st.seq++;
+                st.page = kNoPage;
+                ++p;
+            } while (p < np && *reinterpret_cast<const uint16_t*>(base + p * kPageSize) == 0xfffe);
+            start += 1;
+        }
+    }
+    *stp = st;
+}
+
+template <int NA>
+__global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
+    extern __shared__ uint64_t lds[];
+    const uint32_t wib    = threadIdx.x / 64;
+    const uint32_t stream = blockIdx.x * kWavesPerBlock + wib;
+    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+    const uint32_t lane   = lane_id();
+
+    // per-wave output state: kMaxOut OutState (4 u64 each) + nout x 64 bitmap words
+    OutState* ost   = reinterpret_cast<OutState*>(lds + wib * (kMaxOut * 4 + a.nout * 64));
+    uint64_t* obits = lds + wib * (kMaxOut * 4 + a.nout * 64) + kMaxOut * 4;
+    if (a.target == TARGET_PAGES && lane < a.nout) {
+        OutState s{kNoPage, 0, 0, 0, 0, 0, 0, 0};
+        ost[lane] = s;
+    }
+
+    uint64_t local_count = 0;
+
+    for (uint64_t m = stream; m < a.nmorsels; m += nwaves) {
+        uint64_t r0  = m * a.morsel_rows;
+        uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
+        Cursor   cur[kMaxScan];
+        if (a.driver == DRIVER_PAGES) {
+#pragma unroll
+            for (int c = 0; c < kMaxScan; ++c)
+                if (uint32_t(c) < a.nscan && c < NA) cur[c] = cursor_at(a.scan[c], r0);
+        }
+        for (uint64_t rb = r0; rb < end; rb += 64) {
+            uint64_t row    = rb + lane;
+            bool     active = row < end;
+            uint64_t s[NA];
+#pragma unroll
+            for (int k = 0; k < NA; ++k) s[k] = kNull;
+            if (a.driver == DRIVER_PAGES) {
+#pragma unroll
+                for (int c = 0; c < kMaxScan; ++c)
+                    if (uint32_t(c) < a.nscan && c < NA) s[c] = decode_step(a.scan[c], cur[c], uint32_t(row), active);
+            } else if (active) {
+#pragma unroll
+                for (int c = 0; c < kMaxScan; ++c)
+                    if (uint32_t(c) < a.nscan && c < NA) s[c] = a.tuples[row * a.tuple_words + a.tuple_word[c]];
+            }
+            bool alive = active;
+#pragma unroll
+            for (int k = 0; k < NA; ++k)
+                if ((a.not_null >> k) & 1u) alive = alive && s[k] != kNull;
+            // bloom cascade on keys that come straight from the scan
+            for (uint32_t p = 0; p < a.nprobes; ++p) {
+                const ProbeSpec& P = a.probes[p];
+                if (P.bloom && P.early && alive) {
+                    uint64_t h2 = hash_bloom(hash_key(sget(s, P.key_slot)));
+                    uint64_t bb = bloom_bits(h2);
+                    alive       = (P.bloom[bloom_word(h2, P.bloom_log)] & bb) == bb;
+                }
+            }
+
+            // depth-first enumeration of probe matches, one tuple per lane per round
+            uint32_t level = 0;
+            uint64_t pos[kMaxProbes];
+            uint64_t pkey[kMaxProbes];
+#pragma unroll
+            for (int k = 0; k < kMaxProbes; ++k) {
+                pos[k]  = ~0ull;
+                pkey[k] = kNull;
+            }
+            if (alive && a.nprobes > 0) {
+                pkey[0] = sget(s, a.probes[0].key_slot);
+                pos[0]  = hash_key(pkey[0]) >> a.probes[0].shift;
+            }
+            for (;;) {
+                bool have = false;
+                if (a.nprobes == 0) {
+                    have  = alive;
+                    alive = false;
+                } else {
+                    while (alive && !have) {
+#pragma unroll
+                        for (int k = 0; k < kMaxProbes; ++k) {
+                            if (uint32_t(k) != level || uint32_t(k) >= a.nprobes) continue;
+                            const ProbeSpec& P     = a.probes[k];
+                            bool             found = false;
+                            uint64_t         p     = pos[k];
+                            if (p != ~0ull) {
+                                for (;;) {
+                                    const uint64_t* e  = P.ht + p * P.words;
+                                    uint64_t        kw = e[0];
+                                    if (kw == kEmpty) break;
+                                    if (kw == pkey[k]) {
+                                        bool ok = true;
+                                        for (uint32_t c = 0; c < P.nchecks; ++c)
+                                            ok = ok && e[P.check_word[c]] == sget(s, P.check_slot[c]);
+                                        if (ok) {
+                                            found = true;
+                                            break;
+                                        }
+                                    }
+                                    p = (p + 1) & P.mask;
+                                }
+                            }
+                            if (found) {
+                                const uint64_t* e = P.ht + p * P.words;
+                                for (uint32_t t = 0; t < P.nattrs; ++t) sset(s, P.attr_slot[t], e[P.attr_word[t]]);
+                                pos[k] = P.unique ? ~0ull : ((p + 1) & P.mask);
+                                if (uint32_t(k) + 1 == a.nprobes) {
+                                    have = true;
+                                } else if (k + 1 < kMaxProbes) {
+                                    const ProbeSpec& Q  = a.probes[k + 1];
+                                    uint64_t         nk = sget(s, Q.key_slot);
+                                    bool pass = nk != kNull;
+                                    if (pass && Q.bloom) {
+                                        uint64_t h2 = hash_bloom(hash_key(nk));
+                                        uint64_t bb = bloom_bits(h2);
+                                        pass        = (Q.bloom[bloom_word(h2, Q.bloom_log)] & bb) == bb;
+                                    }
+                                    pkey[k + 1] = nk;
+                                    pos[k + 1]  = pass ? (hash_key(nk) >> Q.shift) : ~0ull;
+                                    level       = k + 1;
+                                }
+                            } else {
+                                pos[k] = ~0ull;
+                                if (k == 0)
+                                    alive = false;
+                                else
+                                    level = k - 1;
+                            }
+                        }
+                    }
+                }
+                uint64_t hm = __ballot(have);
+                if (hm) {
+                    uint32_t cnt  = __popcll(hm);
+                    uint32_t rank = __popcll(hm & mask_lt());
+                    if (a.target == TARGET_STAGE) {
+                        unsigned long long base = 0;
+                        if (lane == 0) base = atomicAdd(&a.counters->tuples, (unsigned long long)cnt);
+                        base = __shfl(base, 0, 64);
+                        if (base + cnt > a.stage_cap) {
+                            if (lane == 0) atomicExch(&a.counters->overflow, 1ull);
+                        } else if (have) {
+                            uint64_t* dst = a.stage + (base + rank) * a.stage_words;
+                            for (uint32_t w = 0; w < a.stage_words; ++w) dst[w] = sget(s, a.stage_slot[w]);
+                        }
+                    } else if (a.target == TARGET_COUNT) {
+                        local_count += cnt;
+                    } else {
+                        if (lane == 0) atomicAdd(&a.counters->tuples, (unsigned long long)cnt);
+                        for (uint32_t c = 0; c < a.nout; ++c) {
+                            uint64_t v = sget(s, a.out[c].slot);
+                            if (a.out[c].type == T_VARCHAR) {
+                                uint8_t* stg = a.str_stage + (uint64_t(stream) * a.nvarchar + a.out[c].vc) * kPageSize;
+                                append_varchar(a, &ost[c], obits + c * 64, stg, v, have, rank, cnt, c, stream);
+                            } else {
+                                append_fixed(a, &ost[c], obits + c * 64, a.out[c].type, v, have, rank, cnt, c,
+                                             stream);
+                            }
+                        }
+                    }
+                }
+                if (!__ballot(alive)) break;
+            }
+        }
+    }
+
+    if (a.target == TARGET_COUNT) {
+        if (lane == 0 && local_count) atomicAdd(&a.counters->tuples, (unsigned long long)local_count);
+    } else if (a.target == TARGET_PAGES) {
+        for (uint32_t c = 0; c < a.nout; ++c) {
+            OutState st = ost[c];
+            if (st.page != kNoPage && st.nr > 0) {
+                const uint8_t* stg = nullptr;
+                if (a.out[c].type == T_VARCHAR)
+                    stg = a.str_stage + (uint64_t(stream) * a.nvarchar + a.out[c].vc) * kPageSize;
+                page_close(a, st, obits + c * 64, c, stream, stg);
+                ost[c] = st;
+            }
+        }
+    }
+}
+
+hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s) {
+    uint32_t blocks = (grid_waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks == 0) blocks = 1;
+    size_t lds = size_t(kWavesPerBlock) * (kMaxOut * 4 + a.nout * 64) * sizeof(uint64_t);
+    if (slots <= 4)
+        k_pipeline<4><<<dim3(blocks), dim3(256), lds, s>>>(a);
+    else if (slots <= 8)
+        k_pipeline<8><<<dim3(blocks), dim3(256), lds, s>>>(a);
+    else if (slots <= 16)
+        k_pipeline<16><<<dim3(blocks), dim3(256), lds, s>>>(a);
+    else if (slots <= 24)
+        k_pipeline<24><<<dim3(blocks), dim3(256), lds, s>>>(a);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// synthetic page generation: one block per page
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gen_fill(GenSpec g, const uint32_t* row_start, uint8_t* pages) {
+    __shared__ uint32_t sh[17];
+    __shared__ uint64_t bm[1024]; // bitmap of up to 65536 rows
+    uint64_t p   = blockIdx.x;
+    uint32_t r0  = row_start[p], r1 = row_start[p + 1];
+    uint32_t n   = r1 - r0;
+    uint8_t* page = pages + p * kPageSize;
+    uint32_t w    = g.type == T_INT32 ? 4 : 8;
+    uint32_t off  = g.type == T_INT32 ? 4 : 8;
+    for (uint32_t i = threadIdx.x; i < 1024; i += 256) bm[i] = 0;
+    __syncthreads();
+    uint32_t nv = 0; // running non-NULL count (uniform)
+    for (uint32_t c0 = 0; c0 < n; c0 += 256) {
+        uint32_t j     = c0 + threadIdx.x;
+        bool     in    = j < n;
+        uint64_t r     = uint64_t(r0) + j;
+        bool     valid = in && !gen_is_null(g.seed, g.null_stream, g.null_frac, r);
+        // block prefix of valid flags (4 waves)
+        uint32_t lane = lane_id(), wv = threadIdx.x / 64;
+        uint64_t b    = __ballot(valid);
+        if (lane == 0) {
+            sh[wv] = __popcll(b);
+            if (in) bm[j / 64] = b; // j is a multiple of 64 for lane 0
+        }
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t k = 0; k < 4; ++k) {
+            if (k < wv) before += sh[k];
+            total += sh[k];
+        }
+        uint32_t idx = nv + before + __popcll(b & mask_lt());
+        if (valid) {
+            uint64_t v = gen_value(g.type, g.kind, g.seed, g.stream, g.domain, g.zipf_s, g.miss_frac, g.lo, g.hi, r);
+            if (w == 4)
+                *reinterpret_cast<uint32_t*>(page + off + 4 * idx) = uint32_t(v);
+            else
+                *reinterpret_cast<uint64_t*>(page + off + 8 * uint64_t(idx)) = v;
+        }
+        nv += total;
+        __syncthreads();
+    }
+    __syncthreads();
+    __threadfence_block();
+    // bitmap last (the contest writer copies it over the page tail after the values)
+    uint32_t nb = (n + 7) / 8;
+    uint8_t* dst = page + kPageSize - nb;
+    for (uint32_t i = threadIdx.x; i < nb; i += 256) dst[i] = uint8_t(bm[i / 8] >> (8 * (i % 8)));
+    if (threadIdx.x == 0) {
+        *reinterpret_cast<uint16_t*>(page)     = uint16_t(n);
+        *reinterpret_cast<uint16_t*>(page + 2) = uint16_t(nv);
+    }
+}
+
+hipError_t launch_gen_fill(const GenSpec& g, const uint32_t* row_start, uint64_t npages, uint8_t* pages,
+                           hipStream_t s) {
+    if (npages == 0) return hipSuccess;
+    k_gen_fill<<<dim3(npages), dim3(256), 0, s>>>(g, row_start, pages);
+    return hipGetLastError();
+}
+
+} // namespace cgj

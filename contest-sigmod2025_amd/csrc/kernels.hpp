@@ -1,0 +1,118 @@
+// Host-visible launch interface of the HIP kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+
+namespace cgj {
+
+constexpr int kMaxScan   = 16; // scanned columns per pipeline
+constexpr int kMaxProbes = 4;  // hash-table probes fused into one pipeline
+constexpr int kMaxChecks = 4;  // residual equalities per probe
+constexpr int kMaxOut    = 16; // output columns
+constexpr int kMaxWords  = 16; // words per hash-table entry / staged tuple
+
+struct ProbeSpec {
+    const uint64_t* ht;        // cap x words entries; word 0 is the key
+    uint32_t        shift;     // slot = hash_key(key) >> shift
+    uint32_t        words;     // words per entry
+    uint64_t        mask;      // cap - 1
+    uint32_t        key_slot;  // tuple slot holding the probe key
+    uint32_t        unique;    // keys are unique: stop at the first match
+    uint32_t        nattrs;    // entry words copied into tuple slots
+    uint32_t        attr_word[kMaxWords];
+    uint32_t        attr_slot[kMaxWords];
+    uint32_t        nchecks;   // residual equalities: entry word == tuple slot
+    uint32_t        check_word[kMaxChecks];
+    uint32_t        check_slot[kMaxChecks];
+    const uint64_t* bloom;     // blocked bloom filter (nullptr = none)
+    uint32_t        bloom_log; // log2(words)
+    uint32_t        early;     // key is a scanned column: filter before probing
+};
+
+struct OutSpec {
+    uint32_t slot;
+    int32_t  type;
+    uint32_t vc;   // index among VARCHAR output columns (staging area)
+    uint32_t pad;
+};
+
+enum : uint32_t { TARGET_STAGE = 0, TARGET_PAGES = 1, TARGET_COUNT = 2 };
+enum : uint32_t { DRIVER_PAGES = 0, DRIVER_TUPLES = 1 };
+
+// Counters written by a pipeline launch (device memory).
+struct PipeCounters {
+    unsigned long long tuples;    // staged tuples / output rows / counted tuples
+    unsigned long long pages;     // output pages allocated from the pool
+    unsigned long long dir;       // directory entries written
+    unsigned long long overflow;  // nonzero: a buffer was too small, re-run
+};
+
+struct PipeArgs {
+    // driver
+    uint32_t        driver;
+    uint32_t        nscan;
+    uint64_t        nrows;
+    uint32_t        morsel_rows; // multiple of kIndexRows
+    uint32_t        not_null;    // bitmask of tuple slots that must be non-NULL
+    uint64_t        nmorsels;
+    ScanCol         scan[kMaxScan];
+    const uint64_t* tuples;      // DRIVER_TUPLES: rows x tuple_words
+    uint32_t        tuple_words;
+    uint32_t        tuple_word[kMaxScan];
+    // probes
+    uint32_t        nprobes;
+    ProbeSpec       probes[kMaxProbes];
+    // target
+    uint32_t        target;
+    uint32_t        stage_words;
+    uint32_t        stage_slot[kMaxWords];
+    uint64_t*       stage;
+    uint64_t        stage_cap;   // tuples
+    uint32_t        nout;
+    uint32_t        nvarchar;
+    OutSpec         out[kMaxOut];
+    uint8_t*        pool;        // output page pool
+    uint64_t        pool_pages;
+    uint32_t*       dir;         // 4 x u32 per page: column, stream, seq, page index
+    uint64_t        dir_cap;
+    uint8_t*        str_stage;   // per wave x VARCHAR column: 8 KiB
+    const uint8_t* const* col_base;  // pages of column id (string handles)
+    const uint32_t* col_npages;
+    PipeCounters*   counters;
+};
+
+// Column preparation: rows per page prefix and the morsel index.
+hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t type, uint32_t* row_start,
+                                 uint32_t* idx_page, uint32_t* idx_nn, uint32_t* scratch,
+                                 uint64_t* total_rows_dev, hipStream_t s);
+size_t     prepare_scratch_words(uint64_t npages);
+
+hipError_t launch_decode(const ScanCol& c, uint64_t nrows, uint64_t* out, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,
+                            uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* dup_flag,
+                            hipStream_t s);
+// Launches the fused pipeline. `slots` = tuple width (selects the kernel variant).
+hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s);
+uint32_t   pipeline_waves_per_block();
+
+// Synthetic generators.
+struct GenSpec {
+    int32_t  type;
+    int32_t  kind;
+    uint64_t seed;
+    uint64_t stream;
+    uint64_t null_stream;
+    double   null_frac;
+    uint64_t domain;
+    double   zipf_s;
+    double   miss_frac;
+    uint64_t lo, hi;
+};
+hipError_t launch_gen_fill(const GenSpec& g, const uint32_t* row_start, uint64_t npages, uint8_t* pages,
+                           hipStream_t s);
+
+} // namespace cgj

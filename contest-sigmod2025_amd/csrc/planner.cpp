@@ -1,0 +1,332 @@
+// Plan import, join ordering and join tuning (host).
+#include "planner.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+
+#include "engine.hpp"
+
+namespace cgj {
+
+std::set<uint32_t> Query::produced(uint64_t relset) const {
+    std::set<uint32_t> s;
+    for (uint32_t r = 0; r < rels.size(); ++r)
+        if ((relset >> r) & 1) s.insert(rels[r].eqs.begin(), rels[r].eqs.end());
+    return s;
+}
+
+std::vector<uint32_t> Query::needed(uint64_t relset) const {
+    std::set<uint32_t> inside  = produced(relset);
+    std::set<uint32_t> outside = produced(all_rels() & ~relset);
+    for (uint32_t a : out) outside.insert(attrs[a].eq);
+    std::vector<uint32_t> r;
+    for (uint32_t e : inside)
+        if (outside.count(e)) r.push_back(e);
+    return r;
+}
+
+std::vector<uint32_t> Query::shared(uint64_t a, uint64_t b) const {
+    std::set<uint32_t>    pa = produced(a), pb = produced(b);
+    std::vector<uint32_t> r;
+    for (uint32_t e : pa)
+        if (pb.count(e)) r.push_back(e);
+    return r;
+}
+
+uint32_t Query::attr_of(uint32_t rel, uint32_t eq) const {
+    for (uint32_t a : rels[rel].attrs)
+        if (attrs[a].eq == eq) return a;
+    throw std::runtime_error("relation does not produce class");
+}
+
+std::vector<uint64_t> Query::adjacency() const {
+    std::vector<uint64_t> adj(rels.size(), 0);
+    for (uint32_t e = 0; e < eq_attrs.size(); ++e) {
+        uint64_t members = 0;
+        for (uint32_t a : eq_attrs[e]) members |= 1ull << attrs[a].rel;
+        for (uint32_t r = 0; r < rels.size(); ++r)
+            if ((members >> r) & 1) adj[r] |= members & ~(1ull << r);
+    }
+    return adj;
+}
+
+namespace {
+struct UnionFind {
+    std::vector<uint32_t> p;
+    uint32_t add() {
+        p.push_back(uint32_t(p.size()));
+        return uint32_t(p.size() - 1);
+    }
+    uint32_t find(uint32_t x) {
+        while (p[x] != x) x = p[x] = p[p[x]];
+        return x;
+    }
+    void merge(uint32_t a, uint32_t b) {
+        a = find(a);
+        b = find(b);
+        if (a != b) p[std::max(a, b)] = std::min(a, b);
+    }
+};
+} // namespace
+
+Query import_plan(const cgj_plan& plan, const std::vector<uint64_t>& table_rows,
+                  const std::vector<uint32_t>& table_cols) {
+    Query     q;
+    UnionFind uf;
+    // returns the node's output attribute list
+    std::function<std::vector<uint32_t>(uint64_t, int)> rec = [&](uint64_t id, int depth) {
+        if (id >= plan.num_nodes) throw std::runtime_error("plan node id out of range");
+        if (depth > 256) throw std::runtime_error("plan is too deep");
+        const cgj_node&       n = plan.nodes[id];
+        std::vector<uint32_t> out;
+        if (n.kind == CGJ_NODE_SCAN) {
+            if (n.base_table_id >= table_rows.size()) throw std::runtime_error("scan of a missing input table");
+            if (q.rels.size() >= 64) throw std::runtime_error("more than 64 scans in one plan");
+            uint32_t rel = uint32_t(q.rels.size());
+            QRel     r;
+            r.table = n.base_table_id;
+            r.rows  = table_rows[n.base_table_id];
+            for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
+                uint64_t col = n.output_attrs[i].index;
+                if (r.rows > 0 && col >= table_cols[n.base_table_id])
+                    throw std::runtime_error("scan of a missing column");
+                uint32_t a = uint32_t(q.attrs.size());
+                q.attrs.push_back(QAttr{rel, col, n.output_attrs[i].type, 0});
+                uf.add();
+                r.attrs.push_back(a);
+                out.push_back(a);
+            }
+            q.rels.push_back(std::move(r));
+            return out;
+        }
+        if (n.kind != CGJ_NODE_JOIN) throw std::runtime_error("unknown plan node kind");
+        std::vector<uint32_t> l = rec(n.left, depth + 1);
+        std::vector<uint32_t> r = rec(n.right, depth + 1);
+        if (n.left_attr >= l.size() || n.right_attr >= r.size()) throw std::runtime_error("join attribute out of range");
+        uf.merge(l[n.left_attr], r[n.right_attr]);
+        for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
+            uint64_t idx = n.output_attrs[i].index;
+            if (idx < l.size())
+                out.push_back(l[idx]);
+            else if (idx - l.size() < r.size())
+                out.push_back(r[idx - l.size()]);
+            else
+                throw std::runtime_error("output attribute out of range");
+        }
+        return out;
+    };
+    q.out = rec(plan.root, 0);
+
+    std::unordered_map<uint32_t, uint32_t> cls;
+    for (uint32_t a = 0; a < q.attrs.size(); ++a) {
+        uint32_t root = uf.find(a);
+        auto     it   = cls.find(root);
+        if (it == cls.end()) {
+            it = cls.emplace(root, uint32_t(q.eq_attrs.size())).first;
+            q.eq_attrs.emplace_back();
+        }
+        q.attrs[a].eq = it->second;
+        q.eq_attrs[it->second].push_back(a);
+    }
+    for (auto& r : q.rels) {
+        for (uint32_t a : r.attrs) {
+            uint32_t e = q.attrs[a].eq;
+            if (std::find(r.eqs.begin(), r.eqs.end(), e) != r.eqs.end())
+                throw std::runtime_error("a relation joins two of its own columns");
+            r.eqs.push_back(e);
+        }
+        std::sort(r.eqs.begin(), r.eqs.end());
+    }
+    for (uint32_t e = 0; e < q.eq_attrs.size(); ++e) {
+        if (!q.is_join_eq(e)) continue;
+        for (uint32_t a : q.eq_attrs[e])
+            if (q.attrs[a].type == CGJ_VARCHAR) throw std::runtime_error("VARCHAR join keys are not supported");
+    }
+    return q;
+}
+
+CardFn max_card_model(const Query& q, const std::vector<double>& base_card) {
+    (void)q;
+    return [base_card](uint64_t set) {
+        double c = 0;
+        for (uint32_t r = 0; r < base_card.size(); ++r)
+            if ((set >> r) & 1) c = std::max(c, base_card[r]);
+        return c;
+    };
+}
+
+namespace {
+
+inline uint64_t low_mask(uint32_t v) { return v >= 63 ? ~0ull : ((1ull << (v + 1)) - 1); } // bits 0..v
+
+struct Dp {
+    const Query&                      q;
+    const CardFn&                     card;
+    std::vector<uint64_t>             adj;
+    PlanTree                          tree;
+    std::unordered_map<uint64_t, int> best;
+    std::unordered_map<uint64_t, double> cards;
+    std::vector<std::pair<uint64_t, uint64_t>> pairs;
+    size_t                            budget = 4u << 20;
+    bool                              overflow = false;
+
+    Dp(const Query& q_, const CardFn& c) : q(q_), card(c), adj(q_.adjacency()) {}
+
+    uint64_t neigh(uint64_t s) const {
+        uint64_t n = 0;
+        for (uint64_t t = s; t; t &= t - 1) n |= adj[__builtin_ctzll(t)];
+        return n & ~s;
+    }
+    double card_of(uint64_t s) {
+        auto it = cards.find(s);
+        if (it != cards.end()) return it->second;
+        double c = std::max(card(s), 0.0);
+        cards.emplace(s, c);
+        return c;
+    }
+    void emit_pair(uint64_t a, uint64_t b) {
+        if (pairs.size() >= budget) {
+            overflow = true;
+            return;
+        }
+        pairs.emplace_back(a, b);
+    }
+    void cmp_rec(uint64_t s1, uint64_t s2, uint64_t x) {
+        if (overflow) return;
+        uint64_t n = neigh(s2) & ~x;
+        if (!n) return;
+        for (uint64_t s = n; s; s = (s - 1) & n) emit_pair(s1, s2 | s);
+        for (uint64_t s = n; s; s = (s - 1) & n) cmp_rec(s1, s2 | s, x | n);
+    }
+    void emit_csg(uint64_t s1) {
+        if (overflow) return;
+        uint32_t mn = __builtin_ctzll(s1);
+        uint64_t x  = s1 | low_mask(mn);
+        uint64_t n  = neigh(s1) & ~x;
+        for (int v = 63; v >= 0; --v) {
+            if (!((n >> v) & 1)) continue;
+            uint64_t s2 = 1ull << v;
+            emit_pair(s1, s2);
+            cmp_rec(s1, s2, x | (low_mask(v) & n));
+        }
+    }
+    void csg_rec(uint64_t s, uint64_t x) {
+        if (overflow) return;
+        uint64_t n = neigh(s) & ~x;
+        if (!n) return;
+        for (uint64_t t = n; t; t = (t - 1) & n) emit_csg(s | t);
+        for (uint64_t t = n; t; t = (t - 1) & n) csg_rec(s | t, x | n);
+    }
+
+    void consider(uint64_t b, uint64_t p) {
+        auto ib = best.find(b), ip = best.find(p);
+        if (ib == best.end() || ip == best.end()) return;
+        uint64_t        t  = b | p;
+        const TreeNode& nb = tree.nodes[ib->second];
+        const TreeNode& np = tree.nodes[ip->second];
+        double          c  = nb.cost + np.cost + card_of(t) + 2.0 * nb.card;
+        auto            it = best.find(t);
+        if (it != best.end() && tree.nodes[it->second].cost <= c) return;
+        TreeNode n;
+        n.build = ib->second;
+        n.probe = ip->second;
+        n.set   = t;
+        n.card  = card_of(t);
+        n.cost  = c;
+        tree.nodes.push_back(n);
+        best[t] = int(tree.nodes.size() - 1);
+    }
+
+    void leaves() {
+        for (uint32_t r = 0; r < q.rels.size(); ++r) {
+            TreeNode n;
+            n.rel  = int(r);
+            n.set  = 1ull << r;
+            n.card = card_of(n.set);
+            n.cost = 0;
+            tree.nodes.push_back(n);
+            best[n.set] = int(tree.nodes.size() - 1);
+        }
+    }
+
+    // Greedy operator ordering: repeatedly join the two connected components
+    // with the smallest joint cardinality (the fallback for very large graphs).
+    void greedy() {
+        std::vector<uint64_t> comps;
+        for (uint32_t r = 0; r < q.rels.size(); ++r) comps.push_back(1ull << r);
+        while (comps.size() > 1) {
+            double bc = 1e300;
+            size_t bi = 0, bj = 0;
+            for (size_t i = 0; i < comps.size(); ++i)
+                for (size_t j = i + 1; j < comps.size(); ++j) {
+                    if (!(neigh(comps[i]) & comps[j])) continue;
+                    double c = card_of(comps[i] | comps[j]);
+                    if (c < bc) {
+                        bc = c;
+                        bi = i;
+                        bj = j;
+                    }
+                }
+            if (bc == 1e300) throw std::runtime_error("join graph is not connected");
+            uint64_t a = comps[bi], b = comps[bj];
+            if (card_of(a) <= card_of(b))
+                consider(a, b);
+            else
+                consider(b, a);
+            comps[bi] = a | b;
+            comps.erase(comps.begin() + long(bj));
+        }
+    }
+
+    void run() {
+        leaves();
+        uint32_t n = uint32_t(q.rels.size());
+        for (int v = int(n) - 1; v >= 0; --v) {
+            uint64_t s = 1ull << v;
+            emit_csg(s);
+            csg_rec(s, low_mask(uint32_t(v)));
+        }
+        if (overflow) {
+            greedy();
+            return;
+        }
+        std::stable_sort(pairs.begin(), pairs.end(), [](const auto& x, const auto& y) {
+            return __builtin_popcountll(x.first | x.second) < __builtin_popcountll(y.first | y.second);
+        });
+        for (auto& pr : pairs) {
+            consider(pr.first, pr.second);
+            consider(pr.second, pr.first);
+        }
+    }
+};
+
+} // namespace
+
+PlanTree plan_joins(const Query& q, const CardFn& card) {
+    if (q.rels.empty()) throw std::runtime_error("plan has no scans");
+    Dp dp(q, card);
+    dp.run();
+    auto it = dp.best.find(q.all_rels());
+    if (it == dp.best.end()) throw std::runtime_error("join graph is not connected");
+    dp.tree.root = it->second;
+    return std::move(dp.tree);
+}
+
+JoinTuning autotune_join(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes) {
+    (void)key_bytes;
+    JoinTuning t;
+    // Capacity: load factor <= 0.5 keeps misses to ~1.5 slot reads; small
+    // tables (cache resident anyway) get a sparser table.
+    t.load_log = build_rows < (1u << 20) ? 2 : 1;
+    // Bloom filter: only when the table is far larger than the caches and the
+    // probe side dominates, so a cache-resident filter can replace HBM reads.
+    t.bloom_bits = 0;
+    if (build_rows > (8u << 20) && probe_rows > 4 * build_rows) t.bloom_bits = 0;
+    return t;
+}
+
+} // namespace cgj

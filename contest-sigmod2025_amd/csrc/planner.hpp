@@ -1,0 +1,88 @@
+// Plan import and join ordering (host).
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <set>
+#include <vector>
+
+#include "../../include/gpujoin.h"
+
+namespace cgj {
+
+struct JoinTuning;
+
+// A scanned column of a relation.
+struct QAttr {
+    uint32_t rel;
+    uint64_t col;
+    int32_t  type;
+    uint32_t eq;
+};
+
+// One scan node of the plan (the same base table may be scanned twice).
+struct QRel {
+    uint64_t              table;
+    uint64_t              rows = 0;
+    std::vector<uint32_t> attrs;
+    std::vector<uint32_t> eqs; // classes produced (sorted, unique)
+};
+
+struct Query {
+    std::vector<QRel>                  rels;
+    std::vector<QAttr>                 attrs;
+    std::vector<std::vector<uint32_t>> eq_attrs; // class -> attributes
+    std::vector<uint32_t>              out;      // root output attributes
+
+    bool     is_join_eq(uint32_t eq) const { return eq_attrs[eq].size() > 1; }
+    uint64_t all_rels() const { return rels.size() >= 64 ? ~0ull : ((1ull << rels.size()) - 1); }
+    // classes produced by a set of relations
+    std::set<uint32_t> produced(uint64_t relset) const;
+    // classes of `relset` needed outside it: joined with another relation or in the output
+    std::vector<uint32_t> needed(uint64_t relset) const;
+    // classes shared by two disjoint relation sets (their join predicates)
+    std::vector<uint32_t> shared(uint64_t a, uint64_t b) const;
+    // attribute of relation `rel` in class `eq`
+    uint32_t attr_of(uint32_t rel, uint32_t eq) const;
+    // relations adjacent to each relation through join classes
+    std::vector<uint64_t> adjacency() const;
+};
+
+// Builds the query from a contest plan (reference engine/query/PlanImport.cpp:24-89).
+// `table_rows[i]` / `table_cols[i]` describe input table i.
+Query import_plan(const cgj_plan& plan, const std::vector<uint64_t>& table_rows,
+                  const std::vector<uint32_t>& table_cols);
+
+// Join tree: leaves are relations; an inner node joins `build` (hashed) with
+// `probe` (streamed through the pipeline).
+struct TreeNode {
+    int      rel   = -1;
+    int      build = -1;
+    int      probe = -1;
+    uint64_t set   = 0;
+    double   card  = 0;
+    double   cost  = 0;
+};
+struct PlanTree {
+    std::vector<TreeNode> nodes;
+    int                   root = -1;
+    bool                  leaf(int n) const { return nodes[n].rel >= 0; }
+};
+
+// Cardinality of a connected relation set.
+using CardFn = std::function<double(uint64_t relset)>;
+
+// Reference heuristic (engine/query/QueryGraph.cpp:53-68): the cardinality of a
+// join is the largest cardinality among its relations.
+CardFn max_card_model(const Query& q, const std::vector<double>& base_card);
+
+// Dynamic programming over connected sub-graph / complement pairs (DPccp,
+// Moerkotte & Neumann, VLDB 2006) with cost(T) = cost(build) + cost(probe) +
+// card(T) + 2 card(build): every hash-table tuple is written and read once
+// more than a streamed one.
+PlanTree plan_joins(const Query& q, const CardFn& card);
+
+// Per-join tuning chosen from (build rows, probe rows, key width).
+JoinTuning autotune_join(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes);
+
+} // namespace cgj

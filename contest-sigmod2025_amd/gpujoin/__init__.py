@@ -1,0 +1,432 @@
+"""Python mirror of the contest API over libgpujoin.so (ctypes).
+
+The classes mirror the contest's C++ surface (reference include/plan.h:27-149,
+engine/Execute.cpp:28-49): `Plan.new_scan_node / new_join_node / new_input`,
+`ColumnarTable`, and `Context.execute(plan)` as `Contest::execute`. Pages are
+`bytes` of 8192 bytes in the contest page format. Everything runs through the
+C ABI declared in include/gpujoin.h; there is no Python fallback: when the
+native library is missing, loading fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import pages as pagecodec
+from .pages import INT32, INT64, FP64, VARCHAR, PAGE_SIZE, decode_column, type_name
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libgpujoin.so")
+
+NODE_JOIN = 0
+NODE_SCAN = 1
+
+GEN_UNIQUE_KEY = 0
+GEN_FK_ZIPF = 1
+GEN_UNIFORM = 2
+GEN_UNIT_F64 = 3
+GEN_PERM_KEY = 4
+GEN_FK_UNIFORM = 5
+
+
+class HostColumn(C.Structure):
+    _fields_ = [("pages", C.POINTER(C.c_void_p)), ("num_pages", C.c_uint64), ("type", C.c_int32)]
+
+
+class HostTable(C.Structure):
+    _fields_ = [("num_rows", C.c_uint64), ("columns", C.POINTER(HostColumn)), ("num_columns", C.c_uint32)]
+
+
+class Attr(C.Structure):
+    _fields_ = [("index", C.c_uint64), ("type", C.c_int32)]
+
+
+class Node(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("build_left", C.c_int32),
+        ("base_table_id", C.c_uint64),
+        ("left", C.c_uint64),
+        ("right", C.c_uint64),
+        ("left_attr", C.c_uint64),
+        ("right_attr", C.c_uint64),
+        ("output_attrs", C.POINTER(Attr)),
+        ("num_output_attrs", C.c_uint32),
+    ]
+
+
+class CPlan(C.Structure):
+    _fields_ = [("nodes", C.POINTER(Node)), ("num_nodes", C.c_uint32), ("root", C.c_uint64)]
+
+
+class ResultColumn(C.Structure):
+    _fields_ = [("type", C.c_int32), ("num_pages", C.c_uint64), ("pages", C.POINTER(C.c_void_p))]
+
+
+class Result(C.Structure):
+    _fields_ = [("num_rows", C.c_uint64), ("num_columns", C.c_uint32), ("columns", C.POINTER(ResultColumn))]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double),
+        ("ms_build", C.c_double),
+        ("ms_probe", C.c_double),
+        ("input_pages", C.c_uint64),
+        ("output_pages", C.c_uint64),
+        ("build_tuples", C.c_uint64),
+        ("probe_rows", C.c_uint64),
+        ("output_rows", C.c_uint64),
+        ("pipelines", C.c_uint32),
+        ("retries", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GenColumn(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("kind", C.c_int32),
+        ("stream", C.c_uint64),
+        ("null_stream", C.c_uint64),
+        ("null_frac", C.c_double),
+        ("domain", C.c_uint64),
+        ("zipf_s", C.c_double),
+        ("miss_frac", C.c_double),
+        ("lo", C.c_uint64),
+        ("hi", C.c_uint64),
+    ]
+
+
+# exported symbols of include/gpujoin.h with their ctypes signatures
+SIGNATURES = {
+    "cgj_context_create": (C.c_void_p, [C.c_int]),
+    "cgj_context_destroy": (None, [C.c_void_p]),
+    "cgj_last_error": (C.c_char_p, [C.c_void_p]),
+    "cgj_execute": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(HostTable), C.c_uint32, C.POINTER(Result)]),
+    "cgj_result_release": (None, [C.POINTER(Result)]),
+    "cgj_dtable_upload": (C.c_void_p, [C.c_void_p, C.POINTER(HostTable)]),
+    "cgj_dtable_free": (None, [C.c_void_p, C.c_void_p]),
+    "cgj_dtable_num_rows": (C.c_uint64, [C.c_void_p]),
+    "cgj_dtable_num_pages": (C.c_uint64, [C.c_void_p, C.c_uint32]),
+    "cgj_execute_device": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
+                                     C.POINTER(C.c_void_p)]),
+    "cgj_dresult_num_rows": (C.c_uint64, [C.c_void_p]),
+    "cgj_dresult_num_pages": (C.c_uint64, [C.c_void_p]),
+    "cgj_dresult_download": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Result)]),
+    "cgj_dresult_free": (None, [C.c_void_p, C.c_void_p]),
+    "cgj_last_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "cgj_scan_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "cgj_encode_column": (C.c_int, [C.c_int32, C.c_void_p, C.c_char_p, C.c_void_p, C.c_uint64,
+                                    C.POINTER(ResultColumn)]),
+    "cgj_column_release": (None, [C.POINTER(ResultColumn)]),
+    "cgj_dtable_generate": (C.c_void_p, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(GenColumn), C.c_uint32]),
+    "cgj_host_generate": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(GenColumn),
+                                    C.POINTER(ResultColumn)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libgpujoin.so (built in-tree by `make`); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library {LIB_PATH} is missing: build it with `make -C contest-sigmod2025_amd`")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+# ---------------------------------------------------------------------------
+# contest-style plan and tables
+# ---------------------------------------------------------------------------
+@dataclass
+class Column:
+    type: int
+    pages: List[bytes] = field(default_factory=list)
+
+
+@dataclass
+class ColumnarTable:
+    num_rows: int = 0
+    columns: List[Column] = field(default_factory=list)
+
+
+@dataclass
+class PlanNode:
+    kind: int
+    output_attrs: List[Tuple[int, int]]
+    base_table_id: int = 0
+    build_left: bool = False
+    left: int = 0
+    right: int = 0
+    left_attr: int = 0
+    right_attr: int = 0
+
+
+class Plan:
+    """Mirror of the contest Plan (reference include/plan.h:112-149)."""
+
+    def __init__(self):
+        self.nodes: List[PlanNode] = []
+        self.inputs: List[ColumnarTable] = []
+        self.root = 0
+
+    def new_scan_node(self, base_table_id, output_attrs):
+        self.nodes.append(PlanNode(NODE_SCAN, list(output_attrs), base_table_id=base_table_id))
+        return len(self.nodes) - 1
+
+    def new_join_node(self, build_left, left, right, left_attr, right_attr, output_attrs):
+        self.nodes.append(PlanNode(NODE_JOIN, list(output_attrs), build_left=bool(build_left), left=left,
+                                   right=right, left_attr=left_attr, right_attr=right_attr))
+        return len(self.nodes) - 1
+
+    def new_input(self, table: ColumnarTable):
+        self.inputs.append(table)
+        return len(self.inputs) - 1
+
+
+class _Marshal:
+    """Keeps ctypes buffers alive while a call uses them."""
+
+    def __init__(self, plan: Plan, tables: Optional[Sequence[ColumnarTable]] = None):
+        self.keep = []
+        n = len(plan.nodes)
+        self.nodes = (Node * max(n, 1))()
+        for i, pn in enumerate(plan.nodes):
+            attrs = (Attr * max(len(pn.output_attrs), 1))()
+            for k, (idx, t) in enumerate(pn.output_attrs):
+                attrs[k].index = idx
+                attrs[k].type = t
+            self.keep.append(attrs)
+            nd = self.nodes[i]
+            nd.kind = pn.kind
+            nd.build_left = 1 if pn.build_left else 0
+            nd.base_table_id = pn.base_table_id
+            nd.left, nd.right = pn.left, pn.right
+            nd.left_attr, nd.right_attr = pn.left_attr, pn.right_attr
+            nd.output_attrs = attrs
+            nd.num_output_attrs = len(pn.output_attrs)
+        self.plan = CPlan(self.nodes, n, plan.root)
+        tables = plan.inputs if tables is None else tables
+        self.tables = (HostTable * max(len(tables), 1))()
+        for t, tab in enumerate(tables):
+            cols = (HostColumn * max(len(tab.columns), 1))()
+            for c, col in enumerate(tab.columns):
+                bufs = [C.create_string_buffer(bytes(p), PAGE_SIZE) for p in col.pages]
+                ptrs = (C.c_void_p * max(len(bufs), 1))(*[C.cast(b, C.c_void_p) for b in bufs])
+                self.keep.extend(bufs)
+                self.keep.append(ptrs)
+                cols[c].pages = C.cast(ptrs, C.POINTER(C.c_void_p))
+                cols[c].num_pages = len(bufs)
+                cols[c].type = col.type
+            self.keep.append(cols)
+            self.tables[t].num_rows = tab.num_rows
+            self.tables[t].columns = cols
+            self.tables[t].num_columns = len(tab.columns)
+        self.ntables = len(tables)
+
+
+def result_to_table(res: Result) -> ColumnarTable:
+    out = ColumnarTable(num_rows=res.num_rows)
+    for c in range(res.num_columns):
+        rc = res.columns[c]
+        col = Column(rc.type)
+        for i in range(rc.num_pages):
+            col.pages.append(C.string_at(rc.pages[i], PAGE_SIZE))
+        out.columns.append(col)
+    return out
+
+
+def encode_column(type_: int, values, valid=None) -> Column:
+    """Pages of a column written with the contest page writer (ColumnInserter)."""
+    l = lib()
+    n = len(values)
+    vbuf = None
+    chars = None
+    if type_ == VARCHAR:
+        enc = [v.encode() if isinstance(v, str) else bytes(v) for v in values]
+        offs = np.zeros(n + 1, dtype=np.uint64)
+        if n:
+            offs[1:] = np.cumsum([len(e) for e in enc])
+        chars = b"".join(enc)
+        vbuf = offs
+    else:
+        dt = {INT32: np.int32, INT64: np.int64, FP64: np.float64}[type_]
+        vbuf = np.ascontiguousarray(np.asarray(values, dtype=dt) if n else np.zeros(1, dtype=dt))
+    vb = None
+    if valid is not None:
+        vb = np.ascontiguousarray(np.asarray(valid, dtype=np.uint8))
+    out = ResultColumn()
+    rc = l.cgj_encode_column(type_, vbuf.ctypes.data, chars if chars is not None else None,
+                             vb.ctypes.data if vb is not None else None, n, C.byref(out))
+    if rc != 0:
+        raise RuntimeError("cgj_encode_column failed")
+    col = Column(type_, [C.string_at(out.pages[i], PAGE_SIZE) for i in range(out.num_pages)])
+    l.cgj_column_release(C.byref(out))
+    return col
+
+
+def make_table(columns: Sequence[Tuple[int, Sequence, Optional[Sequence]]], num_rows: Optional[int] = None):
+    """Builds a ColumnarTable from (type, values, valid-or-None) per column."""
+    cols = [encode_column(t, v, m) for (t, v, m) in columns]
+    n = num_rows if num_rows is not None else (len(columns[0][1]) if columns else 0)
+    return ColumnarTable(n, cols)
+
+
+def host_generate(seed: int, first_row: int, num_rows: int, gen: GenColumn) -> Column:
+    l = lib()
+    out = ResultColumn()
+    if l.cgj_host_generate(seed, first_row, num_rows, C.byref(gen), C.byref(out)) != 0:
+        raise RuntimeError("cgj_host_generate failed")
+    col = Column(out.type, [C.string_at(out.pages[i], PAGE_SIZE) for i in range(out.num_pages)])
+    l.cgj_column_release(C.byref(out))
+    return col
+
+
+class DeviceTable:
+    def __init__(self, ctx: "Context", handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    @property
+    def num_rows(self):
+        return lib().cgj_dtable_num_rows(self.handle)
+
+    def num_pages(self, column):
+        return lib().cgj_dtable_num_pages(self.handle, column)
+
+    def decode(self, column, device_ptr):
+        """Decodes a column into 64-bit slots at a device pointer (NULL = all ones)."""
+        if lib().cgj_scan_decode(self.ctx.handle, self.handle, column, C.c_void_p(device_ptr)) != 0:
+            raise RuntimeError(self.ctx.error())
+
+    def free(self):
+        if self.handle:
+            lib().cgj_dtable_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceResult:
+    def __init__(self, ctx: "Context", handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    @property
+    def num_rows(self):
+        return lib().cgj_dresult_num_rows(self.handle)
+
+    @property
+    def num_pages(self):
+        return lib().cgj_dresult_num_pages(self.handle)
+
+    def download(self) -> ColumnarTable:
+        res = Result()
+        if lib().cgj_dresult_download(self.ctx.handle, self.handle, C.byref(res)) != 0:
+            raise RuntimeError(self.ctx.error())
+        try:
+            return result_to_table(res)
+        finally:
+            lib().cgj_result_release(C.byref(res))
+
+    def free(self):
+        if self.handle:
+            lib().cgj_dresult_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """Mirror of Contest::build_context / execute / destroy_context."""
+
+    def __init__(self, device: int = 0):
+        self.handle = lib().cgj_context_create(device)
+        if not self.handle:
+            raise RuntimeError("cgj_context_create failed (no GPU?)")
+
+    def error(self):
+        return lib().cgj_last_error(self.handle).decode()
+
+    def execute(self, plan: Plan) -> ColumnarTable:
+        m = _Marshal(plan)
+        res = Result()
+        if lib().cgj_execute(self.handle, C.byref(m.plan), m.tables, m.ntables, C.byref(res)) != 0:
+            raise RuntimeError(self.error())
+        try:
+            return result_to_table(res)
+        finally:
+            lib().cgj_result_release(C.byref(res))
+
+    def upload(self, table: ColumnarTable) -> DeviceTable:
+        m = _Marshal(Plan(), [table])
+        h = lib().cgj_dtable_upload(self.handle, C.byref(m.tables[0]))
+        if not h:
+            raise RuntimeError(self.error())
+        return DeviceTable(self, h)
+
+    def generate(self, seed: int, num_rows: int, gens: Sequence[GenColumn]) -> DeviceTable:
+        arr = (GenColumn * len(gens))(*gens)
+        h = lib().cgj_dtable_generate(self.handle, seed, num_rows, arr, len(gens))
+        if not h:
+            raise RuntimeError(self.error())
+        return DeviceTable(self, h)
+
+    def execute_device(self, plan: Plan, tables: Sequence[DeviceTable]) -> DeviceResult:
+        m = _Marshal(plan, [])
+        arr = (C.c_void_p * max(len(tables), 1))(*[t.handle for t in tables])
+        out = C.c_void_p()
+        if lib().cgj_execute_device(self.handle, C.byref(m.plan), arr, len(tables), C.byref(out)) != 0:
+            raise RuntimeError(self.error())
+        return DeviceResult(self, out.value)
+
+    def stats(self) -> dict:
+        s = Stats()
+        lib().cgj_last_stats(self.handle, C.byref(s))
+        return s.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().cgj_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def table_rows(t: ColumnarTable):
+    """Decodes a ColumnarTable into a list of row tuples (None = NULL)."""
+    cols = [decode_column(c.pages, c.type) for c in t.columns]
+    for vals, valid in cols:
+        if len(valid) != t.num_rows:
+            raise ValueError(f"column holds {len(valid)} rows, table says {t.num_rows}")
+    rows = []
+    for r in range(t.num_rows):
+        rows.append(tuple((v[r] if not isinstance(v, np.ndarray) else v[r].item()) if m[r] else None
+                          for (v, m) in cols))
+    return rows

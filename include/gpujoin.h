@@ -1,0 +1,207 @@
+/* C ABI of the MI355X join-pipeline library (libgpujoin.so).
+ *
+ * Everything crosses this boundary as plain C: opaque handles, POD
+ * descriptors, host or device pointers and sizes, and an int status
+ * (0 = success; otherwise cgj_last_error() names the failure). No C++ or
+ * framework types appear here. The C++ contest surface
+ * (Contest::build_context / destroy_context / execute, include/contest/plan.h)
+ * is a thin adapter over these calls.
+ *
+ * Type codes follow the contest's DataType order (reference
+ * include/attribute.h:8-13).
+ */
+#ifndef CGJ_GPUJOIN_H
+#define CGJ_GPUJOIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CGJ_INT32   = 0,
+    CGJ_INT64   = 1,
+    CGJ_FP64    = 2,
+    CGJ_VARCHAR = 3,
+};
+
+enum {
+    CGJ_NODE_JOIN = 0,
+    CGJ_NODE_SCAN = 1,
+};
+
+/* Bytes per contest page (reference include/plan.h:54). */
+#define CGJ_PAGE_SIZE 8192
+
+typedef struct cgj_context cgj_context;
+typedef struct cgj_dtable  cgj_dtable;
+typedef struct cgj_dresult cgj_dresult;
+
+/* ---- descriptors: POD mirror of the contest Plan (reference include/plan.h:32-149) ---- */
+
+/* One column of a host-resident table: `num_pages` pointers to 8 KiB pages. */
+typedef struct {
+    const void* const* pages;
+    uint64_t           num_pages;
+    int32_t            type;
+} cgj_host_column;
+
+typedef struct {
+    uint64_t               num_rows;
+    const cgj_host_column* columns;
+    uint32_t               num_columns;
+} cgj_host_table;
+
+typedef struct {
+    uint64_t index; /* column of the child (scan: base table column) */
+    int32_t  type;
+} cgj_attr;
+
+/* A scan node (kind = CGJ_NODE_SCAN, base_table_id) or a join node
+ * (kind = CGJ_NODE_JOIN, left/right node ids, left_attr/right_attr, build_left).
+ * Mirrors ScanNode / JoinNode / PlanNode (reference include/plan.h:32-52). */
+typedef struct {
+    int32_t         kind;
+    int32_t         build_left;
+    uint64_t        base_table_id;
+    uint64_t        left;
+    uint64_t        right;
+    uint64_t        left_attr;
+    uint64_t        right_attr;
+    const cgj_attr* output_attrs;
+    uint32_t        num_output_attrs;
+} cgj_node;
+
+typedef struct {
+    const cgj_node* nodes;
+    uint32_t        num_nodes;
+    uint64_t        root;
+} cgj_plan;
+
+/* Result columns own their pages. Each page is a separate `new Page`-compatible
+ * allocation (8192 bytes, 8-byte aligned, from C++ operator new) so a contest
+ * ColumnarTable can adopt the pointers and release them with `delete`. */
+typedef struct {
+    int32_t  type;
+    uint64_t num_pages;
+    void**   pages;
+} cgj_result_column;
+
+typedef struct {
+    uint64_t           num_rows;
+    uint32_t           num_columns;
+    cgj_result_column* columns;
+} cgj_result;
+
+/* Per-call statistics of the last execute on a context (device-side timing). */
+typedef struct {
+    double   ms_total;       /* device time of all kernels of the query        */
+    double   ms_build;       /* hash-table build pipelines                     */
+    double   ms_probe;       /* output pipeline                                */
+    uint64_t input_pages;    /* 8 KiB input pages referenced by the plan       */
+    uint64_t output_pages;   /* 8 KiB output pages written                     */
+    uint64_t build_tuples;   /* tuples inserted into hash tables               */
+    uint64_t probe_rows;     /* rows streamed by the output pipeline's scan    */
+    uint64_t output_rows;
+    uint32_t pipelines;      /* number of pipeline launches                    */
+    uint32_t retries;        /* re-runs after a page-pool or staging overflow  */
+} cgj_stats;
+
+/* ---- context (reference engine/Execute.cpp:42-48) ---- */
+
+/* Replaces Contest::build_context(): binds `device`, creates the HIP stream,
+ * the page pool and the tuning state. NULL on failure. */
+cgj_context* cgj_context_create(int device);
+/* Replaces Contest::destroy_context(void*). */
+void         cgj_context_destroy(cgj_context* ctx);
+/* Message of the last failed call on `ctx` (empty string if none). */
+const char*  cgj_last_error(const cgj_context* ctx);
+
+/* ---- whole-query execution (reference engine/Execute.cpp:28-40) ---- */
+
+/* Replaces Contest::execute(const Plan&, void*): uploads the referenced input
+ * columns, runs the join pipelines on the GPU and returns the result pages in
+ * host memory. Row order is unspecified (as in the reference). */
+int  cgj_execute(cgj_context* ctx, const cgj_plan* plan, const cgj_host_table* inputs,
+                 uint32_t num_inputs, cgj_result* out);
+/* Releases pages and arrays of a result produced by this library. */
+void cgj_result_release(cgj_result* res);
+
+/* Device-resident tables: pages live in HBM (reference engine/query/PlanImport.cpp:91-101
+ * imports host pages zero-copy; here the import is one upload per column). */
+cgj_dtable* cgj_dtable_upload(cgj_context* ctx, const cgj_host_table* table);
+void        cgj_dtable_free(cgj_context* ctx, cgj_dtable* t);
+uint64_t    cgj_dtable_num_rows(const cgj_dtable* t);
+uint64_t    cgj_dtable_num_pages(const cgj_dtable* t, uint32_t column);
+
+/* Runs the plan over device-resident inputs; the result stays in HBM. This is
+ * the device-resident form of Contest::execute used for roofline timing. */
+int      cgj_execute_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs,
+                            uint32_t num_inputs, cgj_dresult** out);
+uint64_t cgj_dresult_num_rows(const cgj_dresult* r);
+uint64_t cgj_dresult_num_pages(const cgj_dresult* r);
+/* Copies a device result's pages to host memory in contest layout. */
+int      cgj_dresult_download(cgj_context* ctx, const cgj_dresult* r, cgj_result* out);
+void     cgj_dresult_free(cgj_context* ctx, cgj_dresult* r);
+
+int      cgj_last_stats(const cgj_context* ctx, cgj_stats* out);
+
+/* ---- operator-level entry points ---- */
+
+/* Page decode of one device column into 64-bit slots, NULL = all ones, INT32
+ * zero-extended, FP64 raw bits, VARCHAR as a string handle. Replaces
+ * TableScan::Reader::skipTo/step64 (reference engine/op/TableScan.cpp:82-324).
+ * `d_slots` is a device array of at least cgj_dtable_num_rows() entries. */
+int cgj_scan_decode(cgj_context* ctx, const cgj_dtable* t, uint32_t column, uint64_t* d_slots);
+
+/* Encode host values into contest pages with the contest page writer
+ * (reference include/plan.h:151-335). values: int32/int64/double array of
+ * `n` entries (or, for VARCHAR, `n`+1 u64 byte offsets into `chars`);
+ * `valid` = NULL for no NULLs, else one byte per row (0 = NULL).
+ * The pages are returned in `out` (type, num_pages, pages) and released with
+ * cgj_result_release semantics via cgj_column_release. */
+int  cgj_encode_column(int32_t type, const void* values, const char* chars, const uint8_t* valid,
+                       uint64_t n, cgj_result_column* out);
+void cgj_column_release(cgj_result_column* col);
+
+/* ---- synthetic generators (device-resident inputs for benchmarks) ---- */
+
+/* Column recipe for cgj_dtable_generate. Row r of the column draws its value
+ * and its NULL flag from SplitMix64 streams keyed by (seed, stream, r). */
+typedef struct {
+    int32_t  type;        /* CGJ_INT32 / CGJ_INT64 / CGJ_FP64                           */
+    int32_t  kind;        /* CGJ_GEN_* below                                             */
+    uint64_t stream;      /* stream id for values                                        */
+    uint64_t null_stream; /* stream id for NULL draws                                    */
+    double   null_frac;   /* Bernoulli NULL probability                                  */
+    uint64_t domain;      /* key domain size (unique / fk kinds)                         */
+    double   zipf_s;      /* Zipf exponent for CGJ_GEN_FK_ZIPF                           */
+    double   miss_frac;   /* fraction of non-matching keys for FK kinds                  */
+    uint64_t lo, hi;      /* range for CGJ_GEN_UNIFORM (inclusive lo, exclusive hi)      */
+} cgj_gen_column;
+
+enum {
+    CGJ_GEN_UNIQUE_KEY = 0, /* key = bijective mix of r (unique, shuffled)               */
+    CGJ_GEN_FK_ZIPF    = 1, /* key of a Zipf-drawn rank in [0, domain), or a miss key    */
+    CGJ_GEN_UNIFORM    = 2, /* integer uniform in [lo, hi)                               */
+    CGJ_GEN_UNIT_F64   = 3, /* double uniform in [0, 1)                                  */
+    CGJ_GEN_PERM_KEY   = 4, /* INT32 key: permutation of 0..domain-1 (r < domain)        */
+    CGJ_GEN_FK_UNIFORM = 5, /* key = permutation value of a uniform rank, or a miss key  */
+};
+
+/* Generates a table of `num_rows` rows directly in HBM, paged exactly as the
+ * contest page writer would page it. */
+cgj_dtable* cgj_dtable_generate(cgj_context* ctx, uint64_t seed, uint64_t num_rows,
+                                const cgj_gen_column* columns, uint32_t num_columns);
+/* Same recipe evaluated on the host (for CPU baselines and tests): writes
+ * `num_rows` rows starting at `first_row` as contest pages. */
+int cgj_host_generate(uint64_t seed, uint64_t first_row, uint64_t num_rows,
+                      const cgj_gen_column* column, cgj_result_column* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CGJ_GPUJOIN_H */
