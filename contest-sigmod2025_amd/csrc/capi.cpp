@@ -232,7 +232,34 @@ int cgj_host_generate(uint64_t seed, uint64_t first_row, uint64_t num_rows, cons
                       cgj_result_column* out) {
     try {
         std::memset(out, 0, sizeof(*out));
-        if (g->type == CGJ_VARCHAR) return -1;
+        if (g->type == CGJ_VARCHAR) {
+            if (g->kind != CGJ_GEN_ASCII) return -1;
+            // ASCII strings drawn from the value stream: length uniform in
+            // [lo, hi]; a `miss_frac` share of rows carries a long string.
+            std::vector<uint8_t>  valid(num_rows);
+            std::vector<uint64_t> offs(num_rows + 1, 0);
+            std::string           chars;
+            uint64_t              base = cgj::stream_base(seed, g->stream);
+            for (uint64_t i = 0; i < num_rows; ++i) {
+                uint64_t r = first_row + i;
+                valid[i]   = !cgj::gen_is_null(seed, g->null_stream, g->null_frac, r);
+                uint64_t x = cgj::draw(base, r);
+                uint64_t len;
+                if (cgj::unit(cgj::sm_mix(x ^ 0x5851f42d4c957f2dull)) < g->miss_frac)
+                    len = 8186 + x % (16000 - 8186);
+                else
+                    len = g->lo + (g->hi >= g->lo ? x % (g->hi - g->lo + 1) : 0);
+                if (valid[i]) {
+                    uint64_t z = x;
+                    for (uint64_t k = 0; k < len; ++k) {
+                        if ((k & 7) == 0) z = cgj::sm_mix(z + cgj::kGolden);
+                        chars.push_back(char('a' + (z >> (8 * (k & 7))) % 26));
+                    }
+                }
+                offs[i + 1] = chars.size();
+            }
+            return cgj_encode_column(CGJ_VARCHAR, offs.data(), chars.data(), valid.data(), num_rows, out);
+        }
         std::vector<uint8_t>  valid(num_rows);
         std::vector<uint64_t> vals(num_rows);
         for (uint64_t i = 0; i < num_rows; ++i) {

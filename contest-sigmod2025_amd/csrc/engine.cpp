@@ -256,6 +256,7 @@ struct HashTable {
     DevBuf                slots;
     DevBuf                bloom;
     uint64_t              cap       = 0;
+    uint64_t              rmask     = 0; // linear probing wraps inside regions of rmask + 1 slots
     uint32_t              shift     = 64;
     uint32_t              words     = 1;
     uint32_t              bloom_log = 0;
@@ -308,11 +309,12 @@ public:
 
     std::unique_ptr<DevResult> run();
 
-    double   ms_build = 0, ms_probe = 0;
+    double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0;
     uint32_t launches = 0, retries = 0;
-    uint64_t build_tuples = 0, probe_rows = 0;
+    uint64_t build_tuples = 0, probe_rows = 0, build_bytes = 0, out_ref_pages = 0, probe_input_pages = 0;
 
 private:
+    double                                  last_ms_ = 0;
     Context&                                ctx_;
     const Query&                            q_;
     const PlanTree&                         t_;
@@ -400,6 +402,7 @@ uint64_t Executor::launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& 
     float f = 0;
     CGJ_HIP(hipEventElapsedTime(&f, ev0_, ev1_));
     ms += f;
+    last_ms_ = f;
     ++launches;
     return waves;
 }
@@ -409,35 +412,73 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
     ht->eqs    = eqs;
     ht->words  = uint32_t(eqs.size());
     ht->tuples = staged.count;
-    JoinTuning tn = ctx_.tune(staged.count, probe_est, 8);
-    uint32_t   lg = log2_ceil(std::max<uint64_t>(staged.count, 1)) + tn.load_log;
+    const uint64_t n  = staged.count;
+    JoinTuning     tn = ctx_.tune(n, probe_est, 8);
+    uint32_t       lg = log2_ceil(uint64_t(std::ceil(double(std::max<uint64_t>(n, 1)) / tn.max_load)));
     if (lg < 6) lg = 6;
     ht->cap   = 1ull << lg;
     ht->shift = 64 - lg;
+    ht->rmask = ht->cap - 1;
     ht->slots = DevBuf(&ctx_.heap(), ht->cap * ht->words * sizeof(uint64_t));
-    DevBuf dup(&ctx_.heap(), sizeof(uint32_t));
-    CGJ_HIP(hipMemsetAsync(dup.get(), 0, sizeof(uint32_t), ctx_.stream()));
-    CGJ_HIP(launch_fill_u64(ht->slots.get<uint64_t>(), ht->cap * ht->words, kEmpty, ctx_.stream()));
+    DevBuf flags(&ctx_.heap(), 4 * sizeof(uint32_t));
+    CGJ_HIP(hipMemsetAsync(flags.get(), 0, 4 * sizeof(uint32_t), ctx_.stream()));
     if (tn.bloom_bits) {
-        uint64_t words = std::max<uint64_t>(1, staged.count * tn.bloom_bits / 64);
+        uint64_t words = std::max<uint64_t>(1, n * tn.bloom_bits / 64);
         ht->bloom_log  = log2_ceil(words);
         ht->bloom      = DevBuf(&ctx_.heap(), (1ull << ht->bloom_log) * sizeof(uint64_t));
         ht->has_bloom  = true;
-        CGJ_HIP(launch_fill_u64(ht->bloom.get<uint64_t>(), 1ull << ht->bloom_log, 0, ctx_.stream()));
     }
+    uint64_t* bloom = ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr;
     CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
-    CGJ_HIP(launch_ht_insert(staged.stage.get<uint64_t>(), staged.count, ht->words, ht->slots.get<uint64_t>(),
-                             ht->shift, ht->cap - 1, ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr,
-                             ht->bloom_log, dup.get<uint32_t>(), ctx_.stream()));
+    if (bloom) CGJ_HIP(launch_fill_u64(bloom, 1ull << ht->bloom_log, 0, ctx_.stream()));
+
+    // Region size: the largest power of two whose entries fit a 64 KiB LDS table.
+    uint32_t rs_log = 12;
+    while (rs_log > 4 && (uint64_t(1) << rs_log) * ht->words * 8 > 65536) --rs_log;
+    bool partitioned = tn.partitioned && n >= 65536 && lg > rs_log && lg - rs_log <= 14;
+    bool done        = false;
+    if (partitioned) {
+        uint32_t log_regions = lg - rs_log;
+        uint64_t nregions    = 1ull << log_regions;
+        uint32_t blocks      = uint32_t(std::min<uint64_t>(512, (n + 16383) / 16384));
+        uint64_t chunk       = (n + blocks - 1) / blocks;
+        DevBuf   bh(&ctx_.heap(), nregions * blocks * sizeof(uint32_t));
+        DevBuf   scratch(&ctx_.heap(), scan_scratch_words(nregions * blocks) * sizeof(uint32_t));
+        DevBuf   total(&ctx_.heap(), sizeof(uint64_t));
+        DevBuf   parts(&ctx_.heap(), n * ht->words * sizeof(uint64_t));
+        CGJ_HIP(launch_part_build(staged.stage.get<uint64_t>(), n, ht->words, parts.get<uint64_t>(), bh.get<uint32_t>(),
+                                  blocks, chunk, scratch.get<uint32_t>(), total.get<uint64_t>(),
+                                  ht->slots.get<uint64_t>(), lg, log_regions, bloom, ht->bloom_log,
+                                  flags.get<uint32_t>(), ctx_.stream()));
+        uint32_t f[4];
+        CGJ_HIP(hipMemcpyAsync(f, flags.get(), sizeof(f), hipMemcpyDeviceToHost, ctx_.stream()));
+        CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
+        if (!f[1]) {
+            done       = true;
+            ht->rmask  = (1ull << rs_log) - 1;
+            ht->unique = f[0] == 0;
+        } else {
+            ++retries; // a region overflowed (heavy duplicates): use global inserts
+            CGJ_HIP(hipMemsetAsync(flags.get(), 0, 4 * sizeof(uint32_t), ctx_.stream()));
+            if (bloom) CGJ_HIP(launch_fill_u64(bloom, 1ull << ht->bloom_log, 0, ctx_.stream()));
+        }
+    }
+    if (!done) {
+        CGJ_HIP(launch_fill_u64(ht->slots.get<uint64_t>(), ht->cap * ht->words, kEmpty, ctx_.stream()));
+        CGJ_HIP(launch_ht_insert(staged.stage.get<uint64_t>(), n, ht->words, ht->slots.get<uint64_t>(), ht->shift,
+                                 ht->cap - 1, bloom, ht->bloom_log, flags.get<uint32_t>(), ctx_.stream()));
+        uint32_t f[4];
+        CGJ_HIP(hipMemcpyAsync(f, flags.get(), sizeof(f), hipMemcpyDeviceToHost, ctx_.stream()));
+        CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
+        ht->unique = f[0] == 0;
+    }
     CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
-    uint32_t dflag = 0;
-    CGJ_HIP(hipMemcpyAsync(&dflag, dup.get(), sizeof(dflag), hipMemcpyDeviceToHost, ctx_.stream()));
-    CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
-    float f = 0;
-    CGJ_HIP(hipEventElapsedTime(&f, ev0_, ev1_));
-    ms_build += f;
-    ht->unique = dflag == 0;
-    build_tuples += staged.count;
+    CGJ_HIP(hipEventSynchronize(ev1_));
+    float fms = 0;
+    CGJ_HIP(hipEventElapsedTime(&fms, ev0_, ev1_));
+    ms_build += fms;
+    build_tuples += n;
+    build_bytes += n * ht->words * sizeof(uint64_t);
     staged.stage.reset();
     tables_.push_back(std::move(ht));
     return tables_.back().get();
@@ -505,6 +546,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
     uint32_t                     nslots = 0;
     uint64_t                     nrows  = 0;
     std::vector<uint32_t>        scan_eqs;
+    std::vector<uint64_t>        scan_cols;
     if (d.rel >= 0) {
         a.driver = DRIVER_PAGES;
         scan_eqs = q_.needed(1ull << d.rel);
@@ -514,6 +556,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             uint32_t attr = q_.attr_of(uint32_t(d.rel), e);
             uint64_t col  = q_.attrs[attr].col;
             a.scan[a.nscan++] = column(uint32_t(d.rel), col).view(col_id(uint32_t(d.rel), col));
+            scan_cols.push_back(col);
             slot[e]           = nslots++;
         }
     } else {
@@ -537,7 +580,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         P.ht        = ht->slots.get<uint64_t>();
         P.shift     = ht->shift;
         P.words     = ht->words;
-        P.mask      = ht->cap - 1;
+        P.mask      = ht->rmask;
         P.key_slot  = ks->second;
         P.unique    = ht->unique ? 1 : 0;
         P.bloom     = ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr;
@@ -628,6 +671,19 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         if (!c.overflow) {
             res->nrows       = c.tuples;
             res->total_pages = c.dir;
+            res->pool_pages  = pages;
+            ms_probe_kernel  = last_ms_;
+            // output bytes at the reference's packing (TableTarget.cpp:111-182)
+            for (uint32_t i = 0; i < a.nout; ++i) {
+                int32_t ty = a.out[i].type;
+                if (ty == CGJ_VARCHAR) {
+                    out_ref_pages += c.col_pages[i];
+                    continue;
+                }
+                uint64_t w      = ty == CGJ_INT32 ? 32 : 64;
+                uint64_t budget = 8 * (kPageSize - 4 - (ty == CGJ_INT32 ? 0 : 4));
+                out_ref_pages += (c.tuples + w * c.nnz[i] + budget - 1) / budget;
+            }
             break;
         }
         if (attempt > 2) throw std::runtime_error("output page pool overflow persists");
@@ -635,27 +691,13 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         pages = std::max<uint64_t>(c.pages, c.dir) + 64;
     }
     probe_rows += nrows;
-    // order pages: by column, then wave stream, then sequence
-    std::vector<uint32_t> dir(res->total_pages * 4);
-    if (!dir.empty())
-        CGJ_HIP(hipMemcpy(dir.data(), a.dir, dir.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    std::vector<uint64_t> order(res->total_pages);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
-        const uint32_t* a0 = &dir[4 * x];
-        const uint32_t* b0 = &dir[4 * y];
-        if (a0[0] != b0[0]) return a0[0] < b0[0];
-        if (a0[1] != b0[1]) return a0[1] < b0[1];
-        return a0[2] < b0[2];
-    });
-    res->pages.assign(a.nout, {});
-    for (uint64_t i : order) res->pages[dir[4 * i]].push_back(dir[4 * i + 3]);
+    if (d.rel >= 0)
+        for (uint32_t i = 0; i < a.nscan; ++i) probe_input_pages += column(uint32_t(d.rel), scan_cols[i]).npages;
 }
 
 std::unique_ptr<DevResult> Executor::run() {
     auto res = std::make_unique<DevResult>();
     for (uint32_t a : q_.out) res->types.push_back(q_.attrs[a].type);
-    res->pages.assign(q_.out.size(), {});
     for (auto& r : q_.rels)
         if (r.rows == 0) return res; // an empty input empties the inner join
     register_all_columns();
@@ -674,10 +716,7 @@ std::unique_ptr<DevResult> Executor::run() {
         auto            sh = q_.shared(bn.set, cur);
         if (sh.empty()) throw std::runtime_error("join without a predicate");
         const HashTable* ht = build(jn.build, sh.front());
-        if (!ht) {
-            res->pool.reset();
-            return res; // empty build side
-        }
+        if (!ht) return res; // empty build side
         probes.push_back(ProbeStep{ht, sh.front(), bn.set});
         cur |= bn.set;
     }
@@ -707,22 +746,26 @@ std::unique_ptr<DevResult> Context::execute(const cgj_plan& plan, const std::vec
                 throw std::runtime_error("plan column type differs from the input column type");
     Executor ex(*this, q, tree, inputs);
     auto     res = ex.run();
-    stats               = cgj_stats{};
-    stats.ms_build      = ex.ms_build;
-    stats.ms_probe      = ex.ms_probe;
-    stats.ms_total      = ex.ms_build + ex.ms_probe;
-    stats.build_tuples  = ex.build_tuples;
-    stats.probe_rows    = ex.probe_rows;
-    stats.output_rows   = res->nrows;
-    stats.output_pages  = res->total_pages;
-    stats.pipelines     = ex.launches;
-    stats.retries       = ex.retries;
-    uint64_t in_pages   = 0;
-    for (auto& r : q.rels)
-        for (uint32_t e : q.needed(1ull << (&r - &q.rels[0]))) {
-            uint32_t a = q.attr_of(uint32_t(&r - &q.rels[0]), e);
-            if (r.rows) in_pages += inputs[r.table]->cols[q.attrs[a].col].npages;
-        }
+    stats                   = cgj_stats{};
+    stats.ms_build          = ex.ms_build;
+    stats.ms_probe          = ex.ms_probe;
+    stats.ms_total          = ex.ms_build + ex.ms_probe;
+    stats.ms_probe_kernel   = ex.ms_probe_kernel;
+    stats.build_tuples      = ex.build_tuples;
+    stats.build_bytes       = ex.build_bytes;
+    stats.probe_rows        = ex.probe_rows;
+    stats.probe_input_pages = ex.probe_input_pages;
+    stats.output_rows       = res->nrows;
+    stats.output_pages      = res->total_pages;
+    stats.output_ref_pages  = ex.out_ref_pages;
+    stats.pipelines         = ex.launches;
+    stats.retries           = ex.retries;
+    uint64_t in_pages       = 0;
+    for (uint32_t ri = 0; ri < q.rels.size(); ++ri) {
+        const QRel& r = q.rels[ri];
+        if (!r.rows) continue;
+        for (uint32_t e : q.needed(1ull << ri)) in_pages += inputs[r.table]->cols[q.attrs[q.attr_of(ri, e)].col].npages;
+    }
     stats.input_pages = in_pages;
     return res;
 }
@@ -734,25 +777,44 @@ void Context::download(const DevResult& r, cgj_result& out) {
     for (size_t c = 0; c < r.types.size(); ++c) {
         out.columns[c].type      = r.types[c];
         out.columns[c].num_pages = 0;
-        out.columns[c].pages     = nullptr;
+        out.columns[c].pages     = new void*[1];
     }
     if (r.total_pages == 0) return;
+    // page order per column: by wave stream, then sequence (directory entries)
+    std::vector<uint32_t> dir(r.total_pages * 4);
+    CGJ_HIP(hipMemcpyAsync(dir.data(), r.dir(), dir.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    CGJ_HIP(hipStreamSynchronize(stream_));
+    std::vector<uint64_t> order(r.total_pages);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
+        const uint32_t* a0 = &dir[4 * x];
+        const uint32_t* b0 = &dir[4 * y];
+        if (a0[0] != b0[0]) return a0[0] < b0[0];
+        if (a0[1] != b0[1]) return a0[1] < b0[1];
+        return a0[2] < b0[2];
+    });
+    std::vector<std::vector<uint32_t>> pages(r.types.size());
+    for (uint64_t i : order) {
+        if (dir[4 * i] >= pages.size()) throw std::runtime_error("corrupt page directory");
+        pages[dir[4 * i]].push_back(dir[4 * i + 3]);
+    }
     uint64_t used = 0;
-    for (auto& col : r.pages)
+    for (auto& col : pages)
         for (uint32_t p : col) used = std::max<uint64_t>(used, uint64_t(p) + 1);
     const uint64_t kChunkPages = 8192; // 64 MiB staging
     void*          pinned      = nullptr;
     CGJ_HIP(hipHostMalloc(&pinned, kChunkPages * kPageSize, hipHostMallocDefault));
     try {
         for (size_t c = 0; c < r.types.size(); ++c) {
-            out.columns[c].num_pages = r.pages[c].size();
-            out.columns[c].pages     = new void*[r.pages[c].size() ? r.pages[c].size() : 1];
-            for (size_t i = 0; i < r.pages[c].size(); ++i) out.columns[c].pages[i] = nullptr;
+            delete[] out.columns[c].pages;
+            out.columns[c].pages     = new void*[pages[c].size() ? pages[c].size() : 1];
+            out.columns[c].num_pages = pages[c].size();
+            for (size_t i = 0; i < pages[c].size(); ++i) out.columns[c].pages[i] = nullptr;
         }
         // copy the pool in chunks and scatter pages to their columns
         std::vector<std::pair<uint32_t, uint32_t>> where(used, {~0u, ~0u});
         for (size_t c = 0; c < r.types.size(); ++c)
-            for (size_t i = 0; i < r.pages[c].size(); ++i) where[r.pages[c][i]] = {uint32_t(c), uint32_t(i)};
+            for (size_t i = 0; i < pages[c].size(); ++i) where[pages[c][i]] = {uint32_t(c), uint32_t(i)};
         for (uint64_t p0 = 0; p0 < used; p0 += kChunkPages) {
             uint64_t n = std::min(kChunkPages, used - p0);
             CGJ_HIP(hipMemcpyAsync(pinned, r.pool.get<uint8_t>() + p0 * kPageSize, n * kPageSize,

@@ -95,22 +95,27 @@ struct DevTable {
     std::vector<DevColumn> cols;
 };
 
-// Output pages of a query, still in HBM: `pool` holds the pages, `pages`
-// lists per column the pool page indices in row order.
+// Output pages of a query, still in HBM: `pool` holds `pool_pages` page slots
+// followed by the page directory (column, wave stream, sequence, slot) of the
+// `total_pages` pages written. The per-column page order is derived from the
+// directory when the result is downloaded.
 struct DevResult {
-    uint64_t                           nrows = 0;
-    std::vector<int32_t>               types;
-    std::vector<std::vector<uint32_t>> pages;
-    uint64_t                           total_pages = 0;
-    DevBuf                             pool;
+    uint64_t             nrows       = 0;
+    std::vector<int32_t> types;
+    uint64_t             total_pages = 0;
+    uint64_t             pool_pages  = 0;
+    DevBuf               pool;
+
+    const uint32_t* dir() const { return reinterpret_cast<const uint32_t*>(pool.get<uint8_t>() + pool_pages * 8192); }
 };
 
 class Context;
 
 // Tunables of one hash join, chosen per (build rows, probe rows, key width).
 struct JoinTuning {
-    uint32_t load_log   = 1; // capacity = next_pow2(build) << load_log
-    uint32_t bloom_bits = 0; // bloom bits per key (0 = no bloom filter)
+    double   max_load    = 0.75; // capacity = next_pow2(build / max_load)
+    uint32_t bloom_bits  = 0;    // bloom bits per key (0 = no bloom filter)
+    bool     partitioned = true; // radix-partitioned LDS build (else global CAS inserts)
 };
 
 class Context {

@@ -110,6 +110,23 @@ __global__ void k_morsel_index(const uint8_t* pages, uint64_t npages, int32_t ty
 
 size_t prepare_scratch_words(uint64_t npages) { return npages + (npages + kScanBlock - 1) / kScanBlock + 2; }
 
+// In-place exclusive scan of n u32 values; `sums` holds ceil(n/1024) words.
+__global__ void k_scan_finish(uint32_t* data, uint64_t n, const uint32_t* sums) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) data[i] += sums[i / kScanBlock];
+}
+
+size_t scan_scratch_words(uint64_t n) { return (n + kScanBlock - 1) / kScanBlock + 2; }
+
+hipError_t launch_exclusive_scan(uint32_t* data, uint64_t n, uint32_t* scratch, uint64_t* total, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t nblocks = (n + kScanBlock - 1) / kScanBlock;
+    k_scan_local<<<dim3(nblocks), dim3(kScanBlock), 0, s>>>(data, n, scratch);
+    k_scan_sums<<<dim3(1), dim3(kScanBlock), 0, s>>>(scratch, nblocks, total);
+    k_scan_finish<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(data, n, scratch);
+    return hipGetLastError();
+}
+
 hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t type, uint32_t* row_start,
                                  uint32_t* idx_page, uint32_t* idx_nn, uint32_t* scratch,
                                  uint64_t* total_rows_dev, hipStream_t s) {
@@ -199,6 +216,113 @@ __global__ void k_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, u
     }
 }
 
+// ---------------------------------------------------------------------------
+// radix-partitioned build: the table is split into F regions of S slots (the
+// top bits of a key's slot position select its region, and linear probing
+// wraps inside the region), so each region can be built in LDS by one block
+// and written out with full-line stores.
+// ---------------------------------------------------------------------------
+
+// Per-block histogram of region ids over a contiguous chunk of staged tuples;
+// stored region-major (bh[region * blocks + block]) so that one exclusive scan
+// yields every block's write offset in every region.
+__global__ __launch_bounds__(256) void k_part_hist(const uint64_t* stage, uint64_t n, uint32_t words,
+                                                   uint32_t rshift, uint32_t nregions, uint64_t chunk,
+                                                   uint32_t* bh) {
+    extern __shared__ uint32_t hist[];
+    for (uint32_t i = threadIdx.x; i < nregions; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    uint64_t begin = uint64_t(blockIdx.x) * chunk;
+    uint64_t end   = begin + chunk < n ? begin + chunk : n;
+    for (uint64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+        uint32_t r = uint32_t(hash_key(stage[i * words]) >> rshift);
+        atomicAdd(&hist[r], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nregions; i += blockDim.x) bh[uint64_t(i) * gridDim.x + blockIdx.x] = hist[i];
+}
+
+// Scatters the same chunk into region order using the scanned offsets.
+__global__ __launch_bounds__(256) void k_part_scatter(const uint64_t* stage, uint64_t n, uint32_t words,
+                                                      uint32_t rshift, uint32_t nregions, uint64_t chunk,
+                                                      const uint32_t* offs, uint64_t* out) {
+    extern __shared__ uint32_t cur[];
+    for (uint32_t i = threadIdx.x; i < nregions; i += blockDim.x) cur[i] = offs[uint64_t(i) * gridDim.x + blockIdx.x];
+    __syncthreads();
+    uint64_t begin = uint64_t(blockIdx.x) * chunk;
+    uint64_t end   = begin + chunk < n ? begin + chunk : n;
+    for (uint64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+        const uint64_t* t   = stage + i * words;
+        uint32_t        r   = uint32_t(hash_key(t[0]) >> rshift);
+        uint32_t        pos = atomicAdd(&cur[r], 1u);
+        uint64_t*       d   = out + uint64_t(pos) * words;
+        for (uint32_t w = 0; w < words; ++w) d[w] = t[w];
+    }
+}
+
+// One block per region: inserts the region's tuples into an LDS table of
+// `rslots` slots (linear probing inside the region), then stores the region.
+// flags[0] = duplicate keys seen, flags[1] = a region was too full.
+__global__ __launch_bounds__(256) void k_part_build(const uint64_t* parts, const uint32_t* offs, uint32_t blocks,
+                                                    uint64_t n, uint32_t nregions, uint32_t words,
+                                                    uint32_t rslots, uint32_t shift, uint64_t* ht,
+                                                    uint64_t* bloom, uint32_t bloom_log, uint32_t* flags) {
+    extern __shared__ uint64_t tab[];
+    uint32_t r     = blockIdx.x;
+    uint64_t start = offs[uint64_t(r) * blocks];
+    uint64_t end   = r + 1 < nregions ? offs[uint64_t(r + 1) * blocks] : n;
+    uint64_t cnt   = end - start;
+    uint64_t* dst  = ht + uint64_t(r) * rslots * words;
+    if (cnt >= rslots) { // a full region would leave probes without an empty slot
+        if (threadIdx.x == 0) flags[1] = 1;
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < rslots * words; i += blockDim.x) tab[i] = kEmpty;
+    __syncthreads();
+    bool     dup   = false;
+    uint32_t rmask = rslots - 1;
+    for (uint64_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint64_t* t   = parts + i * words;
+        uint64_t        key = t[0];
+        uint64_t        h   = hash_key(key);
+        uint32_t        pos = uint32_t(h >> shift) & rmask;
+        for (;;) {
+            unsigned long long prev =
+                atomicCAS(reinterpret_cast<unsigned long long*>(tab + uint64_t(pos) * words), kEmpty, key);
+            if (prev == kEmpty) break;
+            dup |= prev == key;
+            pos = (pos + 1) & rmask;
+        }
+        for (uint32_t w = 1; w < words; ++w) tab[uint64_t(pos) * words + w] = t[w];
+        if (bloom) {
+            uint64_t h2 = hash_bloom(h);
+            atomicOr(reinterpret_cast<unsigned long long*>(bloom + bloom_word(h2, bloom_log)), bloom_bits(h2));
+        }
+    }
+    if (dup) flags[0] = 1;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < rslots * words; i += blockDim.x) dst[i] = tab[i];
+}
+
+hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* parts, uint32_t* bh,
+                             uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch, uint64_t* total,
+                             uint64_t* ht, uint32_t log_cap, uint32_t log_regions, uint64_t* bloom,
+                             uint32_t bloom_log, uint32_t* flags, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint32_t nregions = 1u << log_regions;
+    uint32_t rslots   = 1u << (log_cap - log_regions);
+    uint32_t rshift   = 64 - log_regions;
+    size_t   hist_lds = size_t(nregions) * sizeof(uint32_t);
+    k_part_hist<<<dim3(blocks), dim3(256), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh);
+    hipError_t e = launch_exclusive_scan(bh, uint64_t(nregions) * blocks, scan_scratch, total, s);
+    if (e != hipSuccess) return e;
+    k_part_scatter<<<dim3(blocks), dim3(256), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh, parts);
+    size_t tab_lds = size_t(rslots) * words * sizeof(uint64_t);
+    k_part_build<<<dim3(nregions), dim3(256), tab_lds, s>>>(parts, bh, blocks, n, nregions, words, rslots,
+                                                            64 - log_cap, ht, bloom, bloom_log, flags);
+    return hipGetLastError();
+}
+
 hipError_t launch_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,
                             uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* dup_flag,
                             hipStream_t s) {
@@ -275,6 +399,8 @@ __device__ inline void page_close(const PipeArgs& a, OutState& st, uint64_t* bit
             if (b0 + b < nb) bm[b0 + b] = uint8_t(word >> (8 * b));
         if (lane_id() == 0) {
             *reinterpret_cast<uint32_t*>(page) = st.nr | (st.nv << 16);
+            atomicAdd(&a.counters->nnz[col], (unsigned long long)st.nv);
+            atomicAdd(&a.counters->col_pages[col], 1ull);
             unsigned long long d = atomicAdd(&a.counters->dir, 1ull);
             if (d < a.dir_cap) {
                 uint32_t* e = a.dir + 4 * d;
@@ -422,6 +548,7 @@ __device__ void append_varchar(const PipeArgs& a, OutState* stp, uint64_t* bits,
                 }
                 // record the copied page as-is (header already in place)
                 if (!st.bad && lane_id() == 0) {
+                    atomicAdd(&a.counters->col_pages[col], 1ull);
                     unsigned long long d = atomicAdd(&a.counters->dir, 1ull);
                     if (d < a.dir_cap) {
                         uint32_t* e = a.dir + 4 * d;
@@ -539,13 +666,13 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                                             break;
                                         }
                                     }
-                                    p = (p + 1) & P.mask;
+                                    p = (p & ~P.mask) | ((p + 1) & P.mask);
                                 }
                             }
                             if (found) {
                                 const uint64_t* e = P.ht + p * P.words;
                                 for (uint32_t t = 0; t < P.nattrs; ++t) sset(s, P.attr_slot[t], e[P.attr_word[t]]);
-                                pos[k] = P.unique ? ~0ull : ((p + 1) & P.mask);
+                                pos[k] = P.unique ? ~0ull : ((p & ~P.mask) | ((p + 1) & P.mask));
                                 if (uint32_t(k) + 1 == a.nprobes) {
                                     have = true;
                                 } else if (k + 1 < kMaxProbes) {

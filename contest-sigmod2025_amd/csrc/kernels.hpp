@@ -18,7 +18,7 @@ struct ProbeSpec {
     const uint64_t* ht;        // cap x words entries; word 0 is the key
     uint32_t        shift;     // slot = hash_key(key) >> shift
     uint32_t        words;     // words per entry
-    uint64_t        mask;      // cap - 1
+    uint64_t        mask;      // region slots - 1: linear probing wraps inside a region
     uint32_t        key_slot;  // tuple slot holding the probe key
     uint32_t        unique;    // keys are unique: stop at the first match
     uint32_t        nattrs;    // entry words copied into tuple slots
@@ -48,6 +48,8 @@ struct PipeCounters {
     unsigned long long pages;     // output pages allocated from the pool
     unsigned long long dir;       // directory entries written
     unsigned long long overflow;  // nonzero: a buffer was too small, re-run
+    unsigned long long nnz[kMaxOut];       // non-NULL values written per output column
+    unsigned long long col_pages[kMaxOut]; // pages written per output column
 };
 
 struct PipeArgs {
@@ -95,6 +97,15 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 hipError_t launch_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,
                             uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* dup_flag,
                             hipStream_t s);
+size_t     scan_scratch_words(uint64_t n);
+hipError_t launch_exclusive_scan(uint32_t* data, uint64_t n, uint32_t* scratch, uint64_t* total, hipStream_t s);
+// Radix-partitioned build of a table of 2^log_cap slots split into
+// 2^log_regions regions; `bh` holds blocks << log_regions counters, `parts`
+// n x words tuples. flags[0] = duplicate keys, flags[1] = a region overflowed.
+hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* parts, uint32_t* bh,
+                             uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch, uint64_t* total,
+                             uint64_t* ht, uint32_t log_cap, uint32_t log_regions, uint64_t* bloom,
+                             uint32_t bloom_log, uint32_t* flags, hipStream_t s);
 // Launches the fused pipeline. `slots` = tuple width (selects the kernel variant).
 hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s);
 uint32_t   pipeline_waves_per_block();

@@ -31,6 +31,7 @@ GEN_UNIFORM = 2
 GEN_UNIT_F64 = 3
 GEN_PERM_KEY = 4
 GEN_FK_UNIFORM = 5
+GEN_ASCII = 6
 
 
 class HostColumn(C.Structure):
@@ -83,6 +84,10 @@ class Stats(C.Structure):
         ("output_rows", C.c_uint64),
         ("pipelines", C.c_uint32),
         ("retries", C.c_uint32),
+        ("probe_input_pages", C.c_uint64),
+        ("output_ref_pages", C.c_uint64),
+        ("build_bytes", C.c_uint64),
+        ("ms_probe_kernel", C.c_double),
     ]
 
     def as_dict(self):

@@ -1,0 +1,130 @@
+"""Synthetic join workloads (seeded, generated directly in HBM or on the host).
+
+Every column draws from its own counter-based SplitMix64 stream
+(seed = 0x5EED0000 + config number), so the device generator and the host
+generator produce the same rows and any row range can be generated alone.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Callable, List
+
+from . import (FP64, GEN_ASCII, GEN_FK_UNIFORM, GEN_FK_ZIPF, GEN_PERM_KEY, GEN_UNIFORM, GEN_UNIQUE_KEY,
+               GEN_UNIT_F64, INT32, INT64, VARCHAR, ColumnarTable, GenColumn, Plan, host_generate)
+
+
+@dataclass
+class TableSpec:
+    name: str
+    rows: int
+    columns: List[GenColumn]
+    host: bool = False  # generated on the host (VARCHAR) and uploaded
+
+    def host_table(self, seed) -> ColumnarTable:
+        return ColumnarTable(self.rows, [host_generate(seed, 0, self.rows, c) for c in self.columns])
+
+
+@dataclass
+class Workload:
+    name: str
+    seed: int
+    tables: List[TableSpec]
+    make_plan: Callable[[], Plan]
+    description: str
+
+
+def c1(scale: float = 1.0) -> Workload:
+    """1M-row INT32 build (shuffled unique keys + INT32 payload) joined with a
+    10M-row probe (uniform FK + INT64 payload); no NULLs; output
+    {build payload, probe payload}."""
+    nb = max(int(1_000_000 * scale), 1)
+    npr = max(int(10_000_000 * scale), 1)
+    build = TableSpec("build", nb, [
+        GenColumn(type=INT32, kind=GEN_PERM_KEY, stream=1, domain=nb),
+        GenColumn(type=INT32, kind=GEN_UNIFORM, stream=2, lo=0, hi=1 << 31),
+    ])
+    probe = TableSpec("probe", npr, [
+        GenColumn(type=INT32, kind=GEN_FK_UNIFORM, stream=1, domain=nb),
+        GenColumn(type=INT64, kind=GEN_UNIFORM, stream=4, lo=0, hi=1 << 63),
+    ])
+
+    def plan():
+        p = Plan()
+        b = p.new_scan_node(0, [(0, INT32), (1, INT32)])
+        q = p.new_scan_node(1, [(0, INT32), (1, INT64)])
+        p.root = p.new_join_node(True, b, q, 0, 0, [(1, INT32), (3, INT64)])
+        return p
+
+    return Workload("c1", 0x5EED0001, [build, probe], plan,
+                    f"INT32 equi-join, build {nb} unique keys, probe {npr} uniform FKs, no NULLs")
+
+
+def c2(scale: float = 1.0) -> Workload:
+    """50M-row INT64 build (unique shuffled keys, FP64 payload) joined with a
+    500M-row probe (90% Zipf(1.0)-over-rank FKs, 10% misses from a disjoint
+    range, FP64 payload); 5% Bernoulli NULLs in every column; output
+    {build payload, probe payload, probe key}."""
+    nb = max(int(50_000_000 * scale), 1)
+    npr = max(int(500_000_000 * scale), 1)
+    build = TableSpec("build", nb, [
+        GenColumn(type=INT64, kind=GEN_UNIQUE_KEY, stream=1, null_stream=11, null_frac=0.05, domain=nb),
+        GenColumn(type=FP64, kind=GEN_UNIT_F64, stream=2, null_stream=12, null_frac=0.05),
+    ])
+    probe = TableSpec("probe", npr, [
+        GenColumn(type=INT64, kind=GEN_FK_ZIPF, stream=1, null_stream=13, null_frac=0.05, domain=nb, zipf_s=1.0,
+                  miss_frac=0.1),
+        GenColumn(type=FP64, kind=GEN_UNIT_F64, stream=4, null_stream=14, null_frac=0.05),
+    ])
+
+    def plan():
+        p = Plan()
+        b = p.new_scan_node(0, [(0, INT64), (1, FP64)])
+        q = p.new_scan_node(1, [(0, INT64), (1, FP64)])
+        p.root = p.new_join_node(True, b, q, 0, 0, [(1, FP64), (3, FP64), (2, INT64)])
+        return p
+
+    return Workload("c2", 0x5EED0002, [build, probe], plan,
+                    f"INT64 equi-join, build {nb} unique keys, probe {npr} Zipf(1.0) FKs + 10% misses, 5% NULLs")
+
+
+def c3(scale: float = 1.0) -> Workload:
+    """Five-table snowflake: a 36M-row fact table with four INT32 foreign keys
+    (Zipf s=0.8 over key rank) into dimensions of 2.5M, 4M, 1.5M and 0.3M keys
+    that arrive pre-filtered to 1%, 5%, 20% and 100% of their rows; the last
+    dimension carries a VARCHAR column (ASCII, 5-60 bytes, 0.1% long strings).
+    Output {dimension VARCHAR, fact FP64}; all four joins fuse into one probe
+    chain over the fact table."""
+    nf = max(int(36_000_000 * scale), 1)
+    dims = [(2_500_000, 0.01), (4_000_000, 0.05), (1_500_000, 0.20), (300_000, 1.0)]
+    fact_cols = []
+    tables = []
+    for i, (n, sel) in enumerate(dims):
+        dn = max(int(n * scale), 16)
+        rows = max(int(dn * sel), 1)
+        cols = [GenColumn(type=INT32, kind=GEN_PERM_KEY, stream=31 + i, domain=dn)]
+        if i == 3:
+            cols.append(GenColumn(type=VARCHAR, kind=GEN_ASCII, stream=35, lo=5, hi=60, miss_frac=0.001))
+        tables.append(TableSpec(f"dim{i + 1}", rows, cols, host=True))
+        # the fact side ranks keys with its own permutation, so the hot keys are
+        # independent of which keys survived the dimension's filter
+        fact_cols.append(GenColumn(type=INT32, kind=GEN_FK_ZIPF, stream=21 + i, domain=dn, zipf_s=0.8))
+    fact_cols.append(GenColumn(type=FP64, kind=GEN_UNIT_F64, stream=25))
+    tables.insert(0, TableSpec("fact", nf, fact_cols))
+
+    def plan():
+        p = Plan()
+        f = p.new_scan_node(0, [(i, INT32) for i in range(4)] + [(4, FP64)])
+        d = [p.new_scan_node(1 + i, [(0, INT32)]) for i in range(3)]
+        d4 = p.new_scan_node(4, [(0, INT32), (1, VARCHAR)])
+        keep = [(i, INT32) for i in range(4)] + [(4, FP64)]
+        cur = f
+        for i in range(3):
+            cur = p.new_join_node(False, cur, d[i], i, 0, keep)
+        p.root = p.new_join_node(False, cur, d4, 3, 0, [(6, VARCHAR), (4, FP64)])
+        return p
+
+    return Workload("c3", 0x5EED0003, tables, plan,
+                    f"snowflake: fact {nf} rows x 4 pre-filtered dimensions, fused 4-probe chain with bloom filters")
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3}

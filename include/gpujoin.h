@@ -107,6 +107,12 @@ typedef struct {
     uint64_t output_rows;
     uint32_t pipelines;      /* number of pipeline launches                    */
     uint32_t retries;        /* re-runs after a page-pool or staging overflow  */
+    uint64_t probe_input_pages; /* input pages scanned by the output pipeline  */
+    uint64_t output_ref_pages;  /* output pages at the reference's packing:
+                                   ceil((rows + width*nonnull)/bit budget) per
+                                   fixed-width column, pages written otherwise */
+    uint64_t build_bytes;       /* compact hash-table tuples: tuples x words x 8 */
+    double   ms_probe_kernel;   /* HIP-event time of the output pipeline kernel */
 } cgj_stats;
 
 /* ---- context (reference engine/Execute.cpp:42-48) ---- */
@@ -189,6 +195,8 @@ enum {
     CGJ_GEN_UNIT_F64   = 3, /* double uniform in [0, 1)                                  */
     CGJ_GEN_PERM_KEY   = 4, /* INT32 key: permutation of 0..domain-1 (r < domain)        */
     CGJ_GEN_FK_UNIFORM = 5, /* key = permutation value of a uniform rank, or a miss key  */
+    CGJ_GEN_ASCII      = 6, /* VARCHAR, host only: length uniform in [lo, hi], a fraction
+                               miss_frac of rows gets a long string of 8186..16000 bytes */
 };
 
 /* Generates a table of `num_rows` rows directly in HBM, paged exactly as the

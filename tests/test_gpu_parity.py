@@ -124,3 +124,71 @@ def test_device_generator_matches_host(ctx):
         out = torch.empty(n, dtype=torch.int64, device="cuda")
         dt.decode(ci, out.data_ptr())
         assert np.array_equal(out.cpu().numpy().view(np.uint64), oracle.decode_fixed(host, n))
+
+
+def _join_plan(build_cols, probe_cols, bkey, pkey, out):
+    plan = gj.Plan()
+    nb = len(build_cols[0][1])
+    npr = len(probe_cols[0][1])
+    plan.new_input(gj.make_table(build_cols, nb))
+    plan.new_input(gj.make_table(probe_cols, npr))
+    b = plan.new_scan_node(0, [(i, c[0]) for i, c in enumerate(build_cols)])
+    q = plan.new_scan_node(1, [(i, c[0]) for i, c in enumerate(probe_cols)])
+    plan.root = plan.new_join_node(True, b, q, bkey, pkey, out)
+    return plan
+
+
+def _compare_large(ctx, plan):
+    got = ctx.execute(plan)
+    exp = oracle.execute(plan)
+    assert got.num_rows == exp.num_rows
+    for c in got.columns:
+        assert gj.pages.validate_column(c.pages, c.type) == got.num_rows
+    for a, b_ in zip(sorted_rows(slots_of(got)), sorted_rows(slots_of(exp))):
+        assert np.array_equal(a, b_)
+    return ctx.stats()
+
+
+def test_partitioned_build_unique_int64(ctx):
+    """>= 64K build tuples take the radix-partitioned LDS build."""
+    rng = np.random.default_rng(11)
+    nb, npr = 300_000, 1_000_000
+    keys = rng.permutation(np.arange(nb, dtype=np.int64) * 7919 + 3)
+    bval = rng.random(nb)
+    pk = np.where(rng.random(npr) < 0.9, keys[rng.integers(0, nb, npr)], -keys[rng.integers(0, nb, npr)] - 10)
+    plan = _join_plan([(INT64, keys, rng.random(nb) > 0.05), (FP64, bval, rng.random(nb) > 0.05)],
+                      [(INT64, pk, rng.random(npr) > 0.05), (FP64, rng.random(npr), rng.random(npr) > 0.05)],
+                      0, 0, [(1, FP64), (3, FP64), (2, INT64)])
+    say("partitioned build 300K x 1M")
+    s = _compare_large(ctx, plan)
+    assert s["retries"] == 0, s
+
+
+def test_partitioned_build_wide_entries(ctx):
+    """Eight-word entries shrink the LDS region; results stay exact."""
+    rng = np.random.default_rng(12)
+    nb, npr = 150_000, 400_000
+    keys = rng.permutation(nb).astype(np.int32)
+    cols = [(INT32, keys, None)] + [(INT64, rng.integers(0, 1 << 40, nb), rng.random(nb) > 0.1) for _ in range(7)]
+    pk = rng.integers(0, nb + nb // 10, npr).astype(np.int32)
+    out = [(i, INT64) for i in range(1, 8)] + [(8, INT32)]
+    plan = _join_plan(cols, [(INT32, pk, rng.random(npr) > 0.02)], 0, 0, out)
+    say("wide entries 150K x 400K")
+    _compare_large(ctx, plan)
+
+
+def test_heavy_duplicate_build_falls_back(ctx):
+    """A key repeated 20K times overflows its region: the build falls back to
+    global inserts and the join still expands every duplicate."""
+    rng = np.random.default_rng(13)
+    nb = 100_000
+    keys = rng.integers(0, 50_000, nb).astype(np.int32)
+    keys[:20_000] = 7
+    probe = rng.integers(60_000, 1 << 30, 300_000).astype(np.int32)  # misses
+    probe[[5, 77, 1234]] = 7
+    probe[10:400] = rng.integers(0, 50_000, 390)
+    plan = _join_plan([(INT32, keys, None), (INT32, np.arange(nb, dtype=np.int32), None)],
+                      [(INT32, probe, None)], 0, 0, [(1, INT32), (2, INT32)])
+    say("heavy duplicates")
+    s = _compare_large(ctx, plan)
+    assert s["retries"] >= 1
