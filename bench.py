@@ -107,16 +107,9 @@ def main():
         log(f"warmup {i}: {r.num_rows} rows, {ctx.stats()['ms_total']:.2f} ms device")
         r.free()
 
-    try:
-        import torch
-
-        sync = torch.cuda.synchronize
-    except Exception:  # the library synchronises its own stream at the end of a query
-        def sync():
-            return None
-
+    # execute_device returns after the query's last kernel has completed (the
+    # library synchronises its stream), so the host clock brackets whole queries.
     stats = []
-    sync()
     t_start = time.perf_counter()
     for i in range(args.steps):
         r = ctx.execute_device(plan, tables)
@@ -124,7 +117,6 @@ def main():
         s["rows"] = r.num_rows
         stats.append(s)
         r.free()
-    sync()
     elapsed = time.perf_counter() - t_start
     ms_step = elapsed * 1000.0 / args.steps
     for i, s in enumerate(stats):

@@ -152,11 +152,15 @@ int cgj_last_stats(const cgj_context* ctx, cgj_stats* out) {
     return 0;
 }
 
-int cgj_scan_decode(cgj_context* ctx, const cgj_dtable* t, uint32_t column, uint64_t* d_slots) {
+int cgj_scan_decode(cgj_context* ctx, const cgj_dtable* t, uint32_t column, uint64_t* slots) {
     return guarded(ctx, [&] {
         if (column >= t->impl->cols.size()) throw std::runtime_error("column out of range");
         const cgj::DevColumn& c = t->impl->cols[column];
-        CGJ_HIP(cgj::launch_decode(c.view(0), c.nrows, d_slots, ctx->impl->stream()));
+        if (c.nrows && !c.npages) throw std::runtime_error("column was not uploaded");
+        cgj::DevBuf out(&ctx->impl->heap(), std::max<uint64_t>(c.nrows, 1) * sizeof(uint64_t));
+        CGJ_HIP(cgj::launch_decode(c.view(0), c.nrows, out.get<uint64_t>(), ctx->impl->stream()));
+        CGJ_HIP(hipMemcpyAsync(slots, out.get(), c.nrows * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                               ctx->impl->stream()));
         CGJ_HIP(hipStreamSynchronize(ctx->impl->stream()));
     });
 }

@@ -602,8 +602,10 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         }
     }
     if (nslots > 24) throw std::runtime_error("pipeline carries too many columns");
+    // Scanned join columns must be non-NULL (NULL never matches); slots filled
+    // from hash-table entries need no check, their build pipelines filtered them.
     for (auto& kv : slot)
-        if (q_.is_join_eq(kv.first)) a.not_null |= 1u << kv.second;
+        if (kv.second < scanned && q_.is_join_eq(kv.first)) a.not_null |= 1u << kv.second;
 
     a.target = t.kind;
     DevResult* res = t.result;

@@ -313,10 +313,12 @@ class DeviceTable:
     def num_pages(self, column):
         return lib().cgj_dtable_num_pages(self.handle, column)
 
-    def decode(self, column, device_ptr):
-        """Decodes a column into 64-bit slots at a device pointer (NULL = all ones)."""
-        if lib().cgj_scan_decode(self.ctx.handle, self.handle, column, C.c_void_p(device_ptr)) != 0:
+    def decode(self, column):
+        """Decodes a column on the GPU into uint64 slots (NULL = all ones)."""
+        out = np.zeros(max(self.num_rows, 1), dtype=np.uint64)
+        if lib().cgj_scan_decode(self.ctx.handle, self.handle, column, out.ctypes.data) != 0:
             raise RuntimeError(self.ctx.error())
+        return out[: self.num_rows]
 
     def free(self):
         if self.handle:

@@ -159,8 +159,9 @@ int      cgj_last_stats(const cgj_context* ctx, cgj_stats* out);
 /* Page decode of one device column into 64-bit slots, NULL = all ones, INT32
  * zero-extended, FP64 raw bits, VARCHAR as a string handle. Replaces
  * TableScan::Reader::skipTo/step64 (reference engine/op/TableScan.cpp:82-324).
- * `d_slots` is a device array of at least cgj_dtable_num_rows() entries. */
-int cgj_scan_decode(cgj_context* ctx, const cgj_dtable* t, uint32_t column, uint64_t* d_slots);
+ * The kernel writes the slots to HBM; they are then copied to `slots`, a host
+ * array of at least cgj_dtable_num_rows() entries. */
+int cgj_scan_decode(cgj_context* ctx, const cgj_dtable* t, uint32_t column, uint64_t* slots);
 
 /* Encode host values into contest pages with the contest page writer
  * (reference include/plan.h:151-335). values: int32/int64/double array of
