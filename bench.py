@@ -122,6 +122,11 @@ def main():
     for i, s in enumerate(stats):
         log(f"step {i}: build {s['ms_build']:.2f} ms, probe kernel {s['ms_probe_kernel']:.2f} ms, rows {s['rows']}")
 
+    # achievable streaming bandwidth on this GPU: a plain 2 GiB copy (reads + writes)
+    copy_bytes = 2 << 30
+    copy_ms = ctx.copy_probe(copy_bytes, 5)
+    copy_gbs = 2 * copy_bytes / (copy_ms / 1000.0) / 1e9
+
     s = stats[-1]
     b_alg = (s["input_pages"] + s["output_ref_pages"]) * PAGE + 2 * s["build_bytes"]
     b_kernel = (s["probe_input_pages"] + s["output_ref_pages"]) * PAGE + s["build_bytes"]
@@ -159,6 +164,7 @@ def main():
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "algorithmic_bytes": int(b_kernel),
             "kernel_ms": round(k_ms, 3),
+            "copy_gbs": round(copy_gbs, 1),
         },
     }
     if not args.no_cpu_baseline:

@@ -134,6 +134,28 @@ int cgj_execute_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const
     });
 }
 
+size_t cgj_stats_size(void) { return sizeof(cgj_stats); }
+
+int cgj_set_option(cgj_context* ctx, const char* name, double value) {
+    if (!ctx || !name) return -1;
+    return ctx->impl->set_option(name, value) ? 0 : -1;
+}
+
+int cgj_count_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                     uint64_t* rows) {
+    return guarded(ctx, [&] {
+        std::vector<const cgj::DevTable*> views;
+        for (uint32_t i = 0; i < num_inputs; ++i) views.push_back(inputs[i]->impl.get());
+        *rows = ctx->impl->count(*plan, views);
+    });
+}
+
+double cgj_copy_probe(cgj_context* ctx, uint64_t bytes, uint32_t reps) {
+    double ms = -1.0;
+    guarded(ctx, [&] { ms = ctx->impl->copy_probe(bytes, reps); });
+    return ms;
+}
+
 uint64_t cgj_dresult_num_rows(const cgj_dresult* r) { return r ? r->impl->nrows : 0; }
 uint64_t cgj_dresult_num_pages(const cgj_dresult* r) { return r ? r->impl->total_pages : 0; }
 

@@ -10,7 +10,6 @@ namespace cgj {
 constexpr uint32_t kPageSize   = 8192;
 constexpr uint64_t kNull       = ~0ull; // NULL slot value (reference RuntimeValue.hpp:10)
 constexpr uint64_t kEmpty      = ~0ull; // empty hash-table slot; NULL keys are never inserted
-constexpr uint32_t kIndexRows  = 1024;  // row granularity of the per-column morsel index
 constexpr uint32_t kWave       = 64;
 constexpr uint32_t kMaxPageRows = 4096; // rows per output page: 64 lanes x 64 bitmap bits
 
@@ -80,77 +79,72 @@ __device__ inline uint32_t highest_lane(uint64_t m) { return 63 - __clzll(m); }
 
 // ---- page reader ----
 
+// Decode index of one 64-row block of a column (rows 64b .. 64b+63): the page
+// holding the block's first row, the number of non-NULL values before that row
+// in its page, the validity of the 64 rows, and the first lane that lies on
+// the next page (64 = none). A block that touches more than two pages, or a
+// long-string page, is marked `complex` and decoded row by row.
+struct BlockIdx {
+    uint32_t page;
+    uint16_t nn;
+    uint8_t  split;
+    uint8_t  complex;
+    uint64_t valid;
+};
+
 // One scanned column as the kernels see it.
 struct ScanCol {
     const uint8_t*  pages;     // npages x 8 KiB
     const uint32_t* row_start; // npages + 1 prefix of rows per page
-    const uint32_t* idx_page;  // page holding row k * kIndexRows
-    const uint32_t* idx_nn;    // non-NULL values before that row within its page
+    const BlockIdx* blocks;    // ceil(nrows / 64) block entries
     int32_t         type;
     uint32_t        col_id;    // id used in string handles
 };
 
-// Per-wave decode position: the page of the wave's next row and the number of
-// non-NULL values that precede that row in the page.
-struct Cursor {
-    uint32_t page;
-    uint32_t nn;
-};
-
-__device__ inline Cursor cursor_at(const ScanCol& c, uint64_t row) {
-    uint64_t k = row / kIndexRows;
-    return Cursor{c.idx_page[k], c.idx_nn[k]};
+// Value slot of non-NULL value `nn` of page `p`.
+__device__ inline uint64_t load_value(const ScanCol& c, uint32_t p, uint32_t nn, bool is_long) {
+    const uint8_t* page = c.pages + uint64_t(p) * kPageSize;
+    switch (c.type) {
+    case T_INT32: return *reinterpret_cast<const uint32_t*>(page + 4 + 4 * nn);
+    case T_INT64:
+    case T_FP64: return *reinterpret_cast<const uint64_t*>(page + 8 + 8 * uint64_t(nn));
+    default: return make_handle(c.col_id, p, is_long ? 0xffffu : nn);
+    }
 }
 
-// Decodes row `row` of column `c` for every lane; the active lanes must form
-// a prefix of the wave and hold consecutive rows. Replaces the reference's
-// skipTo/step64 (TableScan.cpp:135-324): the validity of each row comes from
-// its own bitmap bit, and its value index is the ballot-prefix of valid rows
-// of the same page, so no pext/pdep or offset tables are needed.
-__device__ inline uint64_t decode_step(const ScanCol& c, Cursor& cur, uint32_t row, bool active) {
-    uint32_t p = cur.page;
-    if (active) {
-        while (c.row_start[p + 1] <= row) ++p;
-    }
-    uint32_t       j    = active ? row - c.row_start[p] : 1;
+// Row-by-row decode (complex blocks): the page by search over the row prefix,
+// validity from the bitmap, value index by counting bitmap bits.
+__device__ inline uint64_t decode_row_slow(const ScanCol& c, uint32_t p, uint32_t row) {
+    while (c.row_start[p + 1] <= row) ++p;
+    uint32_t       j    = row - c.row_start[p];
     const uint8_t* page = c.pages + uint64_t(p) * kPageSize;
-    uint32_t       hdr  = active ? *reinterpret_cast<const uint32_t*>(page) : 0;
-    uint32_t       nr   = hdr & 0xffff;
-    uint32_t       nv   = hdr >> 16;
-    bool           is_long = c.type == T_VARCHAR && nr == 0xffff;
-    bool           valid   = false;
-    if (active) {
-        if (is_long || nv == nr) {
-            valid = true;
-        } else {
-            uint32_t nb = (nr + 7) >> 3;
-            valid       = (page[kPageSize - nb + (j >> 3)] >> (j & 7)) & 1;
-        }
-    }
-    uint64_t jz     = __ballot(active && j == 0);
-    uint64_t starts = jz | 1ull;
-    uint32_t seg    = highest_lane(starts & mask_le());
-    uint64_t vmask  = __ballot(valid);
-    uint32_t base   = (seg == 0 && !(jz & 1ull)) ? cur.nn : 0;
-    uint64_t segm   = mask_lt() & ~((1ull << seg) - 1ull);
-    uint32_t nn     = base + __popcll(vmask & segm);
+    uint32_t       nr   = *reinterpret_cast<const uint16_t*>(page);
+    uint32_t       nv   = *reinterpret_cast<const uint16_t*>(page + 2);
+    if (c.type == T_VARCHAR && nr == 0xffff) return load_value(c, p, 0, true);
+    if (nv == nr) return load_value(c, p, j, false);
+    const uint8_t* bm = page + kPageSize - (nr + 7) / 8;
+    if (!((bm[j >> 3] >> (j & 7)) & 1)) return kNull;
+    uint32_t nn = 0;
+    for (uint32_t b = 0; b < (j >> 3); ++b) nn += __popc(bm[b]);
+    nn += __popc(bm[j >> 3] & ((1u << (j & 7)) - 1));
+    return load_value(c, p, nn, false);
+}
 
-    uint64_t slot = kNull;
-    if (valid) {
-        switch (c.type) {
-        case T_INT32: slot = *reinterpret_cast<const uint32_t*>(page + 4 + 4 * nn); break;
-        case T_INT64:
-        case T_FP64: slot = *reinterpret_cast<const uint64_t*>(page + 8 + 8 * uint64_t(nn)); break;
-        default: slot = make_handle(c.col_id, p, is_long ? 0xffffu : nn); break;
-        }
-    }
-    uint64_t act = __ballot(active);
-    if (act) {
-        uint32_t last = highest_lane(act);
-        cur.page      = __shfl(p, last, 64);
-        cur.nn        = __shfl(nn + (valid ? 1u : 0u), last, 64);
-    }
-    return slot;
+// Decodes row 64*block + lane for every lane (lanes with !active yield NULL).
+// Replaces the reference's skipTo/step64 (TableScan.cpp:135-324): validity and
+// the value index come from the block's 64-bit mask by popcount, so a step
+// costs one uniform index load plus one coalesced value load.
+__device__ inline uint64_t decode_block(const ScanCol& c, uint64_t block, bool active) {
+    const BlockIdx bi   = c.blocks[block];
+    const uint32_t lane = lane_id();
+    if (bi.complex) return active ? decode_row_slow(c, bi.page, uint32_t(block * 64 + lane)) : kNull;
+    bool second = lane >= bi.split;
+    bool valid  = active && ((bi.valid >> lane) & 1ull);
+    if (!valid) return kNull;
+    uint64_t before = bi.valid & mask_lt();
+    uint32_t nn     = second ? __popcll(before & ~((bi.split >= 64 ? ~0ull : (1ull << bi.split)) - 1ull))
+                             : bi.nn + __popcll(before);
+    return load_value(c, bi.page + (second ? 1 : 0), nn, false);
 }
 
 } // namespace cgj

@@ -72,8 +72,7 @@ ScanCol DevColumn::view(uint32_t col_id) const {
     ScanCol c;
     c.pages     = pages.get<uint8_t>();
     c.row_start = row_start.get<uint32_t>();
-    c.idx_page  = idx_page.get<uint32_t>();
-    c.idx_nn    = idx_nn.get<uint32_t>();
+    c.blocks    = blocks.get<BlockIdx>();
     c.type      = type;
     c.col_id    = col_id;
     return c;
@@ -97,19 +96,17 @@ Context::~Context() {
 }
 
 void Context::prepare_column(DevColumn& c) {
-    uint64_t nidx = c.nrows / kIndexRows + 2;
-    c.row_start   = DevBuf(&heap_, (c.npages + 1) * sizeof(uint32_t));
-    c.idx_page    = DevBuf(&heap_, nidx * sizeof(uint32_t));
-    c.idx_nn      = DevBuf(&heap_, nidx * sizeof(uint32_t));
+    c.row_start = DevBuf(&heap_, (c.npages + 1) * sizeof(uint32_t));
+    c.blocks    = DevBuf(&heap_, ((c.nrows + 63) / 64 + 1) * sizeof(BlockIdx));
     if (c.npages == 0) {
         CGJ_HIP(hipMemsetAsync(c.row_start.get(), 0, sizeof(uint32_t), stream_));
+        if (c.nrows) throw std::runtime_error("column has rows but no pages");
         return;
     }
     DevBuf scratch(&heap_, prepare_scratch_words(c.npages) * sizeof(uint32_t) + 16);
     DevBuf total(&heap_, sizeof(uint64_t));
-    CGJ_HIP(launch_prepare_column(c.pages.get<uint8_t>(), c.npages, c.type, c.row_start.get<uint32_t>(),
-                                  c.idx_page.get<uint32_t>(), c.idx_nn.get<uint32_t>(), scratch.get<uint32_t>(),
-                                  total.get<uint64_t>(), stream_));
+    CGJ_HIP(launch_prepare_column(c.pages.get<uint8_t>(), c.npages, c.type, c.nrows, c.row_start.get<uint32_t>(),
+                                  c.blocks.get<BlockIdx>(), scratch.get<uint32_t>(), total.get<uint64_t>(), stream_));
     uint64_t rows = 0;
     CGJ_HIP(hipMemcpyAsync(&rows, total.get(), sizeof(rows), hipMemcpyDeviceToHost, stream_));
     CGJ_HIP(hipStreamSynchronize(stream_));
@@ -236,7 +233,22 @@ std::unique_ptr<DevTable> Context::generate(uint64_t seed, uint64_t nrows, const
 }
 
 JoinTuning Context::tune(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes) const {
-    return autotune_join(build_rows, probe_rows, key_bytes);
+    JoinTuning t = autotune_join(build_rows, probe_rows, key_bytes);
+    if (options.max_load > 0) t.max_load = options.max_load;
+    if (options.bloom_bits >= 0) t.bloom_bits = uint32_t(options.bloom_bits);
+    if (options.partitioned_build >= 0) t.partitioned = options.partitioned_build != 0;
+    return t;
+}
+
+bool Context::set_option(const std::string& name, double v) {
+    if (name == "radix_min_table_bytes") options.radix_min_table_bytes = uint64_t(v);
+    else if (name == "radix_min_rows") options.radix_min_rows = uint64_t(v);
+    else if (name == "max_load") options.max_load = (v > 0 && v < 1) ? v : 0;
+    else if (name == "bloom_bits") options.bloom_bits = int32_t(v);
+    else if (name == "partitioned_build") options.partitioned_build = int32_t(v);
+    else if (name == "lean_kernel") options.lean_kernel = v != 0;
+    else return false;
+    return true;
 }
 
 
@@ -291,6 +303,7 @@ struct Target {
     double                est  = 0;  // estimated tuples
     DevBuf                stage;     // TARGET_STAGE output
     uint64_t              count = 0; // tuples produced
+    uint64_t              slots = 0; // TARGET_STAGE: staging slots used (unused ones have key kEmpty)
     DevResult*            result = nullptr;
 };
 
@@ -307,9 +320,9 @@ public:
         (void)hipEventDestroy(ev1_);
     }
 
-    std::unique_ptr<DevResult> run();
+    std::unique_ptr<DevResult> run(bool count_only = false);
 
-    double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0;
+    double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0, ms_partition = 0;
     uint32_t launches = 0, retries = 0;
     uint64_t build_tuples = 0, probe_rows = 0, build_bytes = 0, out_ref_pages = 0, probe_input_pages = 0;
 
@@ -347,6 +360,9 @@ private:
     const HashTable* make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est);
     void pipeline(Driver d, std::vector<ProbeStep> probes, Target& t);
     uint64_t launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms);
+    uint64_t grid_waves(uint64_t nrows, bool xcd_order = false) const;
+    const Tuples* partition_driver(uint32_t rel, uint32_t key_eq, uint64_t table_bytes);
+    static constexpr uint32_t kMorselRows = 4096;
 };
 
 void Executor::walk(int node, std::vector<int>& joins, int& leaf) const {
@@ -383,20 +399,75 @@ void Executor::register_all_columns() {
                            ctx_.stream()));
 }
 
+uint64_t Executor::grid_waves(uint64_t nrows, bool xcd_order) const {
+    uint64_t morsels = (nrows + kMorselRows - 1) / kMorselRows;
+    uint64_t waves   = std::min<uint64_t>(std::max<uint64_t>(morsels, 1), uint64_t(ctx_.cu_count()) * 16);
+    uint32_t per     = pipeline_waves_per_block() * (xcd_order ? 8 : 1);
+    return (waves + per - 1) / per * per;
+}
+
+const Tuples* Executor::partition_driver(uint32_t rel, uint32_t key_eq, uint64_t table_bytes) {
+    std::vector<uint32_t> eqs = q_.needed(1ull << rel);
+    auto                  it  = std::find(eqs.begin(), eqs.end(), key_eq);
+    if (it == eqs.end()) throw std::runtime_error("probe key is not a column of the driving table");
+    std::iter_swap(eqs.begin(), it);
+    if (eqs.size() > size_t(kMaxScan)) throw std::runtime_error("too many scanned columns in a pipeline");
+    RadixArgs ra;
+    std::memset(&ra, 0, sizeof(ra));
+    ra.nrows = q_.rels[rel].rows;
+    for (uint32_t e : eqs) {
+        uint64_t col = q_.attrs[q_.attr_of(rel, e)].col;
+        if (q_.is_join_eq(e)) ra.not_null |= 1u << ra.nscan;
+        ra.scan[ra.nscan++] = column(rel, col).view(col_id(rel, col));
+    }
+    // partitions of about 2 MiB of hash table each: half an XCD's L2
+    uint64_t nparts = 8;
+    while (nparts < 4096 && table_bytes / nparts > (uint64_t(2) << 20)) nparts <<= 1;
+    ra.nparts        = uint32_t(nparts);
+    ra.rshift        = 64 - log2_ceil(nparts);
+    uint64_t nb      = (ra.nrows + 63) / 64;
+    uint32_t wgs     = uint32_t(std::min<uint64_t>(1024, std::max<uint64_t>(1, (nb + 15) / 16)));
+    ra.blocks_per_wg = (nb + wgs - 1) / wgs;
+    DevBuf bh(&ctx_.heap(), nparts * wgs * sizeof(uint32_t));
+    DevBuf scratch(&ctx_.heap(), scan_scratch_words(nparts * wgs) * sizeof(uint32_t));
+    DevBuf total(&ctx_.heap(), sizeof(uint64_t));
+    auto   tp = std::make_unique<Tuples>();
+    tp->buf   = DevBuf(&ctx_.heap(), std::max<uint64_t>(ra.nrows, 1) * ra.nscan * sizeof(uint64_t));
+    tp->eqs   = eqs;
+    ra.bh     = bh.get<uint32_t>();
+    ra.out    = tp->buf.get<uint64_t>();
+    CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
+    CGJ_HIP(launch_radix_partition(ra, wgs, scratch.get<uint32_t>(), total.get<uint64_t>(), ctx_.stream()));
+    CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
+    uint64_t n = 0;
+    CGJ_HIP(hipMemcpyAsync(&n, total.get(), sizeof(n), hipMemcpyDeviceToHost, ctx_.stream()));
+    CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
+    float f = 0;
+    CGJ_HIP(hipEventElapsedTime(&f, ev0_, ev1_));
+    ms_probe += f;
+    ms_partition += f;
+    tp->n = n;
+    for (uint32_t c = 0; c < ra.nscan; ++c) probe_input_pages += column(rel, q_.attrs[q_.attr_of(rel, eqs[c])].col).npages;
+    tuples_.push_back(std::move(tp));
+    return tuples_.back().get();
+}
+
 uint64_t Executor::launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms) {
     a.counters    = counters_.get<PipeCounters>();
-    a.morsel_rows = 4096;
+    a.morsel_rows = kMorselRows;
     a.nrows       = nrows;
     a.nmorsels    = (nrows + a.morsel_rows - 1) / a.morsel_rows;
     a.col_base    = col_base_.get<const uint8_t* const>();
     a.col_npages  = col_npages_.get<uint32_t>();
     CGJ_HIP(hipMemsetAsync(a.counters, 0, sizeof(PipeCounters), ctx_.stream()));
-    uint64_t waves = std::min<uint64_t>(std::max<uint64_t>(a.nmorsels, 1), uint64_t(ctx_.cu_count()) * 16);
-    if (a.target == TARGET_PAGES && a.nvarchar) {
-        // staging is allocated by the caller for `waves`
-    }
+    uint64_t waves = grid_waves(nrows, a.xcd_order != 0);
     CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
-    if (a.nmorsels) CGJ_HIP(launch_pipeline(a, nslots, uint32_t(waves), ctx_.stream()));
+    if (a.nmorsels) {
+        if (ctx_.options.lean_kernel && probe1_applicable(a, nslots))
+            CGJ_HIP(launch_probe1(a, nslots, uint32_t(waves), ctx_.stream()));
+        else
+            CGJ_HIP(launch_pipeline(a, nslots, uint32_t(waves), ctx_.stream()));
+    }
     CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
     CGJ_HIP(hipEventSynchronize(ev1_));
     float f = 0;
@@ -440,13 +511,14 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
     if (partitioned) {
         uint32_t log_regions = lg - rs_log;
         uint64_t nregions    = 1ull << log_regions;
-        uint32_t blocks      = uint32_t(std::min<uint64_t>(512, (n + 16383) / 16384));
-        uint64_t chunk       = (n + blocks - 1) / blocks;
+        uint64_t slots       = staged.slots;
+        uint32_t blocks      = uint32_t(std::min<uint64_t>(512, (slots + 16383) / 16384));
+        uint64_t chunk       = (slots + blocks - 1) / blocks;
         DevBuf   bh(&ctx_.heap(), nregions * blocks * sizeof(uint32_t));
         DevBuf   scratch(&ctx_.heap(), scan_scratch_words(nregions * blocks) * sizeof(uint32_t));
         DevBuf   total(&ctx_.heap(), sizeof(uint64_t));
         DevBuf   parts(&ctx_.heap(), n * ht->words * sizeof(uint64_t));
-        CGJ_HIP(launch_part_build(staged.stage.get<uint64_t>(), n, ht->words, parts.get<uint64_t>(), bh.get<uint32_t>(),
+        CGJ_HIP(launch_part_build(staged.stage.get<uint64_t>(), slots, n, ht->words, parts.get<uint64_t>(), bh.get<uint32_t>(),
                                   blocks, chunk, scratch.get<uint32_t>(), total.get<uint64_t>(),
                                   ht->slots.get<uint64_t>(), lg, log_regions, bloom, ht->bloom_log,
                                   flags.get<uint32_t>(), ctx_.stream()));
@@ -465,7 +537,7 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
     }
     if (!done) {
         CGJ_HIP(launch_fill_u64(ht->slots.get<uint64_t>(), ht->cap * ht->words, kEmpty, ctx_.stream()));
-        CGJ_HIP(launch_ht_insert(staged.stage.get<uint64_t>(), n, ht->words, ht->slots.get<uint64_t>(), ht->shift,
+        CGJ_HIP(launch_ht_insert(staged.stage.get<uint64_t>(), staged.slots, ht->words, ht->slots.get<uint64_t>(), ht->shift,
                                  ht->cap - 1, bloom, ht->bloom_log, flags.get<uint32_t>(), ctx_.stream()));
         uint32_t f[4];
         CGJ_HIP(hipMemcpyAsync(f, flags.get(), sizeof(f), hipMemcpyDeviceToHost, ctx_.stream()));
@@ -528,10 +600,14 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         mid.kind = TARGET_STAGE;
         mid.eqs  = q_.needed(set);
         mid.est  = t.est;
+        // word 0 must be a join class (never NULL): unused slots are marked by an empty word 0
+        auto je = std::find_if(mid.eqs.begin(), mid.eqs.end(), [&](uint32_t e) { return q_.is_join_eq(e); });
+        if (je == mid.eqs.end()) throw std::runtime_error("materialised chain carries no join column");
+        std::iter_swap(mid.eqs.begin(), je);
         pipeline(d, head, mid);
         auto tp = std::make_unique<Tuples>();
         tp->buf = std::move(mid.stage);
-        tp->n   = mid.count;
+        tp->n   = mid.slots;
         tp->eqs = mid.eqs;
         tuples_.push_back(std::move(tp));
         d.rel    = -1;
@@ -540,8 +616,23 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         probes.erase(probes.begin(), probes.begin() + kMaxProbes);
     }
 
+    // A table probing a hash table far larger than an XCD's L2 is first
+    // radix-partitioned by the probe key's hash; the pipeline then walks the
+    // partitions XCD by XCD so each probes a cache-resident range of the table.
+    bool xcd_order = false;
+    if (d.rel >= 0 && !probes.empty()) {
+        const HashTable* h0 = probes[0].ht;
+        uint64_t         hb = h0->cap * h0->words * sizeof(uint64_t);
+        if (hb >= ctx_.options.radix_min_table_bytes && q_.rels[d.rel].rows >= ctx_.options.radix_min_rows) {
+            d.tuples  = partition_driver(uint32_t(d.rel), probes[0].key_eq, hb);
+            d.rel     = -1;
+            xcd_order = true;
+        }
+    }
+
     PipeArgs a;
     std::memset(&a, 0, sizeof(a));
+    a.xcd_order = xcd_order ? 1 : 0;
     std::map<uint32_t, uint32_t> slot; // class -> tuple slot
     uint32_t                     nslots = 0;
     uint64_t                     nrows  = 0;
@@ -617,22 +708,29 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             if (it == slot.end()) throw std::runtime_error("staged class is not available in the pipeline");
             a.stage_slot[w] = it->second;
         }
-        uint64_t cap = probes.empty() ? nrows : uint64_t(std::min(std::max(t.est * 1.25, 1024.0), 4e9));
-        cap          = std::max<uint64_t>(cap, 1);
+        // Each wave reserves `chunk` slots per atomic; the last reservation of a
+        // wave can stay partly unused (those slots carry key kEmpty).
+        uint64_t waves = grid_waves(nrows, xcd_order);
+        double   per   = (probes.empty() ? double(nrows) : std::max(t.est, 1.0)) / double(waves);
+        uint64_t chunk = std::min<uint64_t>(4096, std::max<uint64_t>(64, uint64_t(per)));
+        uint64_t cap   = probes.empty() ? nrows : uint64_t(std::min(std::max(t.est * 1.25, 1024.0), 4e9));
+        cap            = std::max<uint64_t>(cap, 1) + waves * chunk;
         for (int attempt = 0;; ++attempt) {
-            t.stage      = DevBuf(&ctx_.heap(), cap * a.stage_words * sizeof(uint64_t));
-            a.stage      = t.stage.get<uint64_t>();
-            a.stage_cap  = cap;
+            t.stage       = DevBuf(&ctx_.heap(), cap * a.stage_words * sizeof(uint64_t));
+            a.stage       = t.stage.get<uint64_t>();
+            a.stage_cap   = cap;
+            a.stage_chunk = chunk;
             launch(a, nslots, nrows, *ms);
             PipeCounters c;
             CGJ_HIP(hipMemcpy(&c, counters_.get(), sizeof(c), hipMemcpyDeviceToHost));
             if (!c.overflow) {
                 t.count = c.tuples;
+                t.slots = std::min<uint64_t>(c.pages, cap);
                 break;
             }
             if (attempt > 2) throw std::runtime_error("staging overflow persists");
             ++retries;
-            cap = c.tuples + 1;
+            cap = c.pages + waves * chunk + 64;
         }
         return;
     }
@@ -656,11 +754,12 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         a.out[i].vc   = a.out[i].type == CGJ_VARCHAR ? a.nvarchar++ : 0;
         bytes_per_row += a.out[i].type == CGJ_INT32 ? 4.2 : (a.out[i].type == CGJ_VARCHAR ? 48.0 : 8.2);
     }
-    uint64_t waves   = std::min<uint64_t>(std::max<uint64_t>((nrows + 4095) / 4096, 1), uint64_t(ctx_.cu_count()) * 16);
-    waves            = (waves + 3) / 4 * 4;
+    uint64_t waves   = grid_waves(nrows, xcd_order);
     DevBuf   stage_s = DevBuf(&ctx_.heap(), std::max<uint64_t>(waves * a.nvarchar * kPageSize, 64));
     a.str_stage      = stage_s.get<uint8_t>();
-    uint64_t pages   = uint64_t(std::min(t.est, 4e9) * bytes_per_row / 7800.0) + waves * a.nout + 64;
+    // every wave may hold one partly used reservation of pool pages
+    uint64_t slack   = waves * 32 + 64;
+    uint64_t pages   = uint64_t(std::min(t.est, 4e9) * bytes_per_row / 7800.0) + waves * a.nout + slack;
     for (int attempt = 0;; ++attempt) {
         res->pool    = DevBuf(&ctx_.heap(), pages * kPageSize + pages * 16 + 64);
         a.pool       = res->pool.get<uint8_t>();
@@ -671,10 +770,12 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         PipeCounters c;
         CGJ_HIP(hipMemcpy(&c, counters_.get(), sizeof(c), hipMemcpyDeviceToHost));
         if (!c.overflow) {
-            res->nrows       = c.tuples;
-            res->total_pages = c.dir;
-            res->pool_pages  = pages;
-            ms_probe_kernel  = last_ms_;
+            res->nrows          = c.tuples;
+            res->total_pages    = 0;
+            for (uint32_t i = 0; i < a.nout; ++i) res->total_pages += c.col_pages[i];
+            res->pool_pages     = pages;
+            res->reserved_pages = std::min<uint64_t>(c.pages, pages);
+            ms_probe_kernel     = last_ms_;
             // output bytes at the reference's packing (TableTarget.cpp:111-182)
             for (uint32_t i = 0; i < a.nout; ++i) {
                 int32_t ty = a.out[i].type;
@@ -690,21 +791,21 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         }
         if (attempt > 2) throw std::runtime_error("output page pool overflow persists");
         ++retries;
-        pages = std::max<uint64_t>(c.pages, c.dir) + 64;
+        pages = c.pages + slack;
     }
     probe_rows += nrows;
     if (d.rel >= 0)
         for (uint32_t i = 0; i < a.nscan; ++i) probe_input_pages += column(uint32_t(d.rel), scan_cols[i]).npages;
 }
 
-std::unique_ptr<DevResult> Executor::run() {
+std::unique_ptr<DevResult> Executor::run(bool count_only) {
     auto res = std::make_unique<DevResult>();
     for (uint32_t a : q_.out) res->types.push_back(q_.attrs[a].type);
     for (auto& r : q_.rels)
         if (r.rows == 0) return res; // an empty input empties the inner join
     register_all_columns();
     Target t;
-    t.kind   = TARGET_PAGES;
+    t.kind   = count_only ? TARGET_COUNT : TARGET_PAGES;
     t.est    = t_.nodes[t_.root].card;
     t.result = res.get();
     std::vector<int> joins;
@@ -726,12 +827,40 @@ std::unique_ptr<DevResult> Executor::run() {
     d.rel = leaf;
     d.set = 1ull << leaf;
     pipeline(d, probes, t);
+    if (count_only) res->nrows = t.count;
     return res;
 }
 
 } // namespace
 
 std::unique_ptr<DevResult> Context::execute(const cgj_plan& plan, const std::vector<const DevTable*>& inputs) {
+    return run_query(plan, inputs, false);
+}
+
+uint64_t Context::count(const cgj_plan& plan, const std::vector<const DevTable*>& inputs) {
+    return run_query(plan, inputs, true)->nrows;
+}
+
+double Context::copy_probe(uint64_t bytes, uint32_t reps) {
+    uint64_t   words = std::max<uint64_t>(bytes / 16, 1);
+    DevBuf     a(&heap_, words * 16), b(&heap_, words * 16);
+    hipEvent_t e0, e1;
+    CGJ_HIP(hipEventCreate(&e0));
+    CGJ_HIP(hipEventCreate(&e1));
+    CGJ_HIP(launch_copy(b.get(), a.get(), words, stream_)); // warm-up
+    CGJ_HIP(hipEventRecord(e0, stream_));
+    for (uint32_t i = 0; i < std::max<uint32_t>(reps, 1); ++i) CGJ_HIP(launch_copy(b.get(), a.get(), words, stream_));
+    CGJ_HIP(hipEventRecord(e1, stream_));
+    CGJ_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CGJ_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return double(ms) / std::max<uint32_t>(reps, 1);
+}
+
+std::unique_ptr<DevResult> Context::run_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs,
+                                              bool count_only) {
     std::vector<uint64_t> rows;
     std::vector<uint32_t> cols;
     for (auto* t : inputs) {
@@ -747,12 +876,13 @@ std::unique_ptr<DevResult> Context::execute(const cgj_plan& plan, const std::vec
             if (r.rows && inputs[r.table]->cols[q.attrs[a].col].type != q.attrs[a].type)
                 throw std::runtime_error("plan column type differs from the input column type");
     Executor ex(*this, q, tree, inputs);
-    auto     res = ex.run();
+    auto     res = ex.run(count_only);
     stats                   = cgj_stats{};
     stats.ms_build          = ex.ms_build;
     stats.ms_probe          = ex.ms_probe;
     stats.ms_total          = ex.ms_build + ex.ms_probe;
     stats.ms_probe_kernel   = ex.ms_probe_kernel;
+    stats.ms_probe_partition = ex.ms_partition;
     stats.build_tuples      = ex.build_tuples;
     stats.build_bytes       = ex.build_bytes;
     stats.probe_rows        = ex.probe_rows;
@@ -782,12 +912,16 @@ void Context::download(const DevResult& r, cgj_result& out) {
         out.columns[c].pages     = new void*[1];
     }
     if (r.total_pages == 0) return;
-    // page order per column: by wave stream, then sequence (directory entries)
-    std::vector<uint32_t> dir(r.total_pages * 4);
+    // The directory has one entry per reserved pool page: (column, wave
+    // stream, sequence, valid). Page order per column: stream, then sequence.
+    std::vector<uint32_t> dir(r.reserved_pages * 4);
     CGJ_HIP(hipMemcpyAsync(dir.data(), r.dir(), dir.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
     CGJ_HIP(hipStreamSynchronize(stream_));
-    std::vector<uint64_t> order(r.total_pages);
-    std::iota(order.begin(), order.end(), 0);
+    std::vector<uint64_t> order;
+    order.reserve(r.total_pages);
+    for (uint64_t i = 0; i < r.reserved_pages; ++i)
+        if (dir[4 * i + 3] == 1) order.push_back(i);
+    if (order.size() != r.total_pages) throw std::runtime_error("page directory does not match the page count");
     std::sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
         const uint32_t* a0 = &dir[4 * x];
         const uint32_t* b0 = &dir[4 * y];
@@ -798,7 +932,7 @@ void Context::download(const DevResult& r, cgj_result& out) {
     std::vector<std::vector<uint32_t>> pages(r.types.size());
     for (uint64_t i : order) {
         if (dir[4 * i] >= pages.size()) throw std::runtime_error("corrupt page directory");
-        pages[dir[4 * i]].push_back(dir[4 * i + 3]);
+        pages[dir[4 * i]].push_back(uint32_t(i));
     }
     uint64_t used = 0;
     for (auto& col : pages)

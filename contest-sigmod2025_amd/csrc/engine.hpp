@@ -84,8 +84,7 @@ struct DevColumn {
     uint64_t nrows  = 0;
     DevBuf   pages;     // npages x 8 KiB
     DevBuf   row_start; // u32 [npages + 1]
-    DevBuf   idx_page;  // u32 [nrows / 1024 + 1]
-    DevBuf   idx_nn;    // u32 [nrows / 1024 + 1]
+    DevBuf   blocks;    // BlockIdx [ceil(nrows / 64)]
 
     ScanCol view(uint32_t col_id) const;
 };
@@ -102,8 +101,9 @@ struct DevTable {
 struct DevResult {
     uint64_t             nrows       = 0;
     std::vector<int32_t> types;
-    uint64_t             total_pages = 0;
-    uint64_t             pool_pages  = 0;
+    uint64_t             total_pages    = 0;
+    uint64_t             pool_pages     = 0;
+    uint64_t             reserved_pages = 0; // directory entries to scan (valid flag in word 3)
     DevBuf               pool;
 
     const uint32_t* dir() const { return reinterpret_cast<const uint32_t*>(pool.get<uint8_t>() + pool_pages * 8192); }
@@ -118,10 +118,24 @@ struct JoinTuning {
     bool     partitioned = true; // radix-partitioned LDS build (else global CAS inserts)
 };
 
+// Execution options (cgj_set_option); automatic choices unless overridden.
+struct Options {
+    uint64_t radix_min_table_bytes = uint64_t(64) << 20; // partition the probe side above this table size
+    uint64_t radix_min_rows        = uint64_t(4) << 20;  // ... and this many driving rows
+    double   max_load              = 0;                  // hash-table load factor (0 = automatic)
+    int32_t  bloom_bits            = -1;                 // bloom bits per key (-1 = automatic)
+    int32_t  partitioned_build     = -1;                 // radix-partitioned build (-1 = automatic)
+    bool     lean_kernel           = true;               // use the one-probe kernel where it applies
+};
+
 class Context {
 public:
     explicit Context(int device);
     ~Context();
+
+    Options options;
+    // Sets a named option; false for an unknown name.
+    bool set_option(const std::string& name, double value);
 
     int         device() const { return device_; }
     hipStream_t stream() const { return stream_; }
@@ -133,7 +147,9 @@ public:
     void prepare_column(DevColumn& c);
 
     std::unique_ptr<DevResult> execute(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
-    void download(const DevResult& r, cgj_result& out);
+    uint64_t count(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
+    void     download(const DevResult& r, cgj_result& out);
+    double   copy_probe(uint64_t bytes, uint32_t reps);
 
     cgj_stats   stats{};
     std::string error;
@@ -142,6 +158,9 @@ public:
     uint32_t   cu_count() const { return cus_; }
 
 private:
+    std::unique_ptr<DevResult> run_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs,
+                                         bool count_only);
+
     int         device_;
     uint32_t    cus_    = 256;
     hipStream_t stream_ = nullptr;

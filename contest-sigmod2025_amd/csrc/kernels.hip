@@ -1,8 +1,9 @@
 // HIP kernels of the join pipeline for gfx950 (wave64).
 //
 //  * column preparation: rows per page -> prefix (reference TableScan.cpp:546-582
-//    makeSearchVector) plus a morsel index (page and non-NULL offset of every
-//    1024th row) so that any wave can start decoding at any morsel;
+//    makeSearchVector) plus a block index (page, non-NULL offset and validity
+//    mask of every 64-row block) so that any wave decodes any block with one
+//    index load and one coalesced value load;
 //  * page decode into 64-bit slots (operator entry point and tests);
 //  * hash-table insert: open addressing with linear probing over
 //    [key, attrs...] entries, plus a blocked bloom filter;
@@ -81,31 +82,76 @@ __global__ void k_scan_add(uint32_t* data, uint64_t n, const uint32_t* sums, con
     if (i == 0) out[out_n] = uint32_t(*total);
 }
 
-// Morsel index: for every row k*kIndexRows, its page and the count of non-NULL
-// values before it in that page.
-__global__ void k_morsel_index(const uint8_t* pages, uint64_t npages, int32_t type, const uint32_t* row_start,
-                               uint32_t* idx_page, uint32_t* idx_nn) {
-    uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= npages) return;
-    uint32_t r0 = row_start[p], r1 = row_start[p + 1];
-    if (r0 == r1) return;
-    const uint8_t* page = pages + p * kPageSize;
+__device__ inline bool page_bit(const uint8_t* page, uint32_t nr, uint32_t nv, uint32_t j) {
+    if (nv == nr) return true;
+    const uint8_t* bm = page + kPageSize - (nr + 7) / 8;
+    return (bm[j >> 3] >> (j & 7)) & 1;
+}
+
+// Block index: one entry per 64-row block (see BlockIdx).
+__global__ void k_block_index(const uint8_t* pages, uint64_t npages, int32_t type, const uint32_t* row_start,
+                              uint64_t nrows, BlockIdx* blocks) {
+    uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b * 64 >= nrows) return;
+    uint32_t r0 = uint32_t(b * 64);
+    uint32_t r1 = uint32_t(r0 + 64 < nrows ? r0 + 64 : nrows);
+    // last page whose first row is <= r0 (it holds r0; empty pages are skipped)
+    uint64_t lo = 0, hi = npages;
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) / 2;
+        if (row_start[mid] <= r0) lo = mid; else hi = mid;
+    }
+    uint32_t       p    = uint32_t(lo);
+    const uint8_t* page = pages + uint64_t(p) * kPageSize;
     uint32_t       nr   = *reinterpret_cast<const uint16_t*>(page);
     uint32_t       nv   = *reinterpret_cast<const uint16_t*>(page + 2);
-    bool           dense = (type == T_VARCHAR && nr >= 0xfffe) || nv == nr;
-    uint32_t       k     = (r0 + kIndexRows - 1) / kIndexRows;
-    for (uint32_t row = k * kIndexRows; row < r1; row += kIndexRows, ++k) {
-        uint32_t j  = row - r0;
-        uint32_t nn = j;
-        if (!dense) {
-            const uint8_t* bm = page + kPageSize - (nr + 7) / 8;
-            nn                = 0;
-            for (uint32_t b = 0; b < j / 8; ++b) nn += __popc(bm[b]);
-            if (j & 7) nn += __popc(bm[j / 8] & ((1u << (j & 7)) - 1));
-        }
-        idx_page[k] = uint32_t(p);
-        idx_nn[k]   = nn;
+    BlockIdx       e;
+    e.page    = p;
+    e.nn      = 0;
+    e.split   = 64;
+    e.complex = 0;
+    e.valid   = 0;
+    bool is_long = type == T_VARCHAR && nr >= 0xfffe;
+    if (is_long) {
+        e.complex = 1;
+        blocks[b] = e;
+        return;
     }
+    uint32_t j0 = r0 - row_start[p];
+    uint32_t re = row_start[p + 1];
+    uint32_t n1 = (r1 < re ? r1 : re) - r0;
+    uint32_t nn = 0;
+    if (nv == nr) {
+        nn = j0;
+    } else {
+        const uint8_t* bm = page + kPageSize - (nr + 7) / 8;
+        for (uint32_t k = 0; k < (j0 >> 3); ++k) nn += __popc(bm[k]);
+        if (j0 & 7) nn += __popc(bm[j0 >> 3] & ((1u << (j0 & 7)) - 1));
+    }
+    e.nn = uint16_t(nn);
+    for (uint32_t t = 0; t < n1; ++t)
+        if (page_bit(page, nr, nv, j0 + t)) e.valid |= 1ull << t;
+    if (r0 + n1 < r1) {
+        // the block continues on the next page, which must hold all its remaining rows
+        // (rows remain, so p is not the last page: row_start[npages] = nrows >= r1)
+        uint32_t q = p + 1;
+        if (q >= npages || row_start[q + 1] < r1 || row_start[q] != r0 + n1) {
+            e.complex = 1;
+            blocks[b] = e;
+            return;
+        }
+        const uint8_t* pq  = pages + uint64_t(q) * kPageSize;
+        uint32_t       nrq = *reinterpret_cast<const uint16_t*>(pq);
+        uint32_t       nvq = *reinterpret_cast<const uint16_t*>(pq + 2);
+        if (type == T_VARCHAR && nrq >= 0xfffe) {
+            e.complex = 1;
+        } else {
+            e.split = uint8_t(n1);
+            for (uint32_t t = 0; t < r1 - r0 - n1; ++t)
+                if (page_bit(pq, nrq, nvq, t)) e.valid |= 1ull << (n1 + t);
+        }
+    }
+    blocks[b] = e;
 }
 
 size_t prepare_scratch_words(uint64_t npages) { return npages + (npages + kScanBlock - 1) / kScanBlock + 2; }
@@ -127,9 +173,9 @@ hipError_t launch_exclusive_scan(uint32_t* data, uint64_t n, uint32_t* scratch, 
     return hipGetLastError();
 }
 
-hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t type, uint32_t* row_start,
-                                 uint32_t* idx_page, uint32_t* idx_nn, uint32_t* scratch,
-                                 uint64_t* total_rows_dev, hipStream_t s) {
+hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t type, uint64_t nrows,
+                                 uint32_t* row_start, BlockIdx* blocks, uint32_t* scratch, uint64_t* total_rows_dev,
+                                 hipStream_t s) {
     if (npages == 0) return hipSuccess;
     uint32_t* counts  = scratch;
     uint32_t* sums    = scratch + npages;
@@ -139,8 +185,8 @@ hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t 
     k_scan_sums<<<dim3(1), dim3(kScanBlock), 0, s>>>(sums, nblocks, total_rows_dev);
     k_scan_add<<<dim3((npages + 255) / 256), dim3(256), 0, s>>>(counts, npages, sums, total_rows_dev, row_start,
                                                                 npages);
-    k_morsel_index<<<dim3((npages + 255) / 256), dim3(256), 0, s>>>(pages, npages, type, row_start, idx_page,
-                                                                   idx_nn);
+    uint64_t nb = (nrows + 63) / 64;
+    if (nb) k_block_index<<<dim3((nb + 255) / 256), dim3(256), 0, s>>>(pages, npages, type, row_start, nrows, blocks);
     return hipGetLastError();
 }
 
@@ -150,26 +196,33 @@ hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t 
 __global__ __launch_bounds__(256) void k_decode(ScanCol c, uint64_t nrows, uint64_t* out) {
     uint64_t wave   = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
     uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / 64;
-    uint64_t nm     = (nrows + kIndexRows - 1) / kIndexRows;
-    for (uint64_t m = wave; m < nm; m += nwaves) {
-        uint64_t r0  = m * kIndexRows;
-        uint64_t end = r0 + kIndexRows < nrows ? r0 + kIndexRows : nrows;
-        Cursor   cur = cursor_at(c, r0);
-        for (uint64_t r = r0; r < end; r += 64) {
-            uint64_t row    = r + lane_id();
-            bool     active = row < end;
-            uint64_t v      = decode_step(c, cur, uint32_t(row), active);
-            if (active) out[row] = v;
-        }
+    uint64_t nb     = (nrows + 63) / 64;
+    for (uint64_t b = wave; b < nb; b += nwaves) {
+        uint64_t row    = b * 64 + lane_id();
+        bool     active = row < nrows;
+        uint64_t v      = decode_block(c, b, active);
+        if (active) out[row] = v;
     }
 }
 
 hipError_t launch_decode(const ScanCol& c, uint64_t nrows, uint64_t* out, hipStream_t s) {
     if (nrows == 0) return hipSuccess;
-    uint64_t nm     = (nrows + kIndexRows - 1) / kIndexRows;
-    uint64_t blocks = (nm + 3) / 4;
+    uint64_t nb     = (nrows + 63) / 64;
+    uint64_t blocks = (nb + 3) / 4;
     if (blocks > 8192) blocks = 8192;
     k_decode<<<dim3(blocks), dim3(256), 0, s>>>(c, nrows, out);
+    return hipGetLastError();
+}
+
+// Plain streaming copy (16-byte vectors, grid-stride): the HBM bandwidth probe.
+__global__ __launch_bounds__(256) void k_copy(uint4* dst, const uint4* src, uint64_t n) {
+    uint64_t i      = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(void* dst, const void* src, uint64_t n16, hipStream_t s) {
+    k_copy<<<dim3(4096), dim3(256), 0, s>>>(static_cast<uint4*>(dst), static_cast<const uint4*>(src), n16);
     return hipGetLastError();
 }
 
@@ -197,6 +250,7 @@ __global__ void k_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, u
     for (; i < n; i += stride) {
         const uint64_t* t   = stage + i * words;
         uint64_t        key = t[0];
+        if (key == kEmpty) continue; // unused staging slot
         uint64_t        h   = hash_key(key);
         uint64_t        pos = h >> shift;
         bool            dup = false;
@@ -235,8 +289,9 @@ __global__ __launch_bounds__(256) void k_part_hist(const uint64_t* stage, uint64
     uint64_t begin = uint64_t(blockIdx.x) * chunk;
     uint64_t end   = begin + chunk < n ? begin + chunk : n;
     for (uint64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-        uint32_t r = uint32_t(hash_key(stage[i * words]) >> rshift);
-        atomicAdd(&hist[r], 1u);
+        uint64_t key = stage[i * words];
+        if (key == kEmpty) continue; // unused staging slot
+        atomicAdd(&hist[uint32_t(hash_key(key) >> rshift)], 1u);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nregions; i += blockDim.x) bh[uint64_t(i) * gridDim.x + blockIdx.x] = hist[i];
@@ -252,9 +307,10 @@ __global__ __launch_bounds__(256) void k_part_scatter(const uint64_t* stage, uin
     uint64_t begin = uint64_t(blockIdx.x) * chunk;
     uint64_t end   = begin + chunk < n ? begin + chunk : n;
     for (uint64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-        const uint64_t* t   = stage + i * words;
-        uint32_t        r   = uint32_t(hash_key(t[0]) >> rshift);
-        uint32_t        pos = atomicAdd(&cur[r], 1u);
+        const uint64_t* t = stage + i * words;
+        if (t[0] == kEmpty) continue; // unused staging slot
+        uint32_t r   = uint32_t(hash_key(t[0]) >> rshift);
+        uint32_t pos = atomicAdd(&cur[r], 1u);
         uint64_t*       d   = out + uint64_t(pos) * words;
         for (uint32_t w = 0; w < words; ++w) d[w] = t[w];
     }
@@ -304,8 +360,8 @@ __global__ __launch_bounds__(256) void k_part_build(const uint64_t* parts, const
     for (uint32_t i = threadIdx.x; i < rslots * words; i += blockDim.x) dst[i] = tab[i];
 }
 
-hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* parts, uint32_t* bh,
-                             uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch, uint64_t* total,
+hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real, uint32_t words, uint64_t* parts,
+                             uint32_t* bh, uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch, uint64_t* total,
                              uint64_t* ht, uint32_t log_cap, uint32_t log_regions, uint64_t* bloom,
                              uint32_t bloom_log, uint32_t* flags, hipStream_t s) {
     if (n == 0) return hipSuccess;
@@ -318,8 +374,69 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint32_t words, 
     if (e != hipSuccess) return e;
     k_part_scatter<<<dim3(blocks), dim3(256), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh, parts);
     size_t tab_lds = size_t(rslots) * words * sizeof(uint64_t);
-    k_part_build<<<dim3(nregions), dim3(256), tab_lds, s>>>(parts, bh, blocks, n, nregions, words, rslots,
+    k_part_build<<<dim3(nregions), dim3(256), tab_lds, s>>>(parts, bh, blocks, n_real, nregions, words, rslots,
                                                             64 - log_cap, ht, bloom, bloom_log, flags);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// radix-partitioned probe side: the scanned columns of the driving table are
+// decoded once and scattered by the top bits of the probe key's hash, so that
+// each partition probes one contiguous, cache-sized range of the hash table.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_radix_hist(RadixArgs a) {
+    extern __shared__ uint32_t hist[];
+    for (uint32_t i = threadIdx.x; i < a.nparts; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    uint64_t nb    = (a.nrows + 63) / 64;
+    uint64_t b0    = uint64_t(blockIdx.x) * a.blocks_per_wg;
+    uint64_t b1    = b0 + a.blocks_per_wg < nb ? b0 + a.blocks_per_wg : nb;
+    uint32_t wave  = threadIdx.x / 64;
+    for (uint64_t b = b0 + wave; b < b1; b += blockDim.x / 64) {
+        uint64_t row    = b * 64 + lane_id();
+        bool     active = row < a.nrows;
+        uint64_t key    = decode_block(a.scan[0], b, active);
+        bool     ok     = active && key != kNull;
+        for (uint32_t c = 1; c < a.nscan && ok; ++c)
+            if ((a.not_null >> c) & 1u) ok = decode_block(a.scan[c], b, active) != kNull;
+        if (ok) atomicAdd(&hist[uint32_t(hash_key(key) >> a.rshift)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.nparts; i += blockDim.x) a.bh[uint64_t(i) * gridDim.x + blockIdx.x] = hist[i];
+}
+
+__global__ __launch_bounds__(256) void k_radix_scatter(RadixArgs a) {
+    extern __shared__ uint32_t cur[];
+    for (uint32_t i = threadIdx.x; i < a.nparts; i += blockDim.x) cur[i] = a.bh[uint64_t(i) * gridDim.x + blockIdx.x];
+    __syncthreads();
+    uint64_t nb   = (a.nrows + 63) / 64;
+    uint64_t b0   = uint64_t(blockIdx.x) * a.blocks_per_wg;
+    uint64_t b1   = b0 + a.blocks_per_wg < nb ? b0 + a.blocks_per_wg : nb;
+    uint32_t wave = threadIdx.x / 64;
+    for (uint64_t b = b0 + wave; b < b1; b += blockDim.x / 64) {
+        uint64_t row    = b * 64 + lane_id();
+        bool     active = row < a.nrows;
+        uint64_t v[kMaxScan];
+        bool     ok = active;
+        for (uint32_t c = 0; c < a.nscan; ++c) {
+            v[c] = decode_block(a.scan[c], b, active);
+            if ((a.not_null >> c) & 1u) ok = ok && v[c] != kNull;
+        }
+        if (ok) {
+            uint32_t  pos = atomicAdd(&cur[uint32_t(hash_key(v[0]) >> a.rshift)], 1u);
+            uint64_t* d   = a.out + uint64_t(pos) * a.nscan;
+            for (uint32_t c = 0; c < a.nscan; ++c) d[c] = v[c];
+        }
+    }
+}
+
+hipError_t launch_radix_partition(const RadixArgs& a, uint32_t wgs, uint32_t* scan_scratch, uint64_t* total,
+                                  hipStream_t s) {
+    size_t lds = size_t(a.nparts) * sizeof(uint32_t);
+    k_radix_hist<<<dim3(wgs), dim3(256), lds, s>>>(a);
+    hipError_t e = launch_exclusive_scan(a.bh, uint64_t(a.nparts) * wgs, scan_scratch, total, s);
+    if (e != hipSuccess) return e;
+    k_radix_scatter<<<dim3(wgs), dim3(256), lds, s>>>(a);
     return hipGetLastError();
 }
 
@@ -352,28 +469,49 @@ __device__ inline void sset(uint64_t (&s)[NA], uint32_t i, uint64_t v) {
     for (int k = 0; k < NA; ++k) s[k] = (uint32_t(k) == i) ? v : s[k];
 }
 
+// Per-wave state shared by all output columns (wave-uniform values in LDS).
+// Pool pages and staging slots are reserved in chunks with one atomic per
+// chunk, so no counter sees more than one atomic per chunk of work.
+struct WaveState {
+    uint64_t rows;       // rows emitted (output) / tuples staged
+    uint64_t next;       // next reserved page index / staging slot
+    uint64_t end;        // end of the reserved range
+    uint64_t pad;
+};
+
 // Per-wave state of one output column's open page (wave-uniform values).
 struct OutState {
-    uint32_t page;  // pool index of the open page, kNoPage if none
-    uint32_t nr;    // rows in the page
-    uint32_t nv;    // non-NULL values in the page
-    int32_t  avail; // remaining bit budget
-    uint32_t chars; // VARCHAR: characters staged for the page
-    uint32_t seq;   // pages emitted by this wave for this column
-    uint32_t bad;   // pool exhausted: keep counting, write nothing
-    uint32_t pad;
+    uint32_t page;   // pool index of the open page, kNoPage if none
+    uint32_t nr;     // rows in the page
+    uint32_t nv;     // non-NULL values in the page
+    int32_t  avail;  // remaining bit budget
+    uint32_t chars;  // VARCHAR: characters staged for the page
+    uint32_t seq;    // pages emitted by this wave for this column
+    uint32_t bad;    // pool exhausted: keep counting, write nothing
+    uint32_t npages; // pages written (flushed to the counters at the end)
+    uint64_t nnz;    // non-NULL values written (flushed at the end)
+    uint64_t pad;
 };
-constexpr uint32_t kNoPage = 0xffffffffu;
+constexpr uint32_t kNoPage    = 0xffffffffu;
+constexpr uint32_t kPageChunk = 32; // pool pages reserved per atomic
 
 __device__ inline int32_t fixed_budget(int32_t type) { return 8 * (int32_t(kPageSize) - 4 - (type == T_INT32 ? 0 : 4)); }
 
-__device__ inline void page_open(const PipeArgs& a, OutState& st, uint64_t* bits, int32_t budget) {
-    uint32_t idx = 0;
-    if (lane_id() == 0) idx = uint32_t(atomicAdd(&a.counters->pages, 1ull));
-    idx      = __shfl(idx, 0, 64);
-    st.bad   = idx >= a.pool_pages ? 1u : 0u;
+// Wave-uniform state: every lane holds (or writes) the same values.
+__device__ inline void page_open(const PipeArgs& a, WaveState& ws, OutState& st, uint64_t* bits, int32_t budget) {
+    uint64_t next = ws.next, end = ws.end;
+    if (next == end) {
+        unsigned long long base = 0;
+        if (lane_id() == 0) base = atomicAdd(&a.counters->pages, (unsigned long long)kPageChunk);
+        base   = __shfl(base, 0, 64);
+        next   = base;
+        ws.end = base + kPageChunk;
+    }
+    uint64_t idx = next;
+    ws.next      = next + 1;
+    st.bad = idx >= a.pool_pages ? 1u : 0u;
     if (st.bad && lane_id() == 0) atomicExch(&a.counters->overflow, 1ull);
-    st.page  = idx;
+    st.page  = st.bad ? 0u : uint32_t(idx);
     st.nr    = 0;
     st.nv    = 0;
     st.chars = 0;
@@ -381,7 +519,13 @@ __device__ inline void page_open(const PipeArgs& a, OutState& st, uint64_t* bits
     bits[lane_id()] = 0;
 }
 
-// Writes header, (staged characters) and bitmap of the open page and records it in the directory.
+// Records page `page` in the directory slot of the same index.
+__device__ inline void dir_put(const PipeArgs& a, uint32_t page, uint32_t col, uint32_t stream, uint32_t seq) {
+    uint4 e = make_uint4(col, stream, seq, 1u);
+    *reinterpret_cast<uint4*>(a.dir + 4 * uint64_t(page)) = e;
+}
+
+// Writes header, (staged characters) and bitmap of the open page and records it.
 __device__ inline void page_close(const PipeArgs& a, OutState& st, uint64_t* bits, uint32_t col, uint32_t stream,
                                   const uint8_t* chars) {
     if (!st.bad) {
@@ -399,19 +543,10 @@ __device__ inline void page_close(const PipeArgs& a, OutState& st, uint64_t* bit
             if (b0 + b < nb) bm[b0 + b] = uint8_t(word >> (8 * b));
         if (lane_id() == 0) {
             *reinterpret_cast<uint32_t*>(page) = st.nr | (st.nv << 16);
-            atomicAdd(&a.counters->nnz[col], (unsigned long long)st.nv);
-            atomicAdd(&a.counters->col_pages[col], 1ull);
-            unsigned long long d = atomicAdd(&a.counters->dir, 1ull);
-            if (d < a.dir_cap) {
-                uint32_t* e = a.dir + 4 * d;
-                e[0]        = col;
-                e[1]        = stream;
-                e[2]        = st.seq;
-                e[3]        = st.page;
-            } else {
-                atomicExch(&a.counters->overflow, 1ull);
-            }
+            dir_put(a, st.page, col, stream, st.seq);
         }
+        st.npages++;
+        st.nnz += st.nv;
     }
     st.seq++;
     st.page = kNoPage;
@@ -422,23 +557,26 @@ __device__ inline void page_close(const PipeArgs& a, OutState& st, uint64_t* bit
 // (TableTarget.cpp:111-182): each row costs 1 bit, a non-NULL row the value
 // width on top; the values are written straight to the page, the bitmap is
 // held by the lanes (lane i holds rows 64i..64i+63) until the page closes.
-__device__ void append_fixed(const PipeArgs& a, OutState* stp, uint64_t* bits, int32_t type, uint64_t v,
-                             bool have, uint32_t rank, uint32_t cnt, uint32_t col, uint32_t stream) {
+__device__ inline void append_fixed(const PipeArgs& a, WaveState& ws, OutState& st, uint64_t* bits, int32_t type,
+                                    uint64_t v, bool have, uint32_t rank, uint32_t cnt, uint32_t col,
+                                    uint32_t stream) {
     const uint32_t w      = type == T_INT32 ? 4 : 8;
     const uint32_t off    = type == T_INT32 ? 4 : 8;
     const int32_t  budget = fixed_budget(type);
     const bool     valid  = have && v != kNull;
+    const uint64_t vall   = __ballot(valid);
+    const uint64_t le     = mask_le();
     uint32_t       start  = 0;
-    OutState       st     = *stp;
     while (start < cnt) {
-        if (st.page == kNoPage) page_open(a, st, bits, budget);
+        if (st.page == kNoPage) page_open(a, ws, st, bits, budget);
+        // bit cost of the batch up to this lane: 1 per row + width per value
         bool     part = have && rank >= start;
-        uint32_t cost = part ? 1u + (valid ? w * 8 : 0u) : 0u;
-        uint32_t cum  = wave_incl_scan(cost);
+        uint64_t pm   = __ballot(part);
+        uint32_t cum  = __popcll(pm & le) + 8 * w * __popcll(pm & vall & le);
         bool     fits = part && int32_t(cum) <= st.avail && (st.nr + (rank - start)) < kMaxPageRows;
         uint64_t fm   = __ballot(fits);
         uint32_t k    = __popcll(fm);
-        uint64_t vm   = __ballot(fits && valid);
+        uint64_t vm   = fm & vall;
         if (fits && valid && !st.bad) {
             uint32_t vr  = __popcll(vm & mask_lt());
             uint8_t* dst = a.pool + uint64_t(st.page) * kPageSize + off + uint64_t(st.nv + vr) * w;
@@ -446,19 +584,16 @@ __device__ void append_fixed(const PipeArgs& a, OutState* stp, uint64_t* bits, i
                 *reinterpret_cast<uint32_t*>(dst) = uint32_t(v);
             else
                 *reinterpret_cast<uint64_t*>(dst) = v;
+            uint32_t bit = st.nr + (rank - start);
+            atomicOr(reinterpret_cast<unsigned long long*>(&bits[bit >> 6]), 1ull << (bit & 63));
         }
-        uint64_t pat = wave_or((fits && valid) ? (1ull << (rank - start)) : 0ull);
-        uint32_t wi = st.nr >> 6, sh = st.nr & 63;
-        if (lane_id() == wi) bits[wi] |= pat << sh;
-        if (sh && lane_id() == wi + 1) bits[wi + 1] |= pat >> (64 - sh);
-        uint32_t used = k ? __shfl(cum, highest_lane(fm), 64) : 0;
+        uint32_t nvk = __popcll(vm);
         st.nr += k;
-        st.nv += __popcll(vm);
-        st.avail -= int32_t(used);
+        st.nv += nvk;
+        st.avail -= int32_t(k + 8 * w * nvk);
         start += k;
         if (start < cnt || st.nr >= kMaxPageRows) page_close(a, st, bits, col, stream, nullptr);
     }
-    *stp = st;
 }
 
 // Resolves a string handle to its bytes (short strings) or flags a long string.
@@ -470,8 +605,8 @@ __device__ inline const uint8_t* resolve_string(const PipeArgs& a, uint64_t h, u
         len     = 0;
         return page;
     }
-    is_long         = false;
-    uint32_t nv     = *reinterpret_cast<const uint16_t*>(page + 2);
+    is_long               = false;
+    uint32_t        nv    = *reinterpret_cast<const uint16_t*>(page + 2);
     const uint16_t* offs  = reinterpret_cast<const uint16_t*>(page + 4);
     uint32_t        end   = offs[idx];
     uint32_t        begin = idx ? offs[idx - 1] : 0;
@@ -483,22 +618,21 @@ __device__ inline const uint8_t* resolve_string(const PipeArgs& a, uint64_t h, u
 // NULL costs 1 bit, a short string 17 + 8*len bits; the end offsets go
 // straight to the page, the characters to the wave's 8 KiB staging area until
 // the page closes; a long string closes the page and is copied page by page.
-__device__ void append_varchar(const PipeArgs& a, OutState* stp, uint64_t* bits, uint8_t* stage, uint64_t v,
-                               bool have, uint32_t rank, uint32_t cnt, uint32_t col, uint32_t stream) {
-    const int32_t budget = 8 * (int32_t(kPageSize) - 4);
-    const bool    valid  = have && v != kNull;
-    uint32_t      len    = 0;
-    bool          is_long = false;
-    const uint8_t* src   = nullptr;
+__device__ void append_varchar(const PipeArgs& a, WaveState& ws, OutState& st, uint64_t* bits, uint8_t* stage,
+                               uint64_t v, bool have, uint32_t rank, uint32_t cnt, uint32_t col, uint32_t stream) {
+    const int32_t  budget  = 8 * (int32_t(kPageSize) - 4);
+    const bool     valid   = have && v != kNull;
+    uint32_t       len     = 0;
+    bool           is_long = false;
+    const uint8_t* src     = nullptr;
     if (valid) src = resolve_string(a, v, len, is_long);
-    OutState st    = *stp;
     uint32_t start = 0;
     while (start < cnt) {
         uint64_t lm      = __ballot(have && is_long && rank >= start);
         uint32_t seg_end = cnt;
         if (lm) seg_end = __shfl(rank, __ffsll((unsigned long long)lm) - 1, 64);
         while (start < seg_end) {
-            if (st.page == kNoPage) page_open(a, st, bits, budget);
+            if (st.page == kNoPage) page_open(a, ws, st, bits, budget);
             bool     part = have && rank >= start && rank < seg_end;
             uint32_t cost = part ? (valid ? 17u + 8u * len : 1u) : 0u;
             uint32_t cum  = wave_incl_scan(cost);
@@ -509,8 +643,8 @@ __device__ void append_varchar(const PipeArgs& a, OutState* stp, uint64_t* bits,
             uint32_t clen = (fits && valid) ? len : 0u;
             uint32_t cinc = wave_incl_scan(clen);
             if (fits && valid && !st.bad) {
-                uint32_t vr  = __popcll(vm & mask_lt());
-                uint8_t* pg  = a.pool + uint64_t(st.page) * kPageSize;
+                uint32_t vr     = __popcll(vm & mask_lt());
+                uint8_t* pg     = a.pool + uint64_t(st.page) * kPageSize;
                 uint32_t endoff = st.chars + cinc;
                 *reinterpret_cast<uint16_t*>(pg + 4 + 2 * (st.nv + vr)) = uint16_t(endoff);
                 uint8_t* dst = stage + (endoff - len);
@@ -535,39 +669,79 @@ __device__ void append_varchar(const PipeArgs& a, OutState* stp, uint64_t* bits,
             uint32_t       src_lane = __ffsll((unsigned long long)lm) - 1;
             uint64_t       h        = __shfl(v, src_lane, 64);
             uint32_t       cid      = handle_col(h);
-            uint64_t       p0       = handle_page(h);
+            uint64_t       p        = handle_page(h);
             const uint8_t* base     = a.col_base[cid];
             uint32_t       np       = a.col_npages[cid];
-            uint64_t       p        = p0;
             do {
-                page_open(a, st, bits, budget);
+                page_open(a, ws, st, bits, budget);
                 if (!st.bad) {
                     const uint64_t* s64 = reinterpret_cast<const uint64_t*>(base + p * kPageSize);
                     uint64_t*       d64 = reinterpret_cast<uint64_t*>(a.pool + uint64_t(st.page) * kPageSize);
                     for (uint32_t i = lane_id(); i < kPageSize / 8; i += 64) d64[i] = s64[i];
-                }
-                // record the copied page as-is (header already in place)
-                if (!st.bad && lane_id() == 0) {
-                    atomicAdd(&a.counters->col_pages[col], 1ull);
-                    unsigned long long d = atomicAdd(&a.counters->dir, 1ull);
-                    if (d < a.dir_cap) {
-                        uint32_t* e = a.dir + 4 * d;
-                        e[0]        = col;
-                        e[1]        = stream;
-                        e[2]        = st.seq;
-                        e[3]        = st.page;
-                    } else {
-                        atomicExch(&a.counters->overflow, 1ull);
-                    }
+                    if (lane_id() == 0) dir_put(a, st.page, col, stream, st.seq);
+                    st.npages++;
                 }
                 st.seq++;
                 st.page = kNoPage;
                 ++p;
             } while (p < np && *reinterpret_cast<const uint16_t*>(base + p * kPageSize) == 0xfffe);
+            st.nnz += 1;
             start += 1;
         }
     }
-    *stp = st;
+}
+
+// Stages the tuples of lanes with `have` into the wave's reserved slots;
+// a batch that does not fit the rest of the reservation spills into a fresh one.
+__device__ inline void stage_put(const PipeArgs& a, WaveState& ws, bool have, uint32_t rank, uint32_t cnt,
+                                 const uint64_t* words_src, uint32_t nwords) {
+    uint64_t next = ws.next, end = ws.end;
+    uint64_t rem  = end - next;
+    uint64_t base2 = 0;
+    if (cnt > rem) {
+        unsigned long long b = 0;
+        if (lane_id() == 0) b = atomicAdd(&a.counters->pages, (unsigned long long)a.stage_chunk);
+        base2 = __shfl(b, 0, 64);
+        if (base2 + a.stage_chunk > a.stage_cap && lane_id() == 0) atomicExch(&a.counters->overflow, 1ull);
+    }
+    if (have) {
+        uint64_t pos = rank < rem ? next + rank : base2 + (rank - rem);
+        if (pos < a.stage_cap) {
+            uint64_t* dst = a.stage + pos * a.stage_words;
+            for (uint32_t w = 0; w < nwords; ++w) dst[w] = words_src[w];
+        }
+    }
+    if (cnt > rem) {
+        ws.next = base2 + (cnt - rem);
+        ws.end  = base2 + a.stage_chunk;
+    } else {
+        ws.next = next + cnt;
+    }
+}
+
+// Closes a column's open page at the end of a wave and flushes its counters.
+__device__ inline void finish_column(const PipeArgs& a, OutState& st, uint64_t* bits, uint32_t c, uint32_t stream,
+                                     const uint8_t* stg) {
+    if (st.page != kNoPage && st.nr > 0)
+        page_close(a, st, bits, c, stream, stg);
+    else if (st.page != kNoPage && !st.bad && lane_id() == 0)
+        a.dir[4 * uint64_t(st.page) + 3] = 0; // opened, never filled
+    if (lane_id() == 0) {
+        if (st.nnz) atomicAdd(&a.counters->nnz[c], (unsigned long long)st.nnz);
+        if (st.npages) atomicAdd(&a.counters->col_pages[c], (unsigned long long)st.npages);
+    }
+}
+
+// Flushes the wave's tuple count and retires the unused part of its last
+// reservation (staging slots become holes, pool pages invalid directory entries).
+__device__ inline void finish_wave(const PipeArgs& a, const WaveState& ws, uint64_t local) {
+    uint32_t lane = lane_id();
+    if (lane == 0 && local) atomicAdd(&a.counters->tuples, (unsigned long long)local);
+    if (a.target == TARGET_STAGE) {
+        for (uint64_t i = ws.next + lane; i < ws.end && i < a.stage_cap; i += 64) a.stage[i * a.stage_words] = kEmpty;
+    } else if (a.target == TARGET_PAGES) {
+        for (uint64_t i = ws.next + lane; i < ws.end && i < a.pool_pages; i += 64) a.dir[4 * i + 3] = 0;
+    }
 }
 
 template <int NA>
@@ -578,41 +752,53 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
     const uint32_t lane   = lane_id();
 
-    // per-wave output state: kMaxOut OutState (4 u64 each) + nout x 64 bitmap words
-    OutState* ost   = reinterpret_cast<OutState*>(lds + wib * (kMaxOut * 4 + a.nout * 64));
-    uint64_t* obits = lds + wib * (kMaxOut * 4 + a.nout * 64) + kMaxOut * 4;
+    // per-wave LDS: kMaxOut OutState (6 words each), nout x 64 bitmap words
+    const uint32_t per_wave = kMaxOut * 6 + a.nout * 64;
+    OutState*      ost      = reinterpret_cast<OutState*>(lds + wib * per_wave);
+    uint64_t*      obits    = lds + wib * per_wave + kMaxOut * 6;
+    WaveState      ws{0, 0, 0, 0};
     if (a.target == TARGET_PAGES && lane < a.nout) {
-        OutState s{kNoPage, 0, 0, 0, 0, 0, 0, 0};
+        OutState s{kNoPage, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         ost[lane] = s;
     }
 
     uint64_t local_count = 0;
 
-    for (uint64_t m = stream; m < a.nmorsels; m += nwaves) {
+    // Morsel schedule. With xcd_order, workgroups are taken to be dispatched
+    // round-robin over the 8 XCDs (block b on XCD b % 8): XCD x walks the x-th
+    // eighth of the morsels in order, so its concurrently running waves touch
+    // one narrow, L2-resident range of a partition-ordered probe.
+    uint64_t m_first = stream, m_end = a.nmorsels, m_step = nwaves;
+    if (a.xcd_order) {
+        uint32_t x    = blockIdx.x % 8;
+        uint32_t bpx  = gridDim.x / 8;
+        uint64_t lo   = a.nmorsels * x / 8;
+        uint64_t hi   = a.nmorsels * (x + 1) / 8;
+        m_first       = lo + uint64_t(blockIdx.x / 8) * kWavesPerBlock + wib;
+        m_end         = hi;
+        m_step        = uint64_t(bpx) * kWavesPerBlock;
+    }
+    for (uint64_t m = m_first; m < m_end; m += m_step) {
         uint64_t r0  = m * a.morsel_rows;
         uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
-        Cursor   cur[kMaxScan];
-        if (a.driver == DRIVER_PAGES) {
-#pragma unroll
-            for (int c = 0; c < kMaxScan; ++c)
-                if (uint32_t(c) < a.nscan && c < NA) cur[c] = cursor_at(a.scan[c], r0);
-        }
         for (uint64_t rb = r0; rb < end; rb += 64) {
             uint64_t row    = rb + lane;
             bool     active = row < end;
             uint64_t s[NA];
 #pragma unroll
             for (int k = 0; k < NA; ++k) s[k] = kNull;
+            bool alive = active;
             if (a.driver == DRIVER_PAGES) {
 #pragma unroll
                 for (int c = 0; c < kMaxScan; ++c)
-                    if (uint32_t(c) < a.nscan && c < NA) s[c] = decode_step(a.scan[c], cur[c], uint32_t(row), active);
+                    if (uint32_t(c) < a.nscan && c < NA) s[c] = decode_block(a.scan[c], rb / 64, active);
             } else if (active) {
+                // materialised tuples: word 0 holds a join class, never NULL except in unused slots
+                alive = a.tuples[row * a.tuple_words] != kEmpty;
 #pragma unroll
                 for (int c = 0; c < kMaxScan; ++c)
                     if (uint32_t(c) < a.nscan && c < NA) s[c] = a.tuples[row * a.tuple_words + a.tuple_word[c]];
             }
-            bool alive = active;
 #pragma unroll
             for (int k = 0; k < NA; ++k)
                 if ((a.not_null >> k) & 1u) alive = alive && s[k] != kNull;
@@ -676,9 +862,9 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                                 if (uint32_t(k) + 1 == a.nprobes) {
                                     have = true;
                                 } else if (k + 1 < kMaxProbes) {
-                                    const ProbeSpec& Q  = a.probes[k + 1];
-                                    uint64_t         nk = sget(s, Q.key_slot);
-                                    bool pass = nk != kNull;
+                                    const ProbeSpec& Q    = a.probes[k + 1];
+                                    uint64_t         nk   = sget(s, Q.key_slot);
+                                    bool             pass = nk != kNull;
                                     if (pass && Q.bloom) {
                                         uint64_t h2 = hash_bloom(hash_key(nk));
                                         uint64_t bb = bloom_bits(h2);
@@ -703,28 +889,26 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                     uint32_t cnt  = __popcll(hm);
                     uint32_t rank = __popcll(hm & mask_lt());
                     if (a.target == TARGET_STAGE) {
-                        unsigned long long base = 0;
-                        if (lane == 0) base = atomicAdd(&a.counters->tuples, (unsigned long long)cnt);
-                        base = __shfl(base, 0, 64);
-                        if (base + cnt > a.stage_cap) {
-                            if (lane == 0) atomicExch(&a.counters->overflow, 1ull);
-                        } else if (have) {
-                            uint64_t* dst = a.stage + (base + rank) * a.stage_words;
-                            for (uint32_t w = 0; w < a.stage_words; ++w) dst[w] = sget(s, a.stage_slot[w]);
-                        }
+                        uint64_t words[kMaxWords];
+#pragma unroll
+                        for (int w = 0; w < kMaxWords; ++w)
+                            words[w] = uint32_t(w) < a.stage_words ? sget(s, a.stage_slot[w]) : 0;
+                        stage_put(a, ws, have, rank, cnt, words, a.stage_words);
+                        local_count += cnt;
                     } else if (a.target == TARGET_COUNT) {
                         local_count += cnt;
                     } else {
-                        if (lane == 0) atomicAdd(&a.counters->tuples, (unsigned long long)cnt);
+                        local_count += cnt;
                         for (uint32_t c = 0; c < a.nout; ++c) {
                             uint64_t v = sget(s, a.out[c].slot);
+                            OutState st = ost[c];
                             if (a.out[c].type == T_VARCHAR) {
                                 uint8_t* stg = a.str_stage + (uint64_t(stream) * a.nvarchar + a.out[c].vc) * kPageSize;
-                                append_varchar(a, &ost[c], obits + c * 64, stg, v, have, rank, cnt, c, stream);
+                                append_varchar(a, ws, st, obits + c * 64, stg, v, have, rank, cnt, c, stream);
                             } else {
-                                append_fixed(a, &ost[c], obits + c * 64, a.out[c].type, v, have, rank, cnt, c,
-                                             stream);
+                                append_fixed(a, ws, st, obits + c * 64, a.out[c].type, v, have, rank, cnt, c, stream);
                             }
+                            ost[c] = st;
                         }
                     }
                 }
@@ -733,26 +917,164 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
         }
     }
 
-    if (a.target == TARGET_COUNT) {
-        if (lane == 0 && local_count) atomicAdd(&a.counters->tuples, (unsigned long long)local_count);
-    } else if (a.target == TARGET_PAGES) {
+    if (a.target == TARGET_PAGES) {
         for (uint32_t c = 0; c < a.nout; ++c) {
             OutState st = ost[c];
-            if (st.page != kNoPage && st.nr > 0) {
-                const uint8_t* stg = nullptr;
-                if (a.out[c].type == T_VARCHAR)
-                    stg = a.str_stage + (uint64_t(stream) * a.nvarchar + a.out[c].vc) * kPageSize;
-                page_close(a, st, obits + c * 64, c, stream, stg);
-                ost[c] = st;
+            const uint8_t* stg = a.out[c].type == T_VARCHAR
+                                     ? a.str_stage + (uint64_t(stream) * a.nvarchar + a.out[c].vc) * kPageSize
+                                     : nullptr;
+            finish_column(a, st, obits + c * 64, c, stream, stg);
+        }
+    }
+    finish_wave(a, ws, local_count);
+}
+
+// Lean pipeline for the dominant shape: one probe, no residual predicates,
+// fixed-width outputs (or a count). Writer state lives in registers and the
+// probe is a flat linear-probing loop, which keeps the instruction count per
+// 64-row block a fraction of the general kernel's.
+constexpr int kLeanOut    = 4;
+constexpr int kLeanScan   = 4;
+constexpr int kLeanUnroll = 1; // 64-row blocks in flight per wave (4 measured slower: occupancy 3, 4x code)
+
+template <int NA>
+__global__ __launch_bounds__(256) void k_probe1(PipeArgs a) {
+    extern __shared__ uint64_t lds[];
+    const uint32_t wib    = threadIdx.x / 64;
+    const uint32_t stream = blockIdx.x * kWavesPerBlock + wib;
+    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+    const uint32_t lane   = lane_id();
+    uint64_t*      obits  = lds + wib * (kLeanOut * 64);
+    WaveState      ws{0, 0, 0, 0};
+    OutState       st[kLeanOut];
+#pragma unroll
+    for (int c = 0; c < kLeanOut; ++c) st[c] = OutState{kNoPage, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const ProbeSpec& P     = a.probes[0];
+    uint64_t         local = 0;
+
+    uint64_t m_first = stream, m_end = a.nmorsels, m_step = nwaves;
+    if (a.xcd_order) {
+        uint32_t x   = blockIdx.x % 8;
+        uint32_t bpx = gridDim.x / 8;
+        m_first      = a.nmorsels * x / 8 + uint64_t(blockIdx.x / 8) * kWavesPerBlock + wib;
+        m_end        = a.nmorsels * (x + 1) / 8;
+        m_step       = uint64_t(bpx) * kWavesPerBlock;
+    }
+    // Blocks are processed kLeanUnroll at a time: all their index, value and
+    // first hash-table loads are issued before any of them is consumed, so a
+    // wave keeps several dependent-load chains in flight.
+    for (uint64_t m = m_first; m < m_end; m += m_step) {
+        uint64_t r0  = m * a.morsel_rows;
+        uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
+        for (uint64_t rb = r0; rb < end; rb += 64 * kLeanUnroll) {
+            uint64_t s[kLeanUnroll][NA];
+            bool     alive[kLeanUnroll];
+#pragma unroll
+            for (int u = 0; u < kLeanUnroll; ++u) {
+                uint64_t row    = rb + 64 * u + lane;
+                bool     active = row < end;
+#pragma unroll
+                for (int k = 0; k < NA; ++k) s[u][k] = kNull;
+                alive[u] = active;
+                if (a.driver == DRIVER_PAGES) {
+#pragma unroll
+                    for (int c = 0; c < kLeanScan; ++c)
+                        if (uint32_t(c) < a.nscan && c < NA) s[u][c] = decode_block(a.scan[c], rb / 64 + u, active);
+                } else if (active) {
+                    alive[u] = a.tuples[row * a.tuple_words] != kEmpty;
+#pragma unroll
+                    for (int c = 0; c < kLeanScan; ++c)
+                        if (uint32_t(c) < a.nscan && c < NA) s[u][c] = a.tuples[row * a.tuple_words + a.tuple_word[c]];
+                }
+#pragma unroll
+                for (int k = 0; k < NA; ++k)
+                    if ((a.not_null >> k) & 1u) alive[u] = alive[u] && s[u][k] != kNull;
+            }
+            uint64_t key[kLeanUnroll], p[kLeanUnroll], kw[kLeanUnroll];
+#pragma unroll
+            for (int u = 0; u < kLeanUnroll; ++u) {
+                key[u]     = sget(s[u], P.key_slot);
+                uint64_t h = hash_key(key[u]);
+                if (alive[u] && P.bloom) {
+                    uint64_t h2 = hash_bloom(h);
+                    uint64_t bb = bloom_bits(h2);
+                    alive[u]    = (P.bloom[bloom_word(h2, P.bloom_log)] & bb) == bb;
+                }
+                p[u]  = h >> P.shift;
+                kw[u] = alive[u] ? P.ht[p[u] * P.words] : kEmpty;
+            }
+#pragma unroll
+            for (int u = 0; u < kLeanUnroll; ++u) {
+                bool     live = alive[u];
+                uint64_t q    = p[u];
+                uint64_t w0   = kw[u]; // key word of slot q, already loaded
+                bool     first = true;
+                for (;;) {
+                    bool have = false;
+                    while (live) {
+                        uint64_t k0 = first ? w0 : P.ht[q * P.words];
+                        first       = false;
+                        if (k0 == kEmpty) {
+                            live = false;
+                            break;
+                        }
+                        uint64_t hit_slot = q;
+                        q = (q & ~P.mask) | ((q + 1) & P.mask);
+                        if (k0 == key[u]) {
+                            const uint64_t* e = P.ht + hit_slot * P.words;
+                            for (uint32_t t = 0; t < P.nattrs; ++t) sset(s[u], P.attr_slot[t], e[P.attr_word[t]]);
+                            have = true;
+                            if (P.unique) live = false;
+                            break;
+                        }
+                    }
+                    uint64_t hm = __ballot(have);
+                    if (hm) {
+                        uint32_t cnt  = __popcll(hm);
+                        uint32_t rank = __popcll(hm & mask_lt());
+                        local += cnt;
+                        if (a.target == TARGET_PAGES) {
+#pragma unroll
+                            for (int c = 0; c < kLeanOut; ++c)
+                                if (uint32_t(c) < a.nout)
+                                    append_fixed(a, ws, st[c], obits + c * 64, a.out[c].type,
+                                                 sget(s[u], a.out[c].slot), have, rank, cnt, uint32_t(c), stream);
+                        }
+                    }
+                    if (!__ballot(live)) break;
+                }
             }
         }
     }
+    if (a.target == TARGET_PAGES) {
+#pragma unroll
+        for (int c = 0; c < kLeanOut; ++c)
+            if (uint32_t(c) < a.nout) finish_column(a, st[c], obits + c * 64, uint32_t(c), stream, nullptr);
+    }
+    finish_wave(a, ws, local);
+}
+
+bool probe1_applicable(const PipeArgs& a, uint32_t slots) {
+    return slots <= 8 && a.nprobes == 1 && a.probes[0].nchecks == 0 && a.nscan <= uint32_t(kLeanScan) &&
+           (a.target == TARGET_COUNT || (a.target == TARGET_PAGES && a.nvarchar == 0 && a.nout <= uint32_t(kLeanOut)));
+}
+
+hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s) {
+    uint32_t blocks = (grid_waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks == 0) blocks = 1;
+    if (!probe1_applicable(a, slots)) return hipErrorInvalidValue;
+    size_t lds = size_t(kWavesPerBlock) * kLeanOut * 64 * sizeof(uint64_t);
+    if (slots <= 4)
+        k_probe1<4><<<dim3(blocks), dim3(256), lds, s>>>(a);
+    else
+        k_probe1<8><<<dim3(blocks), dim3(256), lds, s>>>(a);
+    return hipGetLastError();
 }
 
 hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s) {
     uint32_t blocks = (grid_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks == 0) blocks = 1;
-    size_t lds = size_t(kWavesPerBlock) * (kMaxOut * 4 + a.nout * 64) * sizeof(uint64_t);
+    size_t lds = size_t(kWavesPerBlock) * (kMaxOut * 6 + a.nout * 64) * sizeof(uint64_t);
     if (slots <= 4)
         k_pipeline<4><<<dim3(blocks), dim3(256), lds, s>>>(a);
     else if (slots <= 8)

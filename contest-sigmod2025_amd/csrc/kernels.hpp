@@ -45,8 +45,8 @@ enum : uint32_t { DRIVER_PAGES = 0, DRIVER_TUPLES = 1 };
 // Counters written by a pipeline launch (device memory).
 struct PipeCounters {
     unsigned long long tuples;    // staged tuples / output rows / counted tuples
-    unsigned long long pages;     // output pages allocated from the pool
-    unsigned long long dir;       // directory entries written
+    unsigned long long pages;     // pool pages (or staging slots) reserved
+    unsigned long long dir;       // unused
     unsigned long long overflow;  // nonzero: a buffer was too small, re-run
     unsigned long long nnz[kMaxOut];       // non-NULL values written per output column
     unsigned long long col_pages[kMaxOut]; // pages written per output column
@@ -57,9 +57,10 @@ struct PipeArgs {
     uint32_t        driver;
     uint32_t        nscan;
     uint64_t        nrows;
-    uint32_t        morsel_rows; // multiple of kIndexRows
+    uint32_t        morsel_rows; // multiple of 64
     uint32_t        not_null;    // bitmask of tuple slots that must be non-NULL
     uint64_t        nmorsels;
+    uint32_t        xcd_order;   // morsel range split across the 8 XCDs (grid multiple of 8 blocks)
     ScanCol         scan[kMaxScan];
     const uint64_t* tuples;      // DRIVER_TUPLES: rows x tuple_words
     uint32_t        tuple_words;
@@ -72,28 +73,46 @@ struct PipeArgs {
     uint32_t        stage_words;
     uint32_t        stage_slot[kMaxWords];
     uint64_t*       stage;
-    uint64_t        stage_cap;   // tuples
+    uint64_t        stage_cap;   // tuple slots; unused slots carry key kEmpty
+    uint64_t        stage_chunk; // slots reserved per atomic (>= 64)
     uint32_t        nout;
     uint32_t        nvarchar;
     OutSpec         out[kMaxOut];
     uint8_t*        pool;        // output page pool
     uint64_t        pool_pages;
-    uint32_t*       dir;         // 4 x u32 per page: column, stream, seq, page index
-    uint64_t        dir_cap;
+    uint32_t*       dir;         // per pool page: column, wave stream, sequence, valid
+    uint64_t        dir_cap;     // unused (the directory has pool_pages entries)
     uint8_t*        str_stage;   // per wave x VARCHAR column: 8 KiB
     const uint8_t* const* col_base;  // pages of column id (string handles)
     const uint32_t* col_npages;
     PipeCounters*   counters;
 };
 
-// Column preparation: rows per page prefix and the morsel index.
-hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t type, uint32_t* row_start,
-                                 uint32_t* idx_page, uint32_t* idx_nn, uint32_t* scratch,
-                                 uint64_t* total_rows_dev, hipStream_t s);
+// Radix partitioning of a driving table by the hash of its probe key.
+struct RadixArgs {
+    ScanCol   scan[kMaxScan]; // column 0 is the probe key
+    uint32_t  nscan;
+    uint32_t  not_null;       // columns that must be non-NULL (join columns)
+    uint64_t  nrows;
+    uint32_t  rshift;         // partition = hash_key(key) >> rshift
+    uint32_t  nparts;
+    uint64_t  blocks_per_wg;  // 64-row blocks per workgroup
+    uint32_t* bh;             // nparts x workgroups counters, scanned into offsets
+    uint64_t* out;            // tuples of nscan words, in partition order
+};
+// Histogram, scan and scatter; `total` receives the number of tuples written.
+hipError_t launch_radix_partition(const RadixArgs& a, uint32_t wgs, uint32_t* scan_scratch, uint64_t* total,
+                                  hipStream_t s);
+
+// Column preparation: rows per page prefix and the block index.
+hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t type, uint64_t nrows,
+                                 uint32_t* row_start, BlockIdx* blocks, uint32_t* scratch, uint64_t* total_rows_dev,
+                                 hipStream_t s);
 size_t     prepare_scratch_words(uint64_t npages);
 
 hipError_t launch_decode(const ScanCol& c, uint64_t nrows, uint64_t* out, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_copy(void* dst, const void* src, uint64_t n16, hipStream_t s);
 hipError_t launch_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,
                             uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* dup_flag,
                             hipStream_t s);
@@ -102,12 +121,18 @@ hipError_t launch_exclusive_scan(uint32_t* data, uint64_t n, uint32_t* scratch, 
 // Radix-partitioned build of a table of 2^log_cap slots split into
 // 2^log_regions regions; `bh` holds blocks << log_regions counters, `parts`
 // n x words tuples. flags[0] = duplicate keys, flags[1] = a region overflowed.
-hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* parts, uint32_t* bh,
+// `n` = staging slots (slots with key kEmpty are skipped), `n_real` = tuples.
+hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real, uint32_t words, uint64_t* parts,
+                             uint32_t* bh,
                              uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch, uint64_t* total,
                              uint64_t* ht, uint32_t log_cap, uint32_t log_regions, uint64_t* bloom,
                              uint32_t bloom_log, uint32_t* flags, hipStream_t s);
 // Launches the fused pipeline. `slots` = tuple width (selects the kernel variant).
 hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s);
+// Lean variant for one probe without residual predicates and a count or
+// fixed-width page target (<= 4 scanned, <= 4 output columns, <= 8 slots).
+hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s);
+bool       probe1_applicable(const PipeArgs& a, uint32_t slots);
 uint32_t   pipeline_waves_per_block();
 
 // Synthetic generators.

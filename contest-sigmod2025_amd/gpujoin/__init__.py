@@ -88,6 +88,7 @@ class Stats(C.Structure):
         ("output_ref_pages", C.c_uint64),
         ("build_bytes", C.c_uint64),
         ("ms_probe_kernel", C.c_double),
+        ("ms_probe_partition", C.c_double),
     ]
 
     def as_dict(self):
@@ -134,6 +135,11 @@ SIGNATURES = {
     "cgj_dtable_generate": (C.c_void_p, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(GenColumn), C.c_uint32]),
     "cgj_host_generate": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(GenColumn),
                                     C.POINTER(ResultColumn)]),
+    "cgj_count_device": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
+                                   C.POINTER(C.c_uint64)]),
+    "cgj_copy_probe": (C.c_double, [C.c_void_p, C.c_uint64, C.c_uint32]),
+    "cgj_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    "cgj_stats_size": (C.c_size_t, []),
 }
 
 _lib = None
@@ -408,6 +414,26 @@ class Context:
         if lib().cgj_execute_device(self.handle, C.byref(m.plan), arr, len(tables), C.byref(out)) != 0:
             raise RuntimeError(self.error())
         return DeviceResult(self, out.value)
+
+    def count_device(self, plan: Plan, tables: Sequence[DeviceTable]) -> int:
+        """Result cardinality of the plan over device tables (no pages written)."""
+        m = _Marshal(plan, [])
+        arr = (C.c_void_p * max(len(tables), 1))(*[t.handle for t in tables])
+        out = C.c_uint64()
+        if lib().cgj_count_device(self.handle, C.byref(m.plan), arr, len(tables), C.byref(out)) != 0:
+            raise RuntimeError(self.error())
+        return out.value
+
+    def copy_probe(self, nbytes: int, reps: int = 5) -> float:
+        """Mean time (ms) of a plain HBM-to-HBM copy of `nbytes` bytes."""
+        ms = lib().cgj_copy_probe(self.handle, nbytes, reps)
+        if ms < 0:
+            raise RuntimeError(self.error())
+        return ms
+
+    def set_option(self, name: str, value: float):
+        if lib().cgj_set_option(self.handle, name.encode(), float(value)) != 0:
+            raise ValueError(f"unknown option {name}")
 
     def stats(self) -> dict:
         s = Stats()

@@ -113,7 +113,20 @@ typedef struct {
                                    fixed-width column, pages written otherwise */
     uint64_t build_bytes;       /* compact hash-table tuples: tuples x words x 8 */
     double   ms_probe_kernel;   /* HIP-event time of the output pipeline kernel */
+    double   ms_probe_partition;/* HIP-event time of the probe side's radix partitioning */
 } cgj_stats;
+
+/* Execution options of a context (0 = success, -1 = unknown name):
+ *   "radix_min_table_bytes" / "radix_min_rows": the probe side is radix-partitioned
+ *       when the first probed hash table and the driving table exceed both;
+ *   "max_load": hash-table load factor (0 = automatic);
+ *   "bloom_bits": bloom bits per key (-1 = automatic, 0 = none);
+ *   "partitioned_build": radix-partitioned LDS build (-1 automatic, 0 off, 1 on);
+ *   "lean_kernel": 1 (default) runs one-probe pipelines on the lean kernel, 0 on the general one. */
+int cgj_set_option(cgj_context* ctx, const char* name, double value);
+
+/* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
+size_t cgj_stats_size(void);
 
 /* ---- context (reference engine/Execute.cpp:42-48) ---- */
 
@@ -153,6 +166,18 @@ int      cgj_dresult_download(cgj_context* ctx, const cgj_dresult* r, cgj_result
 void     cgj_dresult_free(cgj_context* ctx, cgj_dresult* r);
 
 int      cgj_last_stats(const cgj_context* ctx, cgj_stats* out);
+
+/* Counts the result rows of the plan without materialising pages: the same
+ * builds and probe chain, with a counting target. Used for exact sub-plan
+ * cardinalities (the estimator's training labels and q-error judge). */
+int      cgj_count_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                          uint64_t* rows);
+
+/* Bandwidth probe: copies `bytes` bytes between two HBM buffers with a plain
+ * streaming kernel, `reps` times; returns the mean kernel time in ms (or a
+ * negative value on error). Calibrates HBM byte counters and reports the
+ * achievable copy bandwidth next to the roofline peak. */
+double   cgj_copy_probe(cgj_context* ctx, uint64_t bytes, uint32_t reps);
 
 /* ---- operator-level entry points ---- */
 

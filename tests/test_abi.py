@@ -27,6 +27,12 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(gj.SIGNATURES), "ctypes signatures and header disagree"
 
 
+def test_stats_struct_matches_binding():
+    import ctypes
+
+    assert gj.lib().cgj_stats_size() == ctypes.sizeof(gj.Stats)
+
+
 def test_contest_symbols_exported():
     """The C++ contest surface (Contest::build_context/destroy_context/execute)."""
     import subprocess

@@ -207,6 +207,39 @@ def _workload_plan(wl):
     return plan
 
 
+@pytest.fixture
+def radix_ctx(ctx):
+    """The context with the probe-side radix partitioning forced on for any size."""
+    ctx.set_option("radix_min_table_bytes", 0)
+    ctx.set_option("radix_min_rows", 0)
+    yield ctx
+    ctx.set_option("radix_min_table_bytes", 64 << 20)
+    ctx.set_option("radix_min_rows", 4 << 20)
+
+
+def test_radix_probe_c2_recipe(radix_ctx):
+    """Partitioned probe side + XCD-ordered walk: same rows as the oracle."""
+    from gpujoin import workloads
+
+    plan = _workload_plan(workloads.c2(0.005))
+    say("radix probe on c2 recipe at 0.5% scale")
+    s = _compare_large(radix_ctx, plan)
+    assert s["ms_probe_partition"] > 0, s
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_radix_probe_random_plans(radix_ctx, seed):
+    """Multi-way plans (VARCHAR, duplicates, NULLs) with every probe side partitioned."""
+    rng = random.Random(5000 + seed)
+    ntables = rng.randint(2, 5)
+    max_rows = rng.choice([50, 400, 3000])
+    plan = random_plan(rng, ntables=ntables, max_rows=max_rows, key_type=rng.choice([INT32, INT64]),
+                       domain=max(max_rows, 30), long_strings=seed % 4 == 0)
+    say(f"radix random plan seed={seed}")
+    types = [t for _, t in plan.nodes[plan.root].output_attrs]
+    assert check_table(radix_ctx.execute(plan), types) == check_table(oracle.execute(plan), types)
+
+
 def test_c2_recipe_scaled_vs_oracle(ctx):
     """The 50M x 500M INT64 Zipf workload at 1% scale (500K x 5M), 5% NULLs."""
     from gpujoin import workloads
