@@ -49,6 +49,10 @@ __host__ __device__ inline uint32_t handle_idx(uint64_t h) { return uint32_t(h &
 
 // ---- wave64 collectives ----
 __device__ inline uint32_t lane_id() { return __lane_id(); }
+// Index of the wave in its workgroup. readfirstlane tells the compiler the
+// value is wave-uniform, so loop control derived from it stays in scalar
+// registers and index loads become scalar loads.
+__device__ inline uint32_t wave_in_block() { return __builtin_amdgcn_readfirstlane(threadIdx.x / 64); }
 __device__ inline uint64_t mask_lt() {
     uint32_t l = lane_id();
     return l ? (~0ull >> (64 - l)) : 0ull;

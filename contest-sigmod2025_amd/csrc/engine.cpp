@@ -91,20 +91,20 @@ Context::Context(int device) : device_(device) {
 
 Context::~Context() {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    heap_.trim();
+    heap_->trim();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 void Context::prepare_column(DevColumn& c) {
-    c.row_start = DevBuf(&heap_, (c.npages + 1) * sizeof(uint32_t));
-    c.blocks    = DevBuf(&heap_, ((c.nrows + 63) / 64 + 1) * sizeof(BlockIdx));
+    c.row_start = DevBuf(heap_.get(), (c.npages + 1) * sizeof(uint32_t));
+    c.blocks    = DevBuf(heap_.get(), ((c.nrows + 63) / 64 + 1) * sizeof(BlockIdx));
     if (c.npages == 0) {
         CGJ_HIP(hipMemsetAsync(c.row_start.get(), 0, sizeof(uint32_t), stream_));
         if (c.nrows) throw std::runtime_error("column has rows but no pages");
         return;
     }
-    DevBuf scratch(&heap_, prepare_scratch_words(c.npages) * sizeof(uint32_t) + 16);
-    DevBuf total(&heap_, sizeof(uint64_t));
+    DevBuf scratch(heap_.get(), prepare_scratch_words(c.npages) * sizeof(uint32_t) + 16);
+    DevBuf total(heap_.get(), sizeof(uint64_t));
     CGJ_HIP(launch_prepare_column(c.pages.get<uint8_t>(), c.npages, c.type, c.nrows, c.row_start.get<uint32_t>(),
                                   c.blocks.get<BlockIdx>(), scratch.get<uint32_t>(), total.get<uint64_t>(), stream_));
     uint64_t rows = 0;
@@ -132,7 +132,7 @@ std::unique_ptr<DevTable> Context::upload(const cgj_host_table& t, const std::ve
                 continue;
             }
             c.npages = hc.num_pages;
-            c.pages  = DevBuf(&heap_, c.npages * kPageSize + 64);
+            c.pages  = DevBuf(heap_.get(), c.npages * kPageSize + 64);
             uint64_t per = kChunk / kPageSize;
             for (uint64_t p0 = 0; p0 < c.npages; p0 += per) {
                 uint64_t n = std::min(per, c.npages - p0);
@@ -218,9 +218,9 @@ std::unique_ptr<DevTable> Context::generate(uint64_t seed, uint64_t nrows, const
         c.type   = g.type;
         c.nrows  = nrows;
         c.npages = nrows ? rs.size() - 1 : 0;
-        c.pages  = DevBuf(&heap_, c.npages * kPageSize + 64);
+        c.pages  = DevBuf(heap_.get(), c.npages * kPageSize + 64);
         if (c.npages) {
-            DevBuf drs(&heap_, rs.size() * sizeof(uint32_t));
+            DevBuf drs(heap_.get(), rs.size() * sizeof(uint32_t));
             CGJ_HIP(hipMemcpyAsync(drs.get(), rs.data(), rs.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
             CGJ_HIP(hipMemsetAsync(c.pages.get(), 0, c.npages * kPageSize, stream_));
             CGJ_HIP(launch_gen_fill(to_spec(g, seed), drs.get<uint32_t>(), c.npages, c.pages.get<uint8_t>(), stream_));
@@ -241,12 +241,13 @@ JoinTuning Context::tune(uint64_t build_rows, uint64_t probe_rows, uint32_t key_
 }
 
 bool Context::set_option(const std::string& name, double v) {
-    if (name == "radix_min_table_bytes") options.radix_min_table_bytes = uint64_t(v);
+    if (name == "radix_min_table_bytes") options.radix_min_table_bytes = v >= 1.8e19 ? ~uint64_t(0) : uint64_t(v);
     else if (name == "radix_min_rows") options.radix_min_rows = uint64_t(v);
     else if (name == "max_load") options.max_load = (v > 0 && v < 1) ? v : 0;
     else if (name == "bloom_bits") options.bloom_bits = int32_t(v);
     else if (name == "partitioned_build") options.partitioned_build = int32_t(v);
     else if (name == "lean_kernel") options.lean_kernel = v != 0;
+    else if (name == "lean_waves_per_cu") options.lean_waves_per_cu = v >= 4 && v <= 64 ? uint32_t(v) : 32;
     else return false;
     return true;
 }
@@ -360,7 +361,14 @@ private:
     const HashTable* make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est);
     void pipeline(Driver d, std::vector<ProbeStep> probes, Target& t);
     uint64_t launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms);
-    uint64_t grid_waves(uint64_t nrows, bool xcd_order = false) const;
+    uint64_t grid_waves(uint64_t nrows, bool xcd_order = false, uint32_t per_cu = 16) const;
+    // Whether the pipeline runs on the lean one-probe kernel, and its grid.
+    bool     use_lean(const PipeArgs& a, uint32_t nslots) const {
+        return ctx_.options.lean_kernel && probe1_applicable(a, nslots);
+    }
+    uint64_t waves_for(const PipeArgs& a, uint32_t nslots, uint64_t nrows) const {
+        return grid_waves(nrows, a.xcd_order != 0, use_lean(a, nslots) ? ctx_.options.lean_waves_per_cu : 16);
+    }
     const Tuples* partition_driver(uint32_t rel, uint32_t key_eq, uint64_t table_bytes);
     static constexpr uint32_t kMorselRows = 4096;
 };
@@ -399,9 +407,9 @@ void Executor::register_all_columns() {
                            ctx_.stream()));
 }
 
-uint64_t Executor::grid_waves(uint64_t nrows, bool xcd_order) const {
+uint64_t Executor::grid_waves(uint64_t nrows, bool xcd_order, uint32_t per_cu) const {
     uint64_t morsels = (nrows + kMorselRows - 1) / kMorselRows;
-    uint64_t waves   = std::min<uint64_t>(std::max<uint64_t>(morsels, 1), uint64_t(ctx_.cu_count()) * 16);
+    uint64_t waves   = std::min<uint64_t>(std::max<uint64_t>(morsels, 1), uint64_t(ctx_.cu_count()) * per_cu);
     uint32_t per     = pipeline_waves_per_block() * (xcd_order ? 8 : 1);
     return (waves + per - 1) / per * per;
 }
@@ -460,10 +468,10 @@ uint64_t Executor::launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& 
     a.col_base    = col_base_.get<const uint8_t* const>();
     a.col_npages  = col_npages_.get<uint32_t>();
     CGJ_HIP(hipMemsetAsync(a.counters, 0, sizeof(PipeCounters), ctx_.stream()));
-    uint64_t waves = grid_waves(nrows, a.xcd_order != 0);
+    uint64_t waves = waves_for(a, nslots, nrows);
     CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
     if (a.nmorsels) {
-        if (ctx_.options.lean_kernel && probe1_applicable(a, nslots))
+        if (use_lean(a, nslots))
             CGJ_HIP(launch_probe1(a, nslots, uint32_t(waves), ctx_.stream()));
         else
             CGJ_HIP(launch_pipeline(a, nslots, uint32_t(waves), ctx_.stream()));
@@ -754,7 +762,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         a.out[i].vc   = a.out[i].type == CGJ_VARCHAR ? a.nvarchar++ : 0;
         bytes_per_row += a.out[i].type == CGJ_INT32 ? 4.2 : (a.out[i].type == CGJ_VARCHAR ? 48.0 : 8.2);
     }
-    uint64_t waves   = grid_waves(nrows, xcd_order);
+    uint64_t waves   = waves_for(a, nslots, nrows);
     DevBuf   stage_s = DevBuf(&ctx_.heap(), std::max<uint64_t>(waves * a.nvarchar * kPageSize, 64));
     a.str_stage      = stage_s.get<uint8_t>();
     // every wave may hold one partly used reservation of pool pages
@@ -843,7 +851,7 @@ uint64_t Context::count(const cgj_plan& plan, const std::vector<const DevTable*>
 
 double Context::copy_probe(uint64_t bytes, uint32_t reps) {
     uint64_t   words = std::max<uint64_t>(bytes / 16, 1);
-    DevBuf     a(&heap_, words * 16), b(&heap_, words * 16);
+    DevBuf     a(heap_.get(), words * 16), b(heap_.get(), words * 16);
     hipEvent_t e0, e1;
     CGJ_HIP(hipEventCreate(&e0));
     CGJ_HIP(hipEventCreate(&e1));

@@ -30,8 +30,9 @@ struct HipError : std::runtime_error {
     } while (0)
 
 // Size-bucketed caching allocator over hipMalloc: repeated queries of the same
-// shape reuse their buffers instead of paying for hipMalloc/hipFree.
-class DeviceHeap {
+// shape reuse their buffers instead of paying for hipMalloc/hipFree. Buffers
+// share ownership of the heap, so tables and results may outlive their context.
+class DeviceHeap : public std::enable_shared_from_this<DeviceHeap> {
 public:
     ~DeviceHeap();
     void* alloc(size_t bytes);
@@ -48,13 +49,14 @@ private:
 class DevBuf {
 public:
     DevBuf() = default;
-    DevBuf(DeviceHeap* h, size_t bytes) : heap_(h), bytes_(bytes), ptr_(bytes ? h->alloc(bytes) : nullptr) {}
+    DevBuf(DeviceHeap* h, size_t bytes)
+        : heap_(h->shared_from_this()), bytes_(bytes), ptr_(bytes ? h->alloc(bytes) : nullptr) {}
     ~DevBuf() { reset(); }
-    DevBuf(DevBuf&& o) noexcept : heap_(o.heap_), bytes_(o.bytes_), ptr_(o.ptr_) { o.ptr_ = nullptr; }
+    DevBuf(DevBuf&& o) noexcept : heap_(std::move(o.heap_)), bytes_(o.bytes_), ptr_(o.ptr_) { o.ptr_ = nullptr; }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
             reset();
-            heap_  = o.heap_;
+            heap_  = std::move(o.heap_);
             bytes_ = o.bytes_;
             ptr_   = o.ptr_;
             o.ptr_ = nullptr;
@@ -72,9 +74,9 @@ public:
     size_t bytes() const { return bytes_; }
 
 private:
-    DeviceHeap* heap_  = nullptr;
-    size_t      bytes_ = 0;
-    void*       ptr_   = nullptr;
+    std::shared_ptr<DeviceHeap> heap_;
+    size_t                      bytes_ = 0;
+    void*                       ptr_   = nullptr;
 };
 
 // One column of a device-resident table.
@@ -120,12 +122,16 @@ struct JoinTuning {
 
 // Execution options (cgj_set_option); automatic choices unless overridden.
 struct Options {
-    uint64_t radix_min_table_bytes = uint64_t(64) << 20; // partition the probe side above this table size
+    // Probe-side radix partitioning: its scatter costs more than the cache
+    // misses it saves on MI355X (HBM latency is hidden by occupancy), so it is
+    // off unless a size threshold is set.
+    uint64_t radix_min_table_bytes = ~uint64_t(0);       // partition the probe side above this table size
     uint64_t radix_min_rows        = uint64_t(4) << 20;  // ... and this many driving rows
     double   max_load              = 0;                  // hash-table load factor (0 = automatic)
     int32_t  bloom_bits            = -1;                 // bloom bits per key (-1 = automatic)
     int32_t  partitioned_build     = -1;                 // radix-partitioned build (-1 = automatic)
     bool     lean_kernel           = true;               // use the one-probe kernel where it applies
+    uint32_t lean_waves_per_cu     = 32;                 // lean kernel grid: waves per CU (8 per SIMD)
 };
 
 class Context {
@@ -139,7 +145,7 @@ public:
 
     int         device() const { return device_; }
     hipStream_t stream() const { return stream_; }
-    DeviceHeap& heap() { return heap_; }
+    DeviceHeap& heap() { return *heap_; }
 
     // Uploads the table's pages; with `used`, only the flagged columns.
     std::unique_ptr<DevTable> upload(const cgj_host_table& t, const std::vector<bool>* used = nullptr);
@@ -164,7 +170,7 @@ private:
     int         device_;
     uint32_t    cus_    = 256;
     hipStream_t stream_ = nullptr;
-    DeviceHeap  heap_;
+    std::shared_ptr<DeviceHeap> heap_ = std::make_shared<DeviceHeap>();
 };
 
 } // namespace cgj

@@ -391,7 +391,7 @@ __global__ __launch_bounds__(256) void k_radix_hist(RadixArgs a) {
     uint64_t nb    = (a.nrows + 63) / 64;
     uint64_t b0    = uint64_t(blockIdx.x) * a.blocks_per_wg;
     uint64_t b1    = b0 + a.blocks_per_wg < nb ? b0 + a.blocks_per_wg : nb;
-    uint32_t wave  = threadIdx.x / 64;
+    uint32_t wave  = wave_in_block();
     for (uint64_t b = b0 + wave; b < b1; b += blockDim.x / 64) {
         uint64_t row    = b * 64 + lane_id();
         bool     active = row < a.nrows;
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(RadixArgs a) {
     uint64_t nb   = (a.nrows + 63) / 64;
     uint64_t b0   = uint64_t(blockIdx.x) * a.blocks_per_wg;
     uint64_t b1   = b0 + a.blocks_per_wg < nb ? b0 + a.blocks_per_wg : nb;
-    uint32_t wave = threadIdx.x / 64;
+    uint32_t wave = wave_in_block();
     for (uint64_t b = b0 + wave; b < b1; b += blockDim.x / 64) {
         uint64_t row    = b * 64 + lane_id();
         bool     active = row < a.nrows;
@@ -747,7 +747,7 @@ __device__ inline void finish_wave(const PipeArgs& a, const WaveState& ws, uint6
 template <int NA>
 __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
     extern __shared__ uint64_t lds[];
-    const uint32_t wib    = threadIdx.x / 64;
+    const uint32_t wib    = wave_in_block();
     const uint32_t stream = blockIdx.x * kWavesPerBlock + wib;
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
     const uint32_t lane   = lane_id();
@@ -929,28 +929,211 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
     finish_wave(a, ws, local_count);
 }
 
-// Lean pipeline for the dominant shape: one probe, no residual predicates,
-// fixed-width outputs (or a count). Writer state lives in registers and the
-// probe is a flat linear-probing loop, which keeps the instruction count per
-// 64-row block a fraction of the general kernel's.
-constexpr int kLeanOut    = 4;
-constexpr int kLeanScan   = 4;
-constexpr int kLeanUnroll = 1; // 64-row blocks in flight per wave (4 measured slower: occupancy 3, 4x code)
+// ---------------------------------------------------------------------------
+// lean pipeline: one probe without residual predicates whose only scanned
+// join column is the probe key, and up to four fixed-width output columns
+// (or a count)
+// ---------------------------------------------------------------------------
+// Each output column names where its value comes from: the probe key, a
+// column of the driving table (decoded straight into the output register), or
+// a word of the matching hash-table entry. Without tuple slots there are no
+// runtime-indexed register selects. The page-writer state is wave-uniform
+// (scalar registers) and each output column's validity bitmap lives in one
+// 64-bit register per lane (lane i holds rows 64i..64i+63 of the open page):
+// a 64-row batch is appended to a column with one ballot, one masked store and
+// a register OR; the batch's validity in row order comes from a single forward
+// lane permute shared by all columns.
+constexpr int kLeanOut = 4;
 
-template <int NA>
-__global__ __launch_bounds__(256) void k_probe1(PipeArgs a) {
-    extern __shared__ uint64_t lds[];
-    const uint32_t wib    = threadIdx.x / 64;
+enum : uint32_t { SRC_KEY = 0, SRC_SCAN = 1, SRC_HT = 2 };
+
+struct LeanOutCol {
+    ScanCol  col;   // SRC_SCAN over pages
+    uint32_t src;   // SRC_*
+    uint32_t word;  // SRC_SCAN over tuples: tuple word; SRC_HT: entry word
+    int32_t  type;  // output type (fixed width)
+    uint32_t pad;
+};
+
+struct LeanArgs {
+    ScanCol         key;        // probe key column (DRIVER_PAGES)
+    const uint64_t* tuples;     // DRIVER_TUPLES: rows x tuple_words, word 0 never NULL but in holes
+    uint32_t        tuple_words;
+    uint32_t        key_word;   // DRIVER_TUPLES: tuple word of the probe key
+    uint32_t        driver;
+    uint32_t        morsel_rows;
+    uint32_t        xcd_order;
+    uint32_t        unique;
+    uint64_t        nrows;
+    uint64_t        nmorsels;
+    const uint64_t* ht;
+    uint64_t        mask;
+    uint32_t        shift;
+    uint32_t        words;
+    const uint64_t* bloom;
+    uint32_t        bloom_log;
+    uint32_t        nout;
+    LeanOutCol      out[kLeanOut];
+    uint8_t*        pool;
+    uint64_t        pool_pages;
+    uint32_t*       dir;
+    PipeCounters*   counters;
+};
+
+// Open page of one output column (wave-uniform).
+struct LeanPage {
+    uint32_t page;   // pool index, kNoPage if none
+    uint32_t nr;     // rows in the page
+    uint32_t nv;     // non-NULL values in the page
+    int32_t  avail;  // remaining bit budget
+    uint32_t seq;    // pages emitted by this wave for this column
+    uint32_t npages; // pages written
+    uint32_t bad;    // pool exhausted: keep counting, write nothing
+    uint32_t pad;
+    uint64_t nnz;    // non-NULL values written
+};
+
+// The kernel's by-value argument struct, read from the kernarg segment (the
+// first explicit argument sits at offset 0). The empty asm hides where the
+// pointer comes from, so argument loads inside a loop are not hoisted and kept
+// live in scalar registers for the whole kernel (which runs out of scalar
+// registers and spills them into vector lanes); each iteration re-reads the
+// fields it needs with scalar loads that hit the scalar cache.
+template <class T>
+__device__ inline const T& kernel_args() {
+    typedef const __attribute__((address_space(4))) T* KPtr;
+    KPtr p = (KPtr)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const T*)p;
+}
+
+__device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint64_t uniform_u64(uint64_t v) {
+    return uint64_t(uniform_u32(uint32_t(v))) | (uint64_t(uniform_u32(uint32_t(v >> 32))) << 32);
+}
+__device__ inline uint64_t low_mask(uint32_t n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
+
+__device__ inline void lean_open(const LeanArgs& a, WaveState& ws, LeanPage& st, uint64_t& bits, int32_t budget) {
+    uint64_t next = ws.next;
+    if (next == ws.end) {
+        unsigned long long base = 0;
+        if (lane_id() == 0) base = atomicAdd(&a.counters->pages, (unsigned long long)kPageChunk);
+        next   = uniform_u64(base);
+        ws.end = next + kPageChunk;
+    }
+    ws.next  = next + 1;
+    st.bad   = next >= a.pool_pages ? 1u : 0u;
+    if (st.bad && lane_id() == 0) atomicExch(&a.counters->overflow, 1ull);
+    st.page  = st.bad ? 0u : uint32_t(next);
+    st.nr    = 0;
+    st.nv    = 0;
+    st.avail = budget;
+    bits     = 0;
+}
+
+// Same page layout as page_close: header, bitmap bytes at the page end, directory entry.
+__device__ inline void lean_close(const LeanArgs& a, LeanPage& st, uint64_t bits, uint32_t col, uint32_t stream) {
+    if (!st.bad) {
+        uint8_t* page = a.pool + uint64_t(st.page) * kPageSize;
+        uint32_t nb   = (st.nr + 7) / 8;
+        uint8_t* bm   = page + kPageSize - nb;
+        uint32_t b0   = lane_id() * 8;
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b)
+            if (b0 + b < nb) bm[b0 + b] = uint8_t(bits >> (8 * b));
+        if (lane_id() == 0) {
+            *reinterpret_cast<uint32_t*>(page) = st.nr | (st.nv << 16);
+            uint4 e = make_uint4(col, stream, st.seq, 1u);
+            *reinterpret_cast<uint4*>(a.dir + 4 * uint64_t(st.page)) = e;
+        }
+        st.npages++;
+        st.nnz += st.nv;
+    }
+    st.seq++;
+    st.page = kNoPage;
+}
+
+// Writes batch rows [lo, hi) (row order) to the open page. `vm` = lanes with a
+// non-NULL value, `pat` = validity of the batch rows in row order.
+__device__ inline void lean_segment(const LeanArgs& a, LeanPage& st, uint64_t& bits, uint32_t w, uint64_t v,
+                                    bool valid, uint32_t rank, uint64_t vm, uint64_t pat, uint32_t lo, uint32_t hi) {
+    const uint32_t n    = hi - lo;
+    const uint64_t sp   = (pat >> lo) & low_mask(n);
+    const uint32_t nvs  = __popcll(sp);
+    const uint32_t skip = __popcll(pat & low_mask(lo)); // valid rows before the segment
+    if (valid && rank >= lo && rank < hi && !st.bad) {
+        uint32_t vi  = st.nv + __popcll(vm & mask_lt()) - skip;
+        uint8_t* dst = a.pool + uint64_t(st.page) * kPageSize;
+        if (w == 4)
+            *reinterpret_cast<uint32_t*>(dst + 4 + 4 * vi) = uint32_t(v);
+        else
+            *reinterpret_cast<uint64_t*>(dst + 8 + 8 * uint64_t(vi)) = v;
+    }
+    const uint32_t wi = st.nr >> 6, sh = st.nr & 63, lane = lane_id();
+    if (lane == wi) bits |= sp << sh;
+    if (sh && lane == wi + 1) bits |= sp >> (64 - sh);
+    st.nr += n;
+    st.nv += nvs;
+    st.avail -= int32_t(n + 8 * w * nvs);
+}
+
+// Appends a batch of `cnt` rows to a fixed-width column following the
+// reference's bit budget (TableTarget.cpp:111-182): 1 bit per row plus the
+// value width per non-NULL row. A fresh page holds more than 64 rows of any
+// fixed-width type, so a batch spans at most two pages.
+__device__ inline void lean_append(const LeanArgs& a, WaveState& ws, LeanPage& st, uint64_t& bits, uint32_t w,
+                                   uint64_t v, bool valid, uint32_t rank, uint32_t cnt, uint64_t vm, uint64_t pat,
+                                   uint32_t col, uint32_t stream) {
+    const int32_t budget = 8 * (int32_t(kPageSize) - 4 - (w == 4 ? 0 : 4));
+    const int32_t vbits  = int32_t(8 * w);
+    if (st.page == kNoPage) lean_open(a, ws, st, bits, budget);
+    uint32_t k = cnt;
+    if (int32_t(cnt) + vbits * int32_t(__popcll(vm)) > st.avail || st.nr + cnt > kMaxPageRows) {
+        // the leading rows that still fit: lane r tests rows 0..r
+        uint32_t r    = lane_id();
+        int32_t  cost = int32_t(r + 1) + vbits * int32_t(__popcll(pat & mask_le()));
+        bool     fits = r < cnt && cost <= st.avail && st.nr + r + 1 <= kMaxPageRows;
+        k             = __popcll(__ballot(fits));
+    }
+    if (k) lean_segment(a, st, bits, w, v, valid, rank, vm, pat, 0, k);
+    if (k < cnt) {
+        lean_close(a, st, bits, col, stream);
+        lean_open(a, ws, st, bits, budget);
+        lean_segment(a, st, bits, w, v, valid, rank, vm, pat, k, cnt);
+    }
+    if (st.nr >= kMaxPageRows) lean_close(a, st, bits, col, stream);
+}
+
+__device__ inline void lean_finish(const LeanArgs& a, LeanPage& st, uint64_t bits, uint32_t c, uint32_t stream) {
+    if (st.page != kNoPage && st.nr > 0)
+        lean_close(a, st, bits, c, stream);
+    else if (st.page != kNoPage && !st.bad && lane_id() == 0)
+        a.dir[4 * uint64_t(st.page) + 3] = 0; // opened, never filled
+    if (lane_id() == 0) {
+        if (st.nnz) atomicAdd(&a.counters->nnz[c], (unsigned long long)st.nnz);
+        if (st.npages) atomicAdd(&a.counters->col_pages[c], (unsigned long long)st.npages);
+    }
+}
+
+// NO = output columns (0: count only).
+template <int NO>
+__global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
+    (void)karg; // read through kernel_args()
+    const LeanArgs& a     = kernel_args<LeanArgs>();
+    constexpr int  NS     = NO > 0 ? NO : 1;
+    const uint32_t wib    = wave_in_block();
     const uint32_t stream = blockIdx.x * kWavesPerBlock + wib;
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
     const uint32_t lane   = lane_id();
-    uint64_t*      obits  = lds + wib * (kLeanOut * 64);
     WaveState      ws{0, 0, 0, 0};
-    OutState       st[kLeanOut];
+    LeanPage       st[NS];
+    uint64_t       bits[NS];
 #pragma unroll
-    for (int c = 0; c < kLeanOut; ++c) st[c] = OutState{kNoPage, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const ProbeSpec& P     = a.probes[0];
-    uint64_t         local = 0;
+    for (int c = 0; c < NS; ++c) {
+        st[c]   = LeanPage{kNoPage, 0, 0, 0, 0, 0, 0, 0, 0};
+        bits[c] = 0;
+    }
+    uint64_t local = 0;
 
     uint64_t m_first = stream, m_end = a.nmorsels, m_step = nwaves;
     if (a.xcd_order) {
@@ -960,114 +1143,170 @@ __global__ __launch_bounds__(256) void k_probe1(PipeArgs a) {
         m_end        = a.nmorsels * (x + 1) / 8;
         m_step       = uint64_t(bpx) * kWavesPerBlock;
     }
-    // Blocks are processed kLeanUnroll at a time: all their index, value and
-    // first hash-table loads are issued before any of them is consumed, so a
-    // wave keeps several dependent-load chains in flight.
     for (uint64_t m = m_first; m < m_end; m += m_step) {
-        uint64_t r0  = m * a.morsel_rows;
-        uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
-        for (uint64_t rb = r0; rb < end; rb += 64 * kLeanUnroll) {
-            uint64_t s[kLeanUnroll][NA];
-            bool     alive[kLeanUnroll];
+        const uint64_t r0  = m * a.morsel_rows;
+        const uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
+        for (uint64_t rb = r0; rb < end; rb += 64) {
+            const LeanArgs& a      = kernel_args<LeanArgs>(); // re-read per block (scalar cache)
+            const uint64_t  row    = rb + lane;
+            const bool      active = row < end;
+            const uint64_t  blk    = rb / 64;
+            // probe key and the scanned output columns: all loads issued up front
+            uint64_t key  = kNull;
+            bool     hole = false;
+            uint64_t v[NS];
+            if (a.driver == DRIVER_PAGES) {
+                key = decode_block(a.key, blk, active);
 #pragma unroll
-            for (int u = 0; u < kLeanUnroll; ++u) {
-                uint64_t row    = rb + 64 * u + lane;
-                bool     active = row < end;
-#pragma unroll
-                for (int k = 0; k < NA; ++k) s[u][k] = kNull;
-                alive[u] = active;
-                if (a.driver == DRIVER_PAGES) {
-#pragma unroll
-                    for (int c = 0; c < kLeanScan; ++c)
-                        if (uint32_t(c) < a.nscan && c < NA) s[u][c] = decode_block(a.scan[c], rb / 64 + u, active);
-                } else if (active) {
-                    alive[u] = a.tuples[row * a.tuple_words] != kEmpty;
-#pragma unroll
-                    for (int c = 0; c < kLeanScan; ++c)
-                        if (uint32_t(c) < a.nscan && c < NA) s[u][c] = a.tuples[row * a.tuple_words + a.tuple_word[c]];
+                for (int c = 0; c < NO; ++c) v[c] = a.out[c].src == SRC_SCAN ? decode_block(a.out[c].col, blk, active) : kNull;
+            } else {
+                const uint64_t* t = a.tuples + row * a.tuple_words;
+                if (active) {
+                    hole = t[0] == kEmpty;
+                    key  = t[a.key_word];
                 }
 #pragma unroll
-                for (int k = 0; k < NA; ++k)
-                    if ((a.not_null >> k) & 1u) alive[u] = alive[u] && s[u][k] != kNull;
+                for (int c = 0; c < NO; ++c) v[c] = (a.out[c].src == SRC_SCAN && active) ? t[a.out[c].word] : kNull;
             }
-            uint64_t key[kLeanUnroll], p[kLeanUnroll], kw[kLeanUnroll];
-#pragma unroll
-            for (int u = 0; u < kLeanUnroll; ++u) {
-                key[u]     = sget(s[u], P.key_slot);
-                uint64_t h = hash_key(key[u]);
-                if (alive[u] && P.bloom) {
-                    uint64_t h2 = hash_bloom(h);
-                    uint64_t bb = bloom_bits(h2);
-                    alive[u]    = (P.bloom[bloom_word(h2, P.bloom_log)] & bb) == bb;
+            bool           alive = active && !hole && key != kNull;
+            const uint64_t h     = hash_key(key);
+            if (alive && a.bloom) {
+                uint64_t h2 = hash_bloom(h);
+                uint64_t bb = bloom_bits(h2);
+                alive       = (a.bloom[bloom_word(h2, a.bloom_log)] & bb) == bb;
+            }
+            uint64_t q    = h >> a.shift;
+            bool     live = alive;
+            for (;;) {
+                bool            have = false;
+                const uint64_t* he   = a.ht;
+                uint64_t        k1   = 0;
+                while (live) {
+                    // key and first attribute word are loaded together
+                    const uint64_t* e  = a.ht + q * a.words;
+                    const uint64_t  k0 = e[0];
+                    k1                 = a.words > 1 ? e[1] : 0;
+                    if (k0 == kEmpty) {
+                        live = false;
+                        break;
+                    }
+                    q = (q & ~a.mask) | ((q + 1) & a.mask);
+                    if (k0 == key) {
+                        he   = e;
+                        have = true;
+                        if (a.unique) live = false;
+                        break;
+                    }
                 }
-                p[u]  = h >> P.shift;
-                kw[u] = alive[u] ? P.ht[p[u] * P.words] : kEmpty;
-            }
+                const uint64_t hm = __ballot(have);
+                if (hm) {
+                    const uint32_t cnt = __popcll(hm);
+                    local += cnt;
+                    if (NO > 0) {
+                        const uint32_t rank = __popcll(hm & mask_lt());
+                        // forward permute: have-lanes to their rank, the others to cnt.. (a permutation)
+                        const uint32_t dst = have ? rank : cnt + __popcll(~hm & mask_lt());
+                        uint32_t       vbm = 0;
+                        uint64_t       o[NS];
 #pragma unroll
-            for (int u = 0; u < kLeanUnroll; ++u) {
-                bool     live = alive[u];
-                uint64_t q    = p[u];
-                uint64_t w0   = kw[u]; // key word of slot q, already loaded
-                bool     first = true;
-                for (;;) {
-                    bool have = false;
-                    while (live) {
-                        uint64_t k0 = first ? w0 : P.ht[q * P.words];
-                        first       = false;
-                        if (k0 == kEmpty) {
-                            live = false;
-                            break;
+                        for (int c = 0; c < NO; ++c) {
+                            const uint32_t src = a.out[c].src, w = a.out[c].word;
+                            o[c] = src == SRC_KEY ? key : src == SRC_SCAN ? v[c] : (!have ? kNull : w == 1 ? k1 : he[w]);
+                            if (have && o[c] != kNull) vbm |= 1u << c;
                         }
-                        uint64_t hit_slot = q;
-                        q = (q & ~P.mask) | ((q + 1) & P.mask);
-                        if (k0 == key[u]) {
-                            const uint64_t* e = P.ht + hit_slot * P.words;
-                            for (uint32_t t = 0; t < P.nattrs; ++t) sset(s[u], P.attr_slot[t], e[P.attr_word[t]]);
-                            have = true;
-                            if (P.unique) live = false;
-                            break;
+                        const uint32_t recv = uint32_t(__builtin_amdgcn_ds_permute(int(dst << 2), int(vbm)));
+#pragma unroll
+                        for (int c = 0; c < NO; ++c) {
+                            const bool     valid = (vbm >> c) & 1u;
+                            const uint64_t vm    = __ballot(valid);
+                            const uint64_t pat   = __ballot((recv >> c) & 1u);
+                            lean_append(a, ws, st[c], bits[c], a.out[c].type == T_INT32 ? 4u : 8u, o[c], valid, rank,
+                                        cnt, vm, pat, uint32_t(c), stream);
                         }
                     }
-                    uint64_t hm = __ballot(have);
-                    if (hm) {
-                        uint32_t cnt  = __popcll(hm);
-                        uint32_t rank = __popcll(hm & mask_lt());
-                        local += cnt;
-                        if (a.target == TARGET_PAGES) {
-#pragma unroll
-                            for (int c = 0; c < kLeanOut; ++c)
-                                if (uint32_t(c) < a.nout)
-                                    append_fixed(a, ws, st[c], obits + c * 64, a.out[c].type,
-                                                 sget(s[u], a.out[c].slot), have, rank, cnt, uint32_t(c), stream);
-                        }
-                    }
-                    if (!__ballot(live)) break;
                 }
+                if (!__ballot(live)) break;
             }
         }
     }
-    if (a.target == TARGET_PAGES) {
 #pragma unroll
-        for (int c = 0; c < kLeanOut; ++c)
-            if (uint32_t(c) < a.nout) finish_column(a, st[c], obits + c * 64, uint32_t(c), stream, nullptr);
-    }
-    finish_wave(a, ws, local);
+    for (int c = 0; c < NO; ++c) lean_finish(a, st[c], bits[c], uint32_t(c), stream);
+    if (lane == 0 && local) atomicAdd(&a.counters->tuples, (unsigned long long)local);
+    if (NO > 0)
+        for (uint64_t i = ws.next + lane; i < ws.end && i < a.pool_pages; i += 64) a.dir[4 * i + 3] = 0;
 }
 
+// The lean kernel applies when the pipeline has one probe without residual
+// equalities, every scanned join column is the probe key (so NULL filtering is
+// the key's), and the target is a count or at most four fixed-width columns.
 bool probe1_applicable(const PipeArgs& a, uint32_t slots) {
-    return slots <= 8 && a.nprobes == 1 && a.probes[0].nchecks == 0 && a.nscan <= uint32_t(kLeanScan) &&
-           (a.target == TARGET_COUNT || (a.target == TARGET_PAGES && a.nvarchar == 0 && a.nout <= uint32_t(kLeanOut)));
+    (void)slots;
+    if (a.nprobes != 1 || a.probes[0].nchecks != 0 || a.nscan > uint32_t(kMaxScan)) return false;
+    const ProbeSpec& P = a.probes[0];
+    if (P.key_slot >= a.nscan || (a.not_null & ~(1u << P.key_slot)) != 0) return false;
+    if (a.target == TARGET_COUNT) return true;
+    if (a.target != TARGET_PAGES || a.nvarchar != 0 || a.nout > uint32_t(kLeanOut)) return false;
+    for (uint32_t c = 0; c < a.nout; ++c) {
+        if (a.out[c].type == T_VARCHAR) return false;
+        const uint32_t slot = a.out[c].slot;
+        if (slot == P.key_slot || slot < a.nscan) continue;
+        bool found = false;
+        for (uint32_t t = 0; t < P.nattrs; ++t) found = found || P.attr_slot[t] == slot;
+        if (!found) return false;
+    }
+    return true;
 }
 
 hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s) {
     uint32_t blocks = (grid_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks == 0) blocks = 1;
     if (!probe1_applicable(a, slots)) return hipErrorInvalidValue;
-    size_t lds = size_t(kWavesPerBlock) * kLeanOut * 64 * sizeof(uint64_t);
-    if (slots <= 4)
-        k_probe1<4><<<dim3(blocks), dim3(256), lds, s>>>(a);
-    else
-        k_probe1<8><<<dim3(blocks), dim3(256), lds, s>>>(a);
+    const ProbeSpec& P = a.probes[0];
+    LeanArgs         l{};
+    l.key         = a.scan[P.key_slot];
+    l.tuples      = a.tuples;
+    l.tuple_words = a.tuple_words;
+    l.key_word    = a.tuple_word[P.key_slot];
+    l.driver      = a.driver;
+    l.morsel_rows = a.morsel_rows;
+    l.xcd_order   = a.xcd_order;
+    l.unique      = P.unique;
+    l.nrows       = a.nrows;
+    l.nmorsels    = a.nmorsels;
+    l.ht          = P.ht;
+    l.mask        = P.mask;
+    l.shift       = P.shift;
+    l.words       = P.words;
+    l.bloom       = P.bloom;
+    l.bloom_log   = P.bloom_log;
+    l.nout        = a.target == TARGET_PAGES ? a.nout : 0;
+    for (uint32_t c = 0; c < l.nout; ++c) {
+        LeanOutCol&    o    = l.out[c];
+        const uint32_t slot = a.out[c].slot;
+        o.type              = a.out[c].type;
+        if (slot == P.key_slot) {
+            o.src = SRC_KEY;
+        } else if (slot < a.nscan) {
+            o.src  = SRC_SCAN;
+            o.col  = a.scan[slot];
+            o.word = a.tuple_word[slot];
+        } else {
+            o.src = SRC_HT;
+            for (uint32_t t = 0; t < P.nattrs; ++t)
+                if (P.attr_slot[t] == slot) o.word = P.attr_word[t];
+        }
+    }
+    l.pool       = a.pool;
+    l.pool_pages = a.pool_pages;
+    l.dir        = a.dir;
+    l.counters   = a.counters;
+    switch (l.nout) {
+    case 0: k_lean<0><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
+    case 1: k_lean<1><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
+    case 2: k_lean<2><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
+    case 3: k_lean<3><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
+    default: k_lean<4><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
+    }
     return hipGetLastError();
 }
 

@@ -118,11 +118,12 @@ typedef struct {
 
 /* Execution options of a context (0 = success, -1 = unknown name):
  *   "radix_min_table_bytes" / "radix_min_rows": the probe side is radix-partitioned
- *       when the first probed hash table and the driving table exceed both;
+ *       when the first probed hash table and the driving table exceed both (default: never);
  *   "max_load": hash-table load factor (0 = automatic);
  *   "bloom_bits": bloom bits per key (-1 = automatic, 0 = none);
  *   "partitioned_build": radix-partitioned LDS build (-1 automatic, 0 off, 1 on);
- *   "lean_kernel": 1 (default) runs one-probe pipelines on the lean kernel, 0 on the general one. */
+ *   "lean_kernel": 1 (default) runs one-probe pipelines on the lean kernel, 0 on the general one;
+ *   "lean_waves_per_cu": grid of the lean kernel in waves per compute unit (4..64, default 32). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
@@ -151,6 +152,8 @@ void cgj_result_release(cgj_result* res);
 /* Device-resident tables: pages live in HBM (reference engine/query/PlanImport.cpp:91-101
  * imports host pages zero-copy; here the import is one upload per column). */
 cgj_dtable* cgj_dtable_upload(cgj_context* ctx, const cgj_host_table* table);
+/* Tables and device results keep their memory pool alive: they may be freed
+ * before or after their context is destroyed (`ctx` may then be NULL). */
 void        cgj_dtable_free(cgj_context* ctx, cgj_dtable* t);
 uint64_t    cgj_dtable_num_rows(const cgj_dtable* t);
 uint64_t    cgj_dtable_num_pages(const cgj_dtable* t, uint32_t column);

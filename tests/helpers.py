@@ -54,15 +54,28 @@ def _key(v):
     if isinstance(v, bytes):
         return (1, v)
     if isinstance(v, float):
-        return (1, struct.pack(">d", v) if v == v else b"nan")
+        return (1, struct.pack(">d", v))
+    if isinstance(v, tuple):  # NaN, by bit pattern
+        return (1, b"nan" + v[1])
     return (1, v)
 
 
+def _canon(v):
+    """Strings as bytes; a NaN (which never equals itself) as its bit pattern,
+    so rows holding the same NaN bits compare equal."""
+    if isinstance(v, str):
+        return v.encode()
+    if isinstance(v, float) and v != v:
+        return ("nan", struct.pack(">d", v))
+    return v
+
+
 def canon_rows(rows) -> List[tuple]:
-    """Rows with strings as bytes, sorted with NULLs first per column."""
+    """Rows with strings as bytes and NaNs as bit patterns, sorted with NULLs
+    first per column."""
     out = []
     for r in rows:
-        out.append(tuple(v.encode() if isinstance(v, str) else v for v in r))
+        out.append(tuple(_canon(v) for v in r))
     return sorted(out, key=lambda r: tuple(_key(v) for v in r))
 
 

@@ -200,6 +200,68 @@ def test_heavy_duplicate_build_falls_back(ctx):
     assert s["retries"] >= 1
 
 
+def _lean_shape_plan(seed):
+    """Two-table join whose output mixes the probe key, build and probe payloads
+    of every fixed-width type; duplicates, NULLs and misses as the seed says."""
+    rng = np.random.default_rng(700 + seed)
+    nb = int(rng.choice([1, 900, 20_000, 90_000]))
+    npr = int(rng.choice([64, 5_000, 70_000, 200_000]))
+    kt = [INT32, INT64][seed % 2]
+    dup = seed % 3 == 0
+    dom = max(nb // (4 if dup else 1), 1)
+    bkeys = rng.integers(0, dom, nb) if dup else rng.permutation(nb)
+    bkeys = bkeys.astype(np.int32 if kt == INT32 else np.int64) * (1 if kt == INT32 else 1_000_003)
+    pk = rng.integers(0, int(dom * 1.2) + 1, npr).astype(bkeys.dtype) * (1 if kt == INT32 else 1_000_003)
+    nullf = [0.0, 0.05, 0.6][seed % 3]
+
+    def col(t, n):
+        v = rng.random(n) * 1e6 if t == FP64 else rng.integers(-(1 << 30), 1 << 30, n)
+        return (t, v, rng.random(n) >= nullf)
+
+    bpay = [col(t, nb) for t in rng.choice([INT32, INT64, FP64], int(rng.integers(1, 4)))]
+    ppay = [col(t, npr) for t in rng.choice([INT32, INT64, FP64], int(rng.integers(1, 3)))]
+    build = [(kt, bkeys, rng.random(nb) >= nullf / 2)] + bpay
+    probe = [(kt, pk, rng.random(npr) >= nullf / 2)] + ppay
+    layout = [c[0] for c in build] + [c[0] for c in probe]
+    nbc = len(build)
+    choices = [nbc] + list(range(1, nbc)) + list(range(nbc + 1, len(layout)))  # probe key, payloads
+    k = int(rng.integers(1, 5))
+    picks = [int(x) for x in rng.choice(choices, size=k, replace=len(choices) < k)]
+    out = [(i, layout[i]) for i in picks]
+    return _join_plan(build, probe, 0, 0, out)
+
+
+@pytest.mark.parametrize("seed", range(18))
+def test_lean_kernel_shapes(ctx, seed):
+    """One-probe pipelines on the lean kernel: the oracle's rows, and the same
+    rows as the general pipeline kernel; the count target agrees."""
+    plan = _lean_shape_plan(seed)
+    say(f"lean shape seed={seed}")
+    types = [t for _, t in plan.nodes[plan.root].output_attrs]
+    lean = check_table(ctx.execute(plan), types)
+    exp = check_table(oracle.execute(plan), types)
+    assert lean == exp
+    ctx.set_option("lean_kernel", 0)
+    try:
+        general = check_table(ctx.execute(plan), types)
+    finally:
+        ctx.set_option("lean_kernel", 1)
+    assert general == exp
+    tables = [ctx.upload(t) for t in plan.inputs]
+    assert ctx.count_device(plan, tables) == len(exp)
+
+
+def test_lean_kernel_all_null_int32_rows_cap(ctx):
+    """An all-NULL INT32 output column costs one bit per row: pages close at the
+    4096-row bitmap capacity instead of the bit budget."""
+    n = 50_000
+    keys = np.arange(n, dtype=np.int32)
+    plan = _join_plan([(INT32, keys, None), (INT32, keys, np.zeros(n, bool))],
+                      [(INT32, keys[::-1].copy(), None)], 0, 0, [(1, INT32), (2, INT32)])
+    say("lean all-NULL INT32")
+    _compare_large(ctx, plan)
+
+
 def _workload_plan(wl):
     plan = wl.make_plan()
     for t in wl.tables:
@@ -213,7 +275,7 @@ def radix_ctx(ctx):
     ctx.set_option("radix_min_table_bytes", 0)
     ctx.set_option("radix_min_rows", 0)
     yield ctx
-    ctx.set_option("radix_min_table_bytes", 64 << 20)
+    ctx.set_option("radix_min_table_bytes", float(2**64 - 1))
     ctx.set_option("radix_min_rows", 4 << 20)
 
 
@@ -263,3 +325,16 @@ def test_c3_recipe_scaled_vs_oracle(ctx):
     exp = check_table(oracle.execute(plan), types)
     assert got == exp
     assert s["pipelines"] == 5, s  # four dimension builds + one fused probe chain
+
+
+def test_tables_outlive_their_context():
+    """Device tables and results freed after their context is destroyed."""
+    ctx2 = gj.Context(0)
+    keys = np.arange(1000, dtype=np.int32)
+    t = ctx2.upload(gj.make_table([(INT32, keys, None)], 1000))
+    plan = _join_plan([(INT32, keys, None)], [(INT32, keys, None)], 0, 0, [(0, INT32)])
+    r = ctx2.execute_device(plan, [ctx2.upload(tb) for tb in plan.inputs])
+    assert r.num_rows == 1000
+    ctx2.close()
+    r.free()
+    t.free()
