@@ -81,6 +81,18 @@ __device__ inline uint32_t wave_sum(uint32_t v) {
 }
 __device__ inline uint32_t highest_lane(uint64_t m) { return 63 - __clzll(m); }
 
+// Load of data that no kernel writes while the reading kernel runs (column
+// indexes, hash tables built by an earlier launch). Through the constant
+// address space a wave-uniform address becomes a scalar load; otherwise the
+// compiler has to assume the kernel's own stores may alias it and issues a
+// vector load plus readfirstlane. The cast goes through an integer: a direct
+// pointer cast is folded back into a global-memory load. Adjacent dword loads
+// merge into one s_load_dwordx4.
+__device__ inline uint4 load_invariant_u4(const void* p) {
+    const __attribute__((address_space(4))) uint32_t* q = (const __attribute__((address_space(4))) uint32_t*)(uintptr_t)p;
+    return make_uint4(q[0], q[1], q[2], q[3]);
+}
+
 // ---- page reader ----
 
 // Decode index of one 64-row block of a column (rows 64b .. 64b+63): the page
@@ -138,8 +150,21 @@ __device__ inline uint64_t decode_row_slow(const ScanCol& c, uint32_t p, uint32_
 // Replaces the reference's skipTo/step64 (TableScan.cpp:135-324): validity and
 // the value index come from the block's 64-bit mask by popcount, so a step
 // costs one uniform index load plus one coalesced value load.
+// The entry is read as four dwords (a scalar load when `block` is
+// wave-uniform; its u16/u8 fields alone would force byte-wise vector loads).
+__device__ inline BlockIdx load_block_idx(const BlockIdx* blocks, uint64_t block) {
+    const uint4 w = load_invariant_u4(blocks + block);
+    BlockIdx    bi;
+    bi.page    = w.x;
+    bi.nn      = uint16_t(w.y);
+    bi.split   = uint8_t(w.y >> 16);
+    bi.complex = uint8_t(w.y >> 24);
+    bi.valid   = uint64_t(w.z) | (uint64_t(w.w) << 32);
+    return bi;
+}
+
 __device__ inline uint64_t decode_block(const ScanCol& c, uint64_t block, bool active) {
-    const BlockIdx bi   = c.blocks[block];
+    const BlockIdx bi   = load_block_idx(c.blocks, block);
     const uint32_t lane = lane_id();
     if (bi.complex) return active ? decode_row_slow(c, bi.page, uint32_t(block * 64 + lane)) : kNull;
     bool second = lane >= bi.split;

@@ -1115,6 +1115,21 @@ __device__ inline void lean_finish(const LeanArgs& a, LeanPage& st, uint64_t bit
     }
 }
 
+// Key and first attribute word of hash-table entry `slot`; two-word entries
+// (16-byte aligned) take one 16-byte load. Every slot of a region exists, so
+// inspecting a neighbour slot never reads out of bounds.
+__device__ inline void ht_entry2(const uint64_t* ht, uint32_t words, uint64_t slot, uint64_t& k, uint64_t& w1) {
+    const uint64_t* e = ht + slot * words;
+    if (words == 2) {
+        const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(e);
+        k                  = v.x;
+        w1                 = v.y;
+    } else {
+        k  = e[0];
+        w1 = words > 1 ? e[1] : 0;
+    }
+}
+
 // NO = output columns (0: count only).
 template <int NO>
 __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
@@ -1135,6 +1150,10 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
     }
     uint64_t local = 0;
 
+    // Morsel schedule: each wave starts on morsel `stream`, then takes the next
+    // unclaimed one from a device counter (one atomic per 4096-row morsel),
+    // which balances skewed morsels (hot keys, misses) across the waves. The
+    // XCD-ordered walk over a partitioned probe side stays static.
     uint64_t m_first = stream, m_end = a.nmorsels, m_step = nwaves;
     if (a.xcd_order) {
         uint32_t x   = blockIdx.x % 8;
@@ -1143,7 +1162,13 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
         m_end        = a.nmorsels * (x + 1) / 8;
         m_step       = uint64_t(bpx) * kWavesPerBlock;
     }
-    for (uint64_t m = m_first; m < m_end; m += m_step) {
+    for (uint64_t m = m_first; m < m_end;) {
+        uint64_t m_next = m + m_step;
+        if (!a.xcd_order) {
+            unsigned long long t = 0;
+            if (lane == 0) t = atomicAdd(&a.counters->morsel, 1ull);
+            m_next = nwaves + uniform_u64(t);
+        }
         const uint64_t r0  = m * a.morsel_rows;
         const uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
         for (uint64_t rb = r0; rb < end; rb += 64) {
@@ -1175,30 +1200,48 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
                 uint64_t bb = bloom_bits(h2);
                 alive       = (a.bloom[bloom_word(h2, a.bloom_log)] & bb) == bb;
             }
-            uint64_t q    = h >> a.shift;
-            bool     live = alive;
+            const uint64_t* ht    = a.ht;
+            const uint32_t  words = a.words;
+            const uint64_t  rmask = a.mask;
+            uint64_t        q     = h >> a.shift; // next slot to inspect
+            bool            live  = alive;
             for (;;) {
-                bool            have = false;
-                const uint64_t* he   = a.ht;
-                uint64_t        k1   = 0;
-                while (live) {
-                    // key and first attribute word are loaded together
-                    const uint64_t* e  = a.ht + q * a.words;
-                    const uint64_t  k0 = e[0];
-                    k1                 = a.words > 1 ? e[1] : 0;
-                    if (k0 == kEmpty) {
-                        live = false;
-                        break;
-                    }
-                    q = (q & ~a.mask) | ((q + 1) & a.mask);
-                    if (k0 == key) {
-                        he   = e;
-                        have = true;
-                        if (a.unique) live = false;
-                        break;
+                // Each step inspects two consecutive slots of every searching
+                // lane; a lane stops at a match or an empty slot. The wave steps
+                // as long as its longest probe sequence, so two slots per step
+                // halve the dependent hash-table round trips.
+                bool     have = false, searching = live;
+                uint64_t hslot = 0, k1 = 0;
+                while (__ballot(searching)) {
+                    if (searching) {
+                        const uint64_t q1 = (q & ~rmask) | ((q + 1) & rmask);
+                        const uint64_t q2 = (q1 & ~rmask) | ((q1 + 1) & rmask);
+                        uint64_t       ka, wa, kb, wb;
+                        ht_entry2(ht, words, q, ka, wa);
+                        ht_entry2(ht, words, q1, kb, wb);
+                        if (ka == key) {
+                            have  = true;
+                            hslot = q;
+                            k1    = wa;
+                            q     = q1;
+                        } else if (ka == kEmpty) {
+                            live = false;
+                        } else if (kb == key) {
+                            have  = true;
+                            hslot = q1;
+                            k1    = wb;
+                            q     = q2;
+                        } else if (kb == kEmpty) {
+                            live = false;
+                        } else {
+                            q = q2;
+                        }
+                        searching = live && !have;
                     }
                 }
-                const uint64_t hm = __ballot(have);
+                if (a.unique) live = false; // a lane without a match has already stopped
+                const uint64_t* he = ht + hslot * words;
+                const uint64_t  hm = __ballot(have);
                 if (hm) {
                     const uint32_t cnt = __popcll(hm);
                     local += cnt;
@@ -1228,6 +1271,7 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
                 if (!__ballot(live)) break;
             }
         }
+        m = m_next;
     }
 #pragma unroll
     for (int c = 0; c < NO; ++c) lean_finish(a, st[c], bits[c], uint32_t(c), stream);
@@ -1300,7 +1344,24 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
     l.pool_pages = a.pool_pages;
     l.dir        = a.dir;
     l.counters   = a.counters;
-    switch (l.nout) {
+    // With the dynamic morsel schedule a grid larger than one resident wave
+    // set only adds a tail: cap it at what the kernel's occupancy fits.
+    static const void* const kernels[kLeanOut + 1] = {
+        reinterpret_cast<const void*>(&k_lean<0>), reinterpret_cast<const void*>(&k_lean<1>),
+        reinterpret_cast<const void*>(&k_lean<2>), reinterpret_cast<const void*>(&k_lean<3>),
+        reinterpret_cast<const void*>(&k_lean<4>)};
+    const uint32_t no = l.nout < uint32_t(kLeanOut) ? l.nout : uint32_t(kLeanOut);
+    if (!a.xcd_order) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[no], 256, 0) == hipSuccess && per_cu > 0 &&
+            cus > 0) {
+            const uint32_t resident = uint32_t(per_cu) * uint32_t(cus);
+            if (blocks > resident) blocks = resident;
+        }
+    }
+    switch (no) {
     case 0: k_lean<0><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
     case 1: k_lean<1><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
     case 2: k_lean<2><<<dim3(blocks), dim3(256), 0, s>>>(l); break;

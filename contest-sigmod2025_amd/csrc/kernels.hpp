@@ -46,7 +46,7 @@ enum : uint32_t { DRIVER_PAGES = 0, DRIVER_TUPLES = 1 };
 struct PipeCounters {
     unsigned long long tuples;    // staged tuples / output rows / counted tuples
     unsigned long long pages;     // pool pages (or staging slots) reserved
-    unsigned long long dir;       // unused
+    unsigned long long morsel;    // morsels handed out beyond each wave's first (dynamic schedule)
     unsigned long long overflow;  // nonzero: a buffer was too small, re-run
     unsigned long long nnz[kMaxOut];       // non-NULL values written per output column
     unsigned long long col_pages[kMaxOut]; // pages written per output column
