@@ -85,8 +85,10 @@ Context::Context(int device) : device_(device) {
     CGJ_HIP(hipSetDevice(device));
     CGJ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-        cus_ = uint32_t(prop.multiProcessorCount);
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        if (prop.multiProcessorCount > 0) cus_ = uint32_t(prop.multiProcessorCount);
+        if (prop.sharedMemPerBlock > 0) lds_ = uint64_t(prop.sharedMemPerBlock);
+    }
 }
 
 Context::~Context() {
@@ -247,6 +249,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "bloom_bits") options.bloom_bits = int32_t(v);
     else if (name == "partitioned_build") options.partitioned_build = int32_t(v);
     else if (name == "lean_kernel") options.lean_kernel = v != 0;
+    else if (name == "build_regions_log") options.build_regions_log = v >= 1 && v <= 14 ? uint32_t(v) : 14;
     else if (name == "lean_waves_per_cu") options.lean_waves_per_cu = v >= 4 && v <= 64 ? uint32_t(v) : 32;
     else return false;
     return true;
@@ -324,7 +327,7 @@ public:
     std::unique_ptr<DevResult> run(bool count_only = false);
 
     double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0, ms_partition = 0;
-    uint32_t launches = 0, retries = 0;
+    uint32_t launches = 0, retries = 0, probe_lean = 0, probe_waves = 0;
     uint64_t build_tuples = 0, probe_rows = 0, build_bytes = 0, out_ref_pages = 0, probe_input_pages = 0;
 
 private:
@@ -511,10 +514,16 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
     CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
     if (bloom) CGJ_HIP(launch_fill_u64(bloom, 1ull << ht->bloom_log, 0, ctx_.stream()));
 
-    // Region size: the largest power of two whose entries fit a 64 KiB LDS table.
+    // Region size: the largest power of two whose entries fit a 64 KiB LDS
+    // table, raised when the table would need more than 16K regions (the
+    // partition histogram keeps one u32 per region in 64 KiB of LDS), as long
+    // as the region still fits the workgroup's LDS (160 KiB on gfx950).
     uint32_t rs_log = 12;
     while (rs_log > 4 && (uint64_t(1) << rs_log) * ht->words * 8 > 65536) --rs_log;
-    bool partitioned = tn.partitioned && n >= 65536 && lg > rs_log && lg - rs_log <= 14;
+    const uint32_t max_rlog = ctx_.options.build_regions_log;
+    if (lg > rs_log + max_rlog) rs_log = lg - max_rlog;
+    bool partitioned = tn.partitioned && n >= 65536 && lg > rs_log &&
+                       (uint64_t(1) << rs_log) * ht->words * 8 <= ctx_.lds_bytes();
     bool done        = false;
     if (partitioned) {
         uint32_t log_regions = lg - rs_log;
@@ -784,6 +793,8 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             res->pool_pages     = pages;
             res->reserved_pages = std::min<uint64_t>(c.pages, pages);
             ms_probe_kernel     = last_ms_;
+            probe_lean          = use_lean(a, nslots) ? 1 : 0;
+            probe_waves         = uint32_t(waves_for(a, nslots, nrows));
             // output bytes at the reference's packing (TableTarget.cpp:111-182)
             for (uint32_t i = 0; i < a.nout; ++i) {
                 int32_t ty = a.out[i].type;
@@ -891,6 +902,8 @@ std::unique_ptr<DevResult> Context::run_query(const cgj_plan& plan, const std::v
     stats.ms_total          = ex.ms_build + ex.ms_probe;
     stats.ms_probe_kernel   = ex.ms_probe_kernel;
     stats.ms_probe_partition = ex.ms_partition;
+    stats.probe_kernel      = ex.probe_lean;
+    stats.probe_waves       = ex.probe_waves;
     stats.build_tuples      = ex.build_tuples;
     stats.build_bytes       = ex.build_bytes;
     stats.probe_rows        = ex.probe_rows;

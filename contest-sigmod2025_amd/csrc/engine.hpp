@@ -132,6 +132,7 @@ struct Options {
     int32_t  partitioned_build     = -1;                 // radix-partitioned build (-1 = automatic)
     bool     lean_kernel           = true;               // use the one-probe kernel where it applies
     uint32_t lean_waves_per_cu     = 32;                 // lean kernel grid: waves per CU (8 per SIMD)
+    uint32_t build_regions_log     = 14;                 // partitioned build: at most 2^this regions
 };
 
 class Context {
@@ -162,6 +163,7 @@ public:
 
     JoinTuning tune(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes) const;
     uint32_t   cu_count() const { return cus_; }
+    uint64_t   lds_bytes() const { return lds_; } // LDS per workgroup
 
 private:
     std::unique_ptr<DevResult> run_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs,
@@ -169,6 +171,7 @@ private:
 
     int         device_;
     uint32_t    cus_    = 256;
+    uint64_t    lds_    = 65536;
     hipStream_t stream_ = nullptr;
     std::shared_ptr<DeviceHeap> heap_ = std::make_shared<DeviceHeap>();
 };

@@ -374,6 +374,11 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
     if (e != hipSuccess) return e;
     k_part_scatter<<<dim3(blocks), dim3(256), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh, parts);
     size_t tab_lds = size_t(rslots) * words * sizeof(uint64_t);
+    if (tab_lds > 65536) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part_build),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(tab_lds));
+        if (e != hipSuccess) return e;
+    }
     k_part_build<<<dim3(nregions), dim3(256), tab_lds, s>>>(parts, bh, blocks, n_real, nregions, words, rslots,
                                                             64 - log_cap, ht, bloom, bloom_log, flags);
     return hipGetLastError();

@@ -89,6 +89,8 @@ class Stats(C.Structure):
         ("build_bytes", C.c_uint64),
         ("ms_probe_kernel", C.c_double),
         ("ms_probe_partition", C.c_double),
+        ("probe_kernel", C.c_uint32),
+        ("probe_waves", C.c_uint32),
     ]
 
     def as_dict(self):

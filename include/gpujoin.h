@@ -114,6 +114,8 @@ typedef struct {
     uint64_t build_bytes;       /* compact hash-table tuples: tuples x words x 8 */
     double   ms_probe_kernel;   /* HIP-event time of the output pipeline kernel */
     double   ms_probe_partition;/* HIP-event time of the probe side's radix partitioning */
+    uint32_t probe_kernel;      /* output pipeline ran on: 0 the general kernel, 1 the lean one-probe kernel */
+    uint32_t probe_waves;       /* waves requested for the output pipeline */
 } cgj_stats;
 
 /* Execution options of a context (0 = success, -1 = unknown name):

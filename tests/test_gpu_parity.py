@@ -170,6 +170,24 @@ def test_partitioned_build_unique_int64(ctx):
     assert s["retries"] == 0, s
 
 
+def test_partitioned_build_large_regions(ctx):
+    """Few regions of 8192 slots (a 128 KiB LDS table per workgroup), as the
+    largest tables use; same rows as the oracle."""
+    rng = np.random.default_rng(21)
+    nb, npr = 300_000, 900_000
+    keys = rng.permutation(np.arange(nb, dtype=np.int64) * 31 + 7)
+    pk = np.where(rng.random(npr) < 0.8, keys[rng.integers(0, nb, npr)], -keys[rng.integers(0, nb, npr)] - 3)
+    plan = _join_plan([(INT64, keys, rng.random(nb) > 0.05), (INT32, rng.integers(0, 1 << 30, nb), None)],
+                      [(INT64, pk, rng.random(npr) > 0.05)], 0, 0, [(1, INT32), (2, INT64)])
+    ctx.set_option("build_regions_log", 7)
+    try:
+        say("partitioned build with 2^7 regions")
+        s = _compare_large(ctx, plan)
+    finally:
+        ctx.set_option("build_regions_log", 14)
+    assert s["retries"] == 0, s
+
+
 def test_partitioned_build_wide_entries(ctx):
     """Eight-word entries shrink the LDS region; results stay exact."""
     rng = np.random.default_rng(12)
