@@ -983,6 +983,8 @@ struct LeanArgs {
     uint64_t        pool_pages;
     uint32_t*       dir;
     PipeCounters*   counters;
+    uint32_t        page_chunk; // pool pages reserved per atomic (1..kPageChunk)
+    uint32_t        pad;
 };
 
 // Open page of one output column (wave-uniform).
@@ -1011,6 +1013,16 @@ __device__ inline const T& kernel_args() {
     asm volatile("" : "+s"(p));
     return *(const T*)p;
 }
+// Same for a read-only descriptor at device address `addr` (written before the
+// launch): scalar loads through the constant address space, re-read per use.
+template <class T>
+__device__ inline const T& args_at(uintptr_t addr) {
+    typedef const __attribute__((address_space(4))) T* KPtr;
+    KPtr p = (KPtr)addr;
+    asm volatile("" : "+s"(p));
+    return *(const T*)p;
+}
+__device__ inline uintptr_t kernarg_addr() { return (uintptr_t)__builtin_amdgcn_kernarg_segment_ptr(); }
 
 __device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ inline uint64_t uniform_u64(uint64_t v) {
@@ -1021,15 +1033,19 @@ __device__ inline uint64_t low_mask(uint32_t n) { return n >= 64 ? ~0ull : ((1ul
 __device__ inline void lean_open(const LeanArgs& a, WaveState& ws, LeanPage& st, uint64_t& bits, int32_t budget) {
     uint64_t next = ws.next;
     if (next == ws.end) {
-        unsigned long long base = 0;
-        if (lane_id() == 0) base = atomicAdd(&a.counters->pages, (unsigned long long)kPageChunk);
+        const uint32_t     chunk = a.page_chunk;
+        unsigned long long base  = 0;
+        if (lane_id() == 0) base = atomicAdd(&a.counters->pages, (unsigned long long)chunk);
         next   = uniform_u64(base);
-        ws.end = next + kPageChunk;
+        ws.end = next + chunk;
     }
     ws.next  = next + 1;
-    st.bad   = next >= a.pool_pages ? 1u : 0u;
+    // readfirstlane keeps these scalar: the compiler otherwise folds the
+    // uniform test into the lane-0 branch below and the page index comes out
+    // of a divergent join
+    st.bad   = uniform_u32(next >= a.pool_pages ? 1u : 0u);
+    st.page  = uniform_u32(st.bad ? 0u : uint32_t(next));
     if (st.bad && lane_id() == 0) atomicExch(&a.counters->overflow, 1ull);
-    st.page  = st.bad ? 0u : uint32_t(next);
     st.nr    = 0;
     st.nv    = 0;
     st.avail = budget;
@@ -1094,11 +1110,13 @@ __device__ inline void lean_append(const LeanArgs& a, WaveState& ws, LeanPage& s
     if (st.page == kNoPage) lean_open(a, ws, st, bits, budget);
     uint32_t k = cnt;
     if (int32_t(cnt) + vbits * int32_t(__popcll(vm)) > st.avail || st.nr + cnt > kMaxPageRows) {
-        // the leading rows that still fit: lane r tests rows 0..r
+        // the leading rows that still fit: lane r tests rows 0..r (bitwise &,
+        // not &&: a short-circuit branch here makes the compiler treat the
+        // page state after it as divergent and move it to vector registers)
         uint32_t r    = lane_id();
         int32_t  cost = int32_t(r + 1) + vbits * int32_t(__popcll(pat & mask_le()));
-        bool     fits = r < cnt && cost <= st.avail && st.nr + r + 1 <= kMaxPageRows;
-        k             = __popcll(__ballot(fits));
+        bool     fits = (r < cnt) & (cost <= st.avail) & (st.nr + r + 1 <= kMaxPageRows);
+        k             = uniform_u32(__popcll(__ballot(fits)));
     }
     if (k) lean_segment(a, st, bits, w, v, valid, rank, vm, pat, 0, k);
     if (k < cnt) {
@@ -1135,49 +1153,37 @@ __device__ inline void ht_entry2(const uint64_t* ht, uint32_t words, uint64_t sl
     }
 }
 
-// NO = output columns (0: count only).
-template <int NO>
-__global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
-    (void)karg; // read through kernel_args()
-    const LeanArgs& a     = kernel_args<LeanArgs>();
-    constexpr int  NS     = NO > 0 ? NO : 1;
-    const uint32_t wib    = wave_in_block();
-    const uint32_t stream = blockIdx.x * kWavesPerBlock + wib;
-    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-    const uint32_t lane   = lane_id();
-    WaveState      ws{0, 0, 0, 0};
-    LeanPage       st[NS];
-    uint64_t       bits[NS];
+// Per-wave state of the lean pipeline: page reservations, one open page and
+// its bitmap register per output column, and the row count.
+template <int NS>
+struct LeanWave {
+    WaveState ws;
+    LeanPage  st[NS];
+    uint64_t  bits[NS];
+    uint64_t  local;
+};
+
+template <int NS>
+__device__ inline void lean_wave_init(LeanWave<NS>& w) {
+    w.ws = WaveState{0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < NS; ++c) {
-        st[c]   = LeanPage{kNoPage, 0, 0, 0, 0, 0, 0, 0, 0};
-        bits[c] = 0;
+        w.st[c]   = LeanPage{kNoPage, 0, 0, 0, 0, 0, 0, 0, 0};
+        w.bits[c] = 0;
     }
-    uint64_t local = 0;
+    w.local = 0;
+}
 
-    // Morsel schedule: each wave starts on morsel `stream`, then takes the next
-    // unclaimed one from a device counter (one atomic per 4096-row morsel),
-    // which balances skewed morsels (hot keys, misses) across the waves. The
-    // XCD-ordered walk over a partitioned probe side stays static.
-    uint64_t m_first = stream, m_end = a.nmorsels, m_step = nwaves;
-    if (a.xcd_order) {
-        uint32_t x   = blockIdx.x % 8;
-        uint32_t bpx = gridDim.x / 8;
-        m_first      = a.nmorsels * x / 8 + uint64_t(blockIdx.x / 8) * kWavesPerBlock + wib;
-        m_end        = a.nmorsels * (x + 1) / 8;
-        m_step       = uint64_t(bpx) * kWavesPerBlock;
-    }
-    for (uint64_t m = m_first; m < m_end;) {
-        uint64_t m_next = m + m_step;
-        if (!a.xcd_order) {
-            unsigned long long t = 0;
-            if (lane == 0) t = atomicAdd(&a.counters->morsel, 1ull);
-            m_next = nwaves + uniform_u64(t);
-        }
-        const uint64_t r0  = m * a.morsel_rows;
-        const uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
+// Runs the one-probe pipeline over rows [r0, end) (multiples of 64 except at
+// the end) of the descriptor at `args_addr` (re-read per 64-row block).
+// `unique` >= 0 overrides the descriptor's duplicate-free flag.
+template <int NO, int NS>
+__device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end, uint32_t stream, LeanWave<NS>& W,
+                                 int32_t unique) {
+    const uint32_t lane = lane_id();
+    {
         for (uint64_t rb = r0; rb < end; rb += 64) {
-            const LeanArgs& a      = kernel_args<LeanArgs>(); // re-read per block (scalar cache)
+            const LeanArgs& a      = args_at<LeanArgs>(args_addr); // re-read per block (scalar cache)
             const uint64_t  row    = rb + lane;
             const bool      active = row < end;
             const uint64_t  blk    = rb / 64;
@@ -1244,12 +1250,12 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
                         searching = live && !have;
                     }
                 }
-                if (a.unique) live = false; // a lane without a match has already stopped
+                if (unique > 0 || (unique < 0 && a.unique)) live = false; // a lane without a match has stopped
                 const uint64_t* he = ht + hslot * words;
                 const uint64_t  hm = __ballot(have);
                 if (hm) {
                     const uint32_t cnt = __popcll(hm);
-                    local += cnt;
+                    W.local += cnt;
                     if (NO > 0) {
                         const uint32_t rank = __popcll(hm & mask_lt());
                         // forward permute: have-lanes to their rank, the others to cnt.. (a permutation)
@@ -1268,21 +1274,68 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
                             const bool     valid = (vbm >> c) & 1u;
                             const uint64_t vm    = __ballot(valid);
                             const uint64_t pat   = __ballot((recv >> c) & 1u);
-                            lean_append(a, ws, st[c], bits[c], a.out[c].type == T_INT32 ? 4u : 8u, o[c], valid, rank,
-                                        cnt, vm, pat, uint32_t(c), stream);
+                            lean_append(a, W.ws, W.st[c], W.bits[c], a.out[c].type == T_INT32 ? 4u : 8u, o[c], valid,
+                                        rank, cnt, vm, pat, uint32_t(c), stream);
                         }
                     }
                 }
                 if (!__ballot(live)) break;
             }
         }
+    }
+}
+
+// Closes the wave's open pages, adds its counts to the counters and retires
+// the unused part of its page reservation; the wave state starts over.
+template <int NO, int NS>
+__device__ inline void lean_flush(const LeanArgs& a, LeanWave<NS>& W, uint32_t stream) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int c = 0; c < NO; ++c) lean_finish(a, W.st[c], W.bits[c], uint32_t(c), stream);
+    if (lane == 0 && W.local) atomicAdd(&a.counters->tuples, (unsigned long long)W.local);
+    if (NO > 0)
+        for (uint64_t i = W.ws.next + lane; i < W.ws.end && i < a.pool_pages; i += 64) a.dir[4 * i + 3] = 0;
+    lean_wave_init(W);
+}
+
+// NO = output columns (0: count only).
+template <int NO>
+__global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
+    (void)karg; // read through kernel_args()
+    const LeanArgs& a     = kernel_args<LeanArgs>();
+    constexpr int  NS     = NO > 0 ? NO : 1;
+    const uint32_t wib    = wave_in_block();
+    const uint32_t stream = blockIdx.x * kWavesPerBlock + wib;
+    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+    const uint32_t lane   = lane_id();
+    LeanWave<NS>   W;
+    lean_wave_init(W);
+
+    // Morsel schedule: each wave starts on morsel `stream`, then takes the next
+    // unclaimed one from a device counter (one atomic per 4096-row morsel),
+    // which balances skewed morsels (hot keys, misses) across the waves. The
+    // XCD-ordered walk over a partitioned probe side stays static.
+    uint64_t m_first = stream, m_end = a.nmorsels, m_step = nwaves;
+    if (a.xcd_order) {
+        uint32_t x   = blockIdx.x % 8;
+        uint32_t bpx = gridDim.x / 8;
+        m_first      = a.nmorsels * x / 8 + uint64_t(blockIdx.x / 8) * kWavesPerBlock + wib;
+        m_end        = a.nmorsels * (x + 1) / 8;
+        m_step       = uint64_t(bpx) * kWavesPerBlock;
+    }
+    for (uint64_t m = m_first; m < m_end;) {
+        uint64_t m_next = m + m_step;
+        if (!a.xcd_order) {
+            unsigned long long t = 0;
+            if (lane == 0) t = atomicAdd(&a.counters->morsel, 1ull);
+            m_next = nwaves + uniform_u64(t);
+        }
+        const uint64_t r0  = m * a.morsel_rows;
+        const uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
+        lean_rows<NO>(kernarg_addr(), r0, end, stream, W, -1);
         m = m_next;
     }
-#pragma unroll
-    for (int c = 0; c < NO; ++c) lean_finish(a, st[c], bits[c], uint32_t(c), stream);
-    if (lane == 0 && local) atomicAdd(&a.counters->tuples, (unsigned long long)local);
-    if (NO > 0)
-        for (uint64_t i = ws.next + lane; i < ws.end && i < a.pool_pages; i += 64) a.dir[4 * i + 3] = 0;
+    lean_flush<NO>(a, W, stream);
 }
 
 // The lean kernel applies when the pipeline has one probe without residual
@@ -1349,6 +1402,7 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
     l.pool_pages = a.pool_pages;
     l.dir        = a.dir;
     l.counters   = a.counters;
+    l.page_chunk = kPageChunk;
     // With the dynamic morsel schedule a grid larger than one resident wave
     // set only adds a tail: cap it at what the kernel's occupancy fits.
     static const void* const kernels[kLeanOut + 1] = {
