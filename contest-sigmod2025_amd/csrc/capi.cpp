@@ -7,6 +7,7 @@
 
 #include "../../include/contest/plan.h"
 #include "../../include/gpujoin.h"
+#include "batch.hpp"
 #include "engine.hpp"
 #include "gen.hpp"
 
@@ -19,6 +20,9 @@ struct cgj_dtable {
 };
 struct cgj_dresult {
     std::unique_ptr<cgj::DevResult> impl;
+};
+struct cgj_batch {
+    std::unique_ptr<cgj::Batch> impl;
 };
 
 namespace {
@@ -133,6 +137,55 @@ int cgj_execute_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const
         *out = r;
     });
 }
+
+cgj_batch* cgj_batch_create(cgj_context* ctx, const cgj_plan* plans, uint32_t num_plans, cgj_dtable* const* inputs,
+                            const uint32_t* num_inputs) {
+    cgj_batch* b = nullptr;
+    guarded(ctx, [&] {
+        std::vector<cgj_plan>                           ps(plans, plans + num_plans);
+        std::vector<std::vector<const cgj::DevTable*>> ins(num_plans);
+        uint64_t                                        k = 0;
+        for (uint32_t i = 0; i < num_plans; ++i)
+            for (uint32_t t = 0; t < num_inputs[i]; ++t) ins[i].push_back(inputs[k++]->impl.get());
+        auto nb  = std::make_unique<cgj_batch>();
+        nb->impl = std::make_unique<cgj::Batch>(*ctx->impl, ps, ins);
+        b        = nb.release();
+    });
+    return b;
+}
+
+int cgj_batch_run(cgj_context* ctx, cgj_batch* b) {
+    return guarded(ctx, [&] { b->impl->run(); });
+}
+
+int cgj_batch_autotune(cgj_context* ctx, cgj_batch* b) {
+    return guarded(ctx, [&] { b->impl->autotune(); });
+}
+
+int cgj_batch_info(const cgj_batch* b, cgj_batch_stats* out) {
+    if (!b || !out) return -1;
+    out->ms_last  = b->impl->ms_last;
+    out->tasks    = b->impl->tasks;
+    out->waves    = b->impl->waves;
+    out->launches = b->impl->launches;
+    out->retries  = b->impl->retries;
+    out->jobs     = b->impl->size();
+    return 0;
+}
+
+uint64_t cgj_batch_num_rows(const cgj_batch* b, uint32_t plan) {
+    if (!b || plan >= b->impl->size()) return 0;
+    return b->impl->rows(plan);
+}
+
+int cgj_batch_download(cgj_context* ctx, cgj_batch* b, uint32_t plan, cgj_result* out) {
+    return guarded(ctx, [&] {
+        std::memset(out, 0, sizeof(*out));
+        b->impl->download(plan, *out);
+    });
+}
+
+void cgj_batch_free(cgj_context*, cgj_batch* b) { delete b; }
 
 size_t cgj_stats_size(void) { return sizeof(cgj_stats); }
 

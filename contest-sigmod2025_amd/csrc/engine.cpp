@@ -924,6 +924,17 @@ std::unique_ptr<DevResult> Context::run_query(const cgj_plan& plan, const std::v
 }
 
 void Context::download(const DevResult& r, cgj_result& out) {
+    PoolView v;
+    v.nrows          = r.nrows;
+    v.types          = r.types;
+    v.pool           = r.pool.get<uint8_t>();
+    v.pool_pages     = r.pool_pages;
+    v.reserved_pages = r.reserved_pages;
+    v.total_pages    = r.total_pages;
+    download_pool(v, out);
+}
+
+void Context::download_pool(const PoolView& r, cgj_result& out) {
     out.num_rows    = r.nrows;
     out.num_columns = uint32_t(r.types.size());
     out.columns     = new cgj_result_column[r.types.size()];
@@ -936,7 +947,8 @@ void Context::download(const DevResult& r, cgj_result& out) {
     // The directory has one entry per reserved pool page: (column, wave
     // stream, sequence, valid). Page order per column: stream, then sequence.
     std::vector<uint32_t> dir(r.reserved_pages * 4);
-    CGJ_HIP(hipMemcpyAsync(dir.data(), r.dir(), dir.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    CGJ_HIP(hipMemcpyAsync(dir.data(), r.pool + r.pool_pages * kPageSize, dir.size() * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, stream_));
     CGJ_HIP(hipStreamSynchronize(stream_));
     std::vector<uint64_t> order;
     order.reserve(r.total_pages);
@@ -974,7 +986,7 @@ void Context::download(const DevResult& r, cgj_result& out) {
             for (size_t i = 0; i < pages[c].size(); ++i) where[pages[c][i]] = {uint32_t(c), uint32_t(i)};
         for (uint64_t p0 = 0; p0 < used; p0 += kChunkPages) {
             uint64_t n = std::min(kChunkPages, used - p0);
-            CGJ_HIP(hipMemcpyAsync(pinned, r.pool.get<uint8_t>() + p0 * kPageSize, n * kPageSize,
+            CGJ_HIP(hipMemcpyAsync(pinned, r.pool + p0 * kPageSize, n * kPageSize,
                                    hipMemcpyDeviceToHost, stream_));
             CGJ_HIP(hipStreamSynchronize(stream_));
             for (uint64_t i = 0; i < n; ++i) {

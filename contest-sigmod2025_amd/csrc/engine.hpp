@@ -10,6 +10,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -111,6 +112,17 @@ struct DevResult {
     const uint32_t* dir() const { return reinterpret_cast<const uint32_t*>(pool.get<uint8_t>() + pool_pages * 8192); }
 };
 
+// Output pages in a pool (a DevResult or one job of a batch): `pool` holds
+// `pool_pages` page slots followed by their directory.
+struct PoolView {
+    uint64_t             nrows = 0;
+    std::vector<int32_t> types;
+    const uint8_t*       pool           = nullptr;
+    uint64_t             pool_pages     = 0;
+    uint64_t             reserved_pages = 0;
+    uint64_t             total_pages    = 0;
+};
+
 class Context;
 
 // Tunables of one hash join, chosen per (build rows, probe rows, key width).
@@ -135,6 +147,31 @@ struct Options {
     uint32_t build_regions_log     = 14;                 // partitioned build: at most 2^this regions
 };
 
+// Tunables of one batched join.
+struct BatchTune {
+    double   max_load        = 0.5;    // hash-table capacity = next_pow2(build / max_load)
+    uint32_t bloom_bits      = 0;      // bloom bits per key (0 = none)
+    uint32_t probe_task_rows = 65536;  // probe rows per task (multiple of 64)
+    uint32_t build_task_rows = 32768;  // build rows per task (multiple of 64)
+};
+
+// Chooses BatchTune per (build rows, probe rows, key width) size class. Starts
+// from rules (cache sizes, task counts) and keeps the settings that measured
+// fastest per class in Batch::autotune (per-task wall-clock ticks, summed per
+// job in device memory). The table lives in the context, so later batches of
+// similar shapes reuse it.
+class BatchTuner {
+public:
+    using Key = std::tuple<uint32_t, uint32_t, uint32_t>; // log2 build rows, log2 probe/build, key bytes
+    static Key key(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes);
+    BatchTune  pick(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes) const;
+    void       store(const Key& k, const BatchTune& t) { tuned_[k] = t; }
+    size_t     tuned_classes() const { return tuned_.size(); }
+
+private:
+    std::map<Key, BatchTune> tuned_;
+};
+
 class Context {
 public:
     explicit Context(int device);
@@ -156,12 +193,14 @@ public:
     std::unique_ptr<DevResult> execute(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
     uint64_t count(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
     void     download(const DevResult& r, cgj_result& out);
+    void     download_pool(const PoolView& v, cgj_result& out);
     double   copy_probe(uint64_t bytes, uint32_t reps);
 
     cgj_stats   stats{};
     std::string error;
 
     JoinTuning tune(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes) const;
+    BatchTuner batch_tuner; // per size class settings of batched joins (kept across batches)
     uint32_t   cu_count() const { return cus_; }
     uint64_t   lds_bytes() const { return lds_; } // LDS per workgroup
 

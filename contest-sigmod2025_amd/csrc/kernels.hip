@@ -498,7 +498,6 @@ struct OutState {
     uint64_t pad;
 };
 constexpr uint32_t kNoPage    = 0xffffffffu;
-constexpr uint32_t kPageChunk = 32; // pool pages reserved per atomic
 
 __device__ inline int32_t fixed_budget(int32_t type) { return 8 * (int32_t(kPageSize) - 4 - (type == T_INT32 ? 0 : 4)); }
 
@@ -948,44 +947,7 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
 // a 64-row batch is appended to a column with one ballot, one masked store and
 // a register OR; the batch's validity in row order comes from a single forward
 // lane permute shared by all columns.
-constexpr int kLeanOut = 4;
-
-enum : uint32_t { SRC_KEY = 0, SRC_SCAN = 1, SRC_HT = 2 };
-
-struct LeanOutCol {
-    ScanCol  col;   // SRC_SCAN over pages
-    uint32_t src;   // SRC_*
-    uint32_t word;  // SRC_SCAN over tuples: tuple word; SRC_HT: entry word
-    int32_t  type;  // output type (fixed width)
-    uint32_t pad;
-};
-
-struct LeanArgs {
-    ScanCol         key;        // probe key column (DRIVER_PAGES)
-    const uint64_t* tuples;     // DRIVER_TUPLES: rows x tuple_words, word 0 never NULL but in holes
-    uint32_t        tuple_words;
-    uint32_t        key_word;   // DRIVER_TUPLES: tuple word of the probe key
-    uint32_t        driver;
-    uint32_t        morsel_rows;
-    uint32_t        xcd_order;
-    uint32_t        unique;
-    uint64_t        nrows;
-    uint64_t        nmorsels;
-    const uint64_t* ht;
-    uint64_t        mask;
-    uint32_t        shift;
-    uint32_t        words;
-    const uint64_t* bloom;
-    uint32_t        bloom_log;
-    uint32_t        nout;
-    LeanOutCol      out[kLeanOut];
-    uint8_t*        pool;
-    uint64_t        pool_pages;
-    uint32_t*       dir;
-    PipeCounters*   counters;
-    uint32_t        page_chunk; // pool pages reserved per atomic (1..kPageChunk)
-    uint32_t        pad;
-};
+// (LeanArgs, LeanOutCol and the SRC_* codes are declared in kernels.hpp.)
 
 // Open page of one output column (wave-uniform).
 struct LeanPage {
@@ -1336,6 +1298,165 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
         m = m_next;
     }
     lean_flush<NO>(a, W, stream);
+}
+
+// ---------------------------------------------------------------------------
+// batched joins: persistent kernels over device queues of build and probe tasks
+// ---------------------------------------------------------------------------
+// One launch takes the build tasks of every job of the batch, the next launch
+// the probe tasks: the kernel boundary orders every probe after every build
+// (and makes the tables visible to all XCDs), so no wave ever waits on another.
+//
+// Build task: inserts rows [index * morsel_rows, ...) of the job's build table
+// (NULL keys skipped) with compare-and-swap on the key word, then stores the
+// attribute words, and flags duplicate keys.
+__device__ inline void batch_build_task(uintptr_t job_addr, uint64_t index) {
+    const uint32_t  lane = lane_id();
+    const BatchJob& J0   = args_at<BatchJob>(job_addr);
+    const uint64_t  r0   = index * J0.build.morsel_rows;
+    const uint64_t  end  = r0 + J0.build.morsel_rows < J0.build.rows ? r0 + J0.build.morsel_rows : J0.build.rows;
+    bool            dup  = false;
+    for (uint64_t rb = r0; rb < end; rb += 64) {
+        const BatchBuild& B      = args_at<BatchJob>(job_addr).build;
+        const uint64_t    row    = rb + lane;
+        const bool        active = row < end;
+        const uint64_t    blk    = rb / 64;
+        const uint64_t    key    = decode_block(B.key, blk, active);
+        uint64_t          at[kBatchAttrs];
+#pragma unroll
+        for (int t = 0; t < kBatchAttrs; ++t) at[t] = uint32_t(t) < B.nattr ? decode_block(B.attr[t], blk, active) : 0;
+        if (active && key != kNull) {
+            const uint64_t h     = hash_key(key);
+            const uint32_t words = B.words;
+            uint64_t       pos   = h >> B.shift;
+            for (;;) {
+                const unsigned long long prev =
+                    atomicCAS(reinterpret_cast<unsigned long long*>(B.ht + pos * words), kEmpty, key);
+                if (prev == kEmpty) break;
+                dup = dup || prev == key;
+                pos = (pos + 1) & B.mask;
+            }
+#pragma unroll
+            for (int t = 0; t < kBatchAttrs; ++t)
+                if (uint32_t(t) < B.nattr) B.ht[pos * words + 1 + t] = at[t];
+            if (B.bloom) {
+                const uint64_t h2 = hash_bloom(h);
+                atomicOr(reinterpret_cast<unsigned long long*>(B.bloom + bloom_word(h2, B.bloom_log)), bloom_bits(h2));
+            }
+        }
+    }
+    BatchSync* sync = args_at<BatchJob>(job_addr).sync;
+    if (__ballot(dup) && lane == 0) atomicOr(&sync->dup, 1u);
+}
+
+// Probe task (a later launch than every build task): runs the one-probe
+// pipeline over rows [index * morsel_rows, ...) of the probe table and flushes
+// its pages; a table without duplicate keys stops at the first match.
+template <int NO, int NS>
+__device__ inline void batch_probe_task(uintptr_t job_addr, uint64_t index, uint32_t stream, LeanWave<NS>& W) {
+    const uint32_t  lane = lane_id();
+    const BatchJob& J    = args_at<BatchJob>(job_addr);
+    BatchSync*      sync = J.sync;
+    uint32_t        dup  = 0;
+    if (lane == 0) dup = __hip_atomic_load(&sync->dup, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    dup                   = uniform_u32(dup);
+    const uintptr_t pa    = job_addr + offsetof(BatchJob, probe);
+    const LeanArgs& a     = args_at<LeanArgs>(pa);
+    const uint64_t  r0    = index * a.morsel_rows;
+    const uint64_t  end   = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
+    lean_rows<NO>(pa, r0, end, stream, W, dup ? 0 : 1);
+    lean_flush<NO>(args_at<LeanArgs>(pa), W, stream);
+}
+
+// Takes tasks from `queue` until `ntasks` are handed out; each wave's task
+// time (wall clock) is added to its job's counters.
+__global__ __launch_bounds__(256) void k_batch_build(const BatchJob* jobs, const BatchTask* tasks, uint64_t ntasks,
+                                                     unsigned long long* queue) {
+    const uint32_t lane = lane_id();
+    for (;;) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(queue, 1ull);
+        t = uniform_u64(t);
+        if (t >= ntasks) break;
+        const uint4     tk  = load_invariant_u4(tasks + t);
+        const uintptr_t ja  = uintptr_t(jobs + tk.x);
+        const uint64_t  idx = uint64_t(tk.z) | (uint64_t(tk.w) << 32);
+        const uint64_t  t0  = wall_clock64();
+        batch_build_task(ja, idx);
+        const uint64_t dt   = wall_clock64() - t0;
+        BatchSync*     sync = args_at<BatchJob>(ja).sync; // outside the lane-0 branch: scalar operand
+        if (lane == 0) atomicAdd(&sync->build_clocks, (unsigned long long)dt);
+    }
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_batch_probe(const BatchJob* jobs, const BatchTask* tasks, uint64_t ntasks,
+                                                     unsigned long long* queue) {
+    constexpr int  NS     = NO > 0 ? NO : 1;
+    const uint32_t stream = blockIdx.x * kWavesPerBlock + wave_in_block();
+    const uint32_t lane   = lane_id();
+    LeanWave<NS>   W;
+    lean_wave_init(W);
+    for (;;) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(queue, 1ull);
+        t = uniform_u64(t);
+        if (t >= ntasks) break;
+        const uint4     tk  = load_invariant_u4(tasks + t);
+        const uintptr_t ja  = uintptr_t(jobs + tk.x);
+        const uint64_t  idx = uint64_t(tk.z) | (uint64_t(tk.w) << 32);
+        const uint64_t  t0  = wall_clock64();
+        batch_probe_task<NO>(ja, idx, stream, W);
+        const uint64_t dt   = wall_clock64() - t0;
+        BatchSync*     sync = args_at<BatchJob>(ja).sync;
+        if (lane == 0) atomicAdd(&sync->probe_clocks, (unsigned long long)dt);
+    }
+}
+
+uint32_t batch_resident_waves(uint32_t nout) {
+    static const void* const kernels[kLeanOut + 2] = {
+        reinterpret_cast<const void*>(&k_batch_probe<0>), reinterpret_cast<const void*>(&k_batch_probe<1>),
+        reinterpret_cast<const void*>(&k_batch_probe<2>), reinterpret_cast<const void*>(&k_batch_probe<3>),
+        reinterpret_cast<const void*>(&k_batch_probe<4>), reinterpret_cast<const void*>(&k_batch_build)};
+    const uint32_t k = nout == kBatchBuildKernel ? uint32_t(kLeanOut) + 1
+                                                 : (nout < uint32_t(kLeanOut) ? nout : uint32_t(kLeanOut));
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[k], 256, 0) != hipSuccess || per_cu <= 0 ||
+        cus <= 0)
+        return 256 * kWavesPerBlock;
+    return uint32_t(per_cu) * uint32_t(cus) * kWavesPerBlock;
+}
+
+double batch_clock_hz() {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+        return double(khz) * 1000.0;
+    return 100e6;
+}
+
+hipError_t launch_batch(const BatchJob* jobs, const BatchTask* tasks, uint64_t ntasks, unsigned long long* queue,
+                        uint32_t nout, uint32_t max_waves, hipStream_t s) {
+    if (nout > uint32_t(kLeanOut) && nout != kBatchBuildKernel) return hipErrorInvalidValue;
+    if (ntasks == 0) return hipSuccess;
+    // A persistent grid: no more waves than fit the GPU at once (more would
+    // only queue behind the first ones), nor more than there are tasks. The
+    // caller passes the resident count when it launches under stream capture.
+    uint32_t waves = max_waves ? max_waves : batch_resident_waves(nout);
+    if (uint64_t(waves) > ntasks) waves = uint32_t(ntasks);
+    uint32_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks == 0) blocks = 1;
+    switch (nout) {
+    case 0: k_batch_probe<0><<<dim3(blocks), dim3(256), 0, s>>>(jobs, tasks, ntasks, queue); break;
+    case 1: k_batch_probe<1><<<dim3(blocks), dim3(256), 0, s>>>(jobs, tasks, ntasks, queue); break;
+    case 2: k_batch_probe<2><<<dim3(blocks), dim3(256), 0, s>>>(jobs, tasks, ntasks, queue); break;
+    case 3: k_batch_probe<3><<<dim3(blocks), dim3(256), 0, s>>>(jobs, tasks, ntasks, queue); break;
+    case 4: k_batch_probe<4><<<dim3(blocks), dim3(256), 0, s>>>(jobs, tasks, ntasks, queue); break;
+    default: k_batch_build<<<dim3(blocks), dim3(256), 0, s>>>(jobs, tasks, ntasks, queue); break;
+    }
+    return hipGetLastError();
 }
 
 // The lean kernel applies when the pipeline has one probe without residual

@@ -88,6 +88,104 @@ struct PipeArgs {
     PipeCounters*   counters;
 };
 
+// ---- lean one-probe pipeline ----
+constexpr int      kLeanOut   = 4;  // fixed-width output columns
+constexpr uint32_t kPageChunk = 32; // pool pages reserved per atomic (upper bound)
+
+// Where an output column's value comes from.
+enum : uint32_t { SRC_KEY = 0, SRC_SCAN = 1, SRC_HT = 2 };
+
+struct LeanOutCol {
+    ScanCol  col;   // SRC_SCAN over pages
+    uint32_t src;   // SRC_*
+    uint32_t word;  // SRC_SCAN over tuples: tuple word; SRC_HT: entry word
+    int32_t  type;  // output type (fixed width)
+    uint32_t pad;
+};
+
+struct LeanArgs {
+    ScanCol         key;        // probe key column (DRIVER_PAGES)
+    const uint64_t* tuples;     // DRIVER_TUPLES: rows x tuple_words, word 0 never NULL but in holes
+    uint32_t        tuple_words;
+    uint32_t        key_word;   // DRIVER_TUPLES: tuple word of the probe key
+    uint32_t        driver;
+    uint32_t        morsel_rows;
+    uint32_t        xcd_order;
+    uint32_t        unique;
+    uint64_t        nrows;
+    uint64_t        nmorsels;
+    const uint64_t* ht;
+    uint64_t        mask;
+    uint32_t        shift;
+    uint32_t        words;
+    const uint64_t* bloom;
+    uint32_t        bloom_log;
+    uint32_t        nout;
+    LeanOutCol      out[kLeanOut];
+    uint8_t*        pool;
+    uint64_t        pool_pages;
+    uint32_t*       dir;
+    PipeCounters*   counters;
+    uint32_t        page_chunk; // pool pages reserved per atomic (1..kPageChunk)
+    uint32_t        pad;
+};
+
+// ---- batched joins: one persistent launch over a device work queue ----
+constexpr int kBatchAttrs = 3; // build-side words besides the key
+
+// Build side of one batched join: the build table's key and attribute columns
+// are inserted into `ht` (global compare-and-swap, linear probing over the
+// whole table) in tasks of `morsel_rows` rows.
+struct BatchBuild {
+    ScanCol   key;
+    ScanCol   attr[kBatchAttrs];
+    uint32_t  nattr;
+    uint32_t  morsel_rows;
+    uint64_t  rows;
+    uint32_t  morsels;
+    uint32_t  bloom_log;
+    uint64_t* ht;
+    uint64_t  mask;  // capacity - 1
+    uint32_t  shift; // slot = hash >> shift
+    uint32_t  words;
+    uint64_t* bloom;
+};
+
+// Per-job flags and timing, in device memory (zeroed before every run).
+struct BatchSync {
+    unsigned int       dup;          // a duplicate build key was seen
+    unsigned int       pad[3];
+    unsigned long long build_clocks; // wall-clock ticks spent in build tasks
+    unsigned long long probe_clocks; // ... in probe tasks
+};
+
+struct BatchJob {
+    LeanArgs   probe; // nmorsels = probe tasks of morsel_rows rows each
+    BatchBuild build;
+    BatchSync* sync;
+    uint64_t   pad;
+};
+
+// A queue entry: build task `index` or probe task `index` of job `job`.
+struct BatchTask {
+    uint32_t job;
+    uint32_t kind; // 0 build, 1 probe
+    uint64_t index;
+};
+
+// `nout` value that selects the build kernel in launch_batch / batch_resident_waves.
+constexpr uint32_t kBatchBuildKernel = 0xffffffffu;
+// Runs `ntasks` tasks of one kind with a persistent grid of `max_waves` waves
+// (0: as many as fit the GPU); `queue` is the zeroed task counter. With
+// nout = kBatchBuildKernel the tasks are build tasks; otherwise they are probe
+// tasks of jobs with `nout` output columns, launched after the build launch.
+hipError_t launch_batch(const BatchJob* jobs, const BatchTask* tasks, uint64_t ntasks, unsigned long long* queue,
+                        uint32_t nout, uint32_t max_waves, hipStream_t s);
+// Waves of the build kernel / the probe kernel for `nout` outputs that fit the GPU at once.
+uint32_t batch_resident_waves(uint32_t nout);
+// Ticks per second of the clock behind BatchSync's *_clocks.
+double batch_clock_hz();
+
 // Radix partitioning of a driving table by the hash of its probe key.
 struct RadixArgs {
     ScanCol   scan[kMaxScan]; // column 0 is the probe key

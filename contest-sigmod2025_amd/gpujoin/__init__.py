@@ -112,6 +112,20 @@ class GenColumn(C.Structure):
     ]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("ms_last", C.c_double),
+        ("tasks", C.c_uint64),
+        ("waves", C.c_uint32),
+        ("launches", C.c_uint32),
+        ("retries", C.c_uint32),
+        ("jobs", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # exported symbols of include/gpujoin.h with their ctypes signatures
 SIGNATURES = {
     "cgj_context_create": (C.c_void_p, [C.c_int]),
@@ -142,6 +156,14 @@ SIGNATURES = {
     "cgj_copy_probe": (C.c_double, [C.c_void_p, C.c_uint64, C.c_uint32]),
     "cgj_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     "cgj_stats_size": (C.c_size_t, []),
+    "cgj_batch_create": (C.c_void_p, [C.c_void_p, C.POINTER(CPlan), C.c_uint32, C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_uint32)]),
+    "cgj_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cgj_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cgj_batch_info": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    "cgj_batch_num_rows": (C.c_uint64, [C.c_void_p, C.c_uint32]),
+    "cgj_batch_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(Result)]),
+    "cgj_batch_free": (None, [C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -374,6 +396,60 @@ class DeviceResult:
             pass
 
 
+class Batch:
+    """Independent two-table join plans over device tables, run together by one
+    persistent kernel launch (cgj_batch_*)."""
+
+    def __init__(self, ctx: "Context", plans: Sequence[Plan], tables: Sequence[Sequence[DeviceTable]]):
+        self.ctx = ctx
+        self._marshals = [_Marshal(p, []) for p in plans]
+        cplans = (CPlan * max(len(plans), 1))(*[m.plan for m in self._marshals])
+        flat = [t.handle for ts in tables for t in ts]
+        arr = (C.c_void_p * max(len(flat), 1))(*flat)
+        counts = (C.c_uint32 * max(len(plans), 1))(*[len(ts) for ts in tables])
+        self._tables = tables
+        self.handle = lib().cgj_batch_create(ctx.handle, cplans, len(plans), arr, counts)
+        if not self.handle:
+            raise RuntimeError(ctx.error())
+        self.size = len(plans)
+
+    def run(self):
+        if lib().cgj_batch_run(self.ctx.handle, self.handle) != 0:
+            raise RuntimeError(self.ctx.error())
+
+    def autotune(self):
+        if lib().cgj_batch_autotune(self.ctx.handle, self.handle) != 0:
+            raise RuntimeError(self.ctx.error())
+
+    def info(self) -> dict:
+        s = BatchStats()
+        lib().cgj_batch_info(self.handle, C.byref(s))
+        return s.as_dict()
+
+    def num_rows(self, j: int) -> int:
+        return lib().cgj_batch_num_rows(self.handle, j)
+
+    def download(self, j: int) -> ColumnarTable:
+        res = Result()
+        if lib().cgj_batch_download(self.ctx.handle, self.handle, j, C.byref(res)) != 0:
+            raise RuntimeError(self.ctx.error())
+        try:
+            return result_to_table(res)
+        finally:
+            lib().cgj_result_release(C.byref(res))
+
+    def free(self):
+        if self.handle:
+            lib().cgj_batch_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Context:
     """Mirror of Contest::build_context / execute / destroy_context."""
 
@@ -425,6 +501,10 @@ class Context:
         if lib().cgj_count_device(self.handle, C.byref(m.plan), arr, len(tables), C.byref(out)) != 0:
             raise RuntimeError(self.error())
         return out.value
+
+    def batch(self, plans: Sequence[Plan], tables: Sequence[Sequence[DeviceTable]]) -> "Batch":
+        """Prepares plans (each one join of two scans) for batched execution."""
+        return Batch(self, plans, tables)
 
     def copy_probe(self, nbytes: int, reps: int = 5) -> float:
         """Mean time (ms) of a plain HBM-to-HBM copy of `nbytes` bytes."""

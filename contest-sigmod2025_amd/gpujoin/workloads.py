@@ -6,6 +6,8 @@ generator produce the same rows and any row range can be generated alone.
 """
 from __future__ import annotations
 
+import math
+import random
 from dataclasses import dataclass
 from typing import Callable, List
 
@@ -127,4 +129,48 @@ def c3(scale: float = 1.0) -> Workload:
                     f"snowflake: fact {nf} rows x 4 pre-filtered dimensions, fused 4-probe chain with bloom filters")
 
 
+@dataclass
+class BatchWorkload:
+    name: str
+    jobs: List[Workload]  # independent two-table joins
+    description: str
+
+
+def c4(scale: float = 1.0, njobs: int = 256) -> BatchWorkload:
+    """`njobs` independent INT32 equi-joins submitted together. Build rows are
+    log-uniform in [1K, 2M] (times `scale`; unique shuffled keys + INT32
+    payload); the probe side has 10x the build rows (FK uniform over the build
+    keys, 5% non-matching keys, INT32 payload). Output {build payload, probe
+    payload}. Sizes come from a seeded generator; every column has its own
+    stream."""
+    rng = random.Random(0x5EED0004)
+    jobs = []
+
+    def plan():
+        p = Plan()
+        b = p.new_scan_node(0, [(0, INT32), (1, INT32)])
+        q = p.new_scan_node(1, [(0, INT32), (1, INT32)])
+        p.root = p.new_join_node(True, b, q, 0, 0, [(1, INT32), (3, INT32)])
+        return p
+
+    for j in range(njobs):
+        nb = max(int(round(math.exp(rng.uniform(math.log(1000), math.log(2_000_000))) * scale)), 64)
+        npr = 10 * nb
+        s = 1000 + 8 * j
+        build = TableSpec(f"build{j}", nb, [
+            GenColumn(type=INT32, kind=GEN_PERM_KEY, stream=s, domain=nb),
+            GenColumn(type=INT32, kind=GEN_UNIFORM, stream=s + 1, lo=0, hi=1 << 31),
+        ])
+        probe = TableSpec(f"probe{j}", npr, [
+            GenColumn(type=INT32, kind=GEN_FK_UNIFORM, stream=s + 2, domain=nb, miss_frac=0.05),
+            GenColumn(type=INT32, kind=GEN_UNIFORM, stream=s + 3, lo=0, hi=1 << 31),
+        ])
+        jobs.append(Workload(f"c4.{j}", 0x5EED0004, [build, probe], plan, f"INT32 join {nb} x {npr}"))
+    total_b = sum(w.tables[0].rows for w in jobs)
+    total_p = sum(w.tables[1].rows for w in jobs)
+    return BatchWorkload("c4", jobs, f"{njobs} batched INT32 equi-joins, build log-uniform [1K, 2M] x {scale:g} "
+                                     f"({total_b} rows), probe 10x build ({total_p} rows), 5% misses")
+
+
 WORKLOADS = {"c1": c1, "c2": c2, "c3": c3}
+BATCH_WORKLOADS = {"c4": c4}

@@ -184,6 +184,39 @@ int      cgj_count_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* co
  * achievable copy bandwidth next to the roofline peak. */
 double   cgj_copy_probe(cgj_context* ctx, uint64_t bytes, uint32_t reps);
 
+/* ---- batched joins ----
+ * Many independent plans, each one equi-join of two scans with up to four
+ * fixed-width output columns (the reference runs such queries one
+ * Contest::execute at a time, engine/Execute.cpp:28-40 → QueryPlan.cpp:414-678
+ * per join). A batch runs them all in one hipGraph launch: table fills plus one
+ * persistent kernel per output width, fed by a device queue of build and
+ * probe tasks. No reference interface is replaced; this is the batched form of
+ * cgj_execute_device over the same plans, with the same results per plan. */
+typedef struct cgj_batch cgj_batch;
+
+typedef struct {
+    double   ms_last;  /* device time of the last cgj_batch_run (graph launch) */
+    uint64_t tasks;    /* queue entries (build + probe tasks)                  */
+    uint32_t waves;    /* persistent grid (resident waves)                     */
+    uint32_t launches; /* kernel nodes in the graph                            */
+    uint32_t retries;  /* re-runs after a job's page pool overflowed           */
+    uint32_t jobs;
+} cgj_batch_stats;
+
+/* `inputs` holds num_inputs[0] tables of plan 0, then those of plan 1, ...
+ * Plans of any other shape are rejected (NULL; cgj_last_error says why). */
+cgj_batch* cgj_batch_create(cgj_context* ctx, const cgj_plan* plans, uint32_t num_plans, cgj_dtable* const* inputs,
+                            const uint32_t* num_inputs);
+int        cgj_batch_run(cgj_context* ctx, cgj_batch* b);
+/* Tunes task sizes, load factor and bloom bits per (build rows, probe rows,
+ * key width) class by timing candidate settings on this batch (several runs);
+ * the winners stay in the context for later batches. */
+int        cgj_batch_autotune(cgj_context* ctx, cgj_batch* b);
+int        cgj_batch_info(const cgj_batch* b, cgj_batch_stats* out);
+uint64_t   cgj_batch_num_rows(const cgj_batch* b, uint32_t plan);
+int        cgj_batch_download(cgj_context* ctx, cgj_batch* b, uint32_t plan, cgj_result* out);
+void       cgj_batch_free(cgj_context* ctx, cgj_batch* b);
+
 /* ---- operator-level entry points ---- */
 
 /* Page decode of one device column into 64-bit slots, NULL = all ones, INT32
