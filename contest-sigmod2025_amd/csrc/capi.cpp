@@ -9,6 +9,7 @@
 #include "../../include/gpujoin.h"
 #include "batch.hpp"
 #include "engine.hpp"
+#include "estimator.hpp"
 #include "gen.hpp"
 
 struct cgj_context {
@@ -23,6 +24,9 @@ struct cgj_dresult {
 };
 struct cgj_batch {
     std::unique_ptr<cgj::Batch> impl;
+};
+struct cgj_mlp {
+    std::shared_ptr<cgj::Mlp> impl; // shared with a context's learned estimator
 };
 
 namespace {
@@ -187,6 +191,49 @@ int cgj_batch_download(cgj_context* ctx, cgj_batch* b, uint32_t plan, cgj_result
 
 void cgj_batch_free(cgj_context*, cgj_batch* b) { delete b; }
 
+cgj_mlp* cgj_mlp_create(cgj_context* ctx, uint32_t in_features, uint64_t seed) {
+    cgj_mlp* m = nullptr;
+    guarded(ctx, [&] {
+        auto nm  = std::make_unique<cgj_mlp>();
+        nm->impl = std::make_shared<cgj::Mlp>(*ctx->impl, in_features, seed);
+        m        = nm.release();
+    });
+    return m;
+}
+
+uint64_t cgj_mlp_num_params(const cgj_mlp* m) { return m ? m->impl->num_params() : 0; }
+
+int cgj_mlp_set_params(cgj_context* ctx, cgj_mlp* m, const float* params) {
+    return guarded(ctx, [&] { m->impl->set_params(params); });
+}
+
+int cgj_mlp_get_params(cgj_context* ctx, const cgj_mlp* m, float* params) {
+    return guarded(ctx, [&] { m->impl->get_params(params); });
+}
+
+int cgj_mlp_loss_grad(cgj_context* ctx, cgj_mlp* m, const float* x, const float* y, uint32_t n, float* loss,
+                      float* grads) {
+    return guarded(ctx, [&] { *loss = m->impl->loss_grad(x, y, n, grads); });
+}
+
+int cgj_mlp_train(cgj_context* ctx, cgj_mlp* m, const float* x, const float* y, uint32_t n, uint32_t epochs,
+                  uint32_t batch, float lr, uint64_t seed, float* final_loss) {
+    return guarded(ctx, [&] {
+        float l = m->impl->train(x, y, n, epochs, batch, lr, seed);
+        if (final_loss) *final_loss = l;
+    });
+}
+
+int cgj_mlp_predict(cgj_context* ctx, cgj_mlp* m, const float* x, uint32_t n, float* out) {
+    return guarded(ctx, [&] { m->impl->predict(x, n, out); });
+}
+
+int cgj_mlp_predict_f32(cgj_context* ctx, cgj_mlp* m, const float* x, uint32_t n, float* out) {
+    return guarded(ctx, [&] { m->impl->predict_f32(x, n, out); });
+}
+
+void cgj_mlp_free(cgj_context*, cgj_mlp* m) { delete m; }
+
 size_t cgj_stats_size(void) { return sizeof(cgj_stats); }
 
 int cgj_set_option(cgj_context* ctx, const char* name, double value) {
@@ -200,6 +247,79 @@ int cgj_count_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* 
         std::vector<const cgj::DevTable*> views;
         for (uint32_t i = 0; i < num_inputs; ++i) views.push_back(inputs[i]->impl.get());
         *rows = ctx->impl->count(*plan, views);
+    });
+}
+
+namespace {
+std::vector<const cgj::DevTable*> views_of(cgj_dtable* const* inputs, uint32_t n) {
+    std::vector<const cgj::DevTable*> v;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!inputs[i]) throw std::runtime_error("null input table");
+        v.push_back(inputs[i]->impl.get());
+    }
+    return v;
+}
+} // namespace
+
+uint32_t cgj_num_features(void) { return cgj::kFeatures; }
+
+int cgj_set_estimator(cgj_context* ctx, uint32_t mode, cgj_mlp* mlp, float max_sigma) {
+    return guarded(ctx, [&] {
+        if (mode > cgj::EST_LEARNED) throw std::runtime_error("unknown estimator mode");
+        if (mode == cgj::EST_LEARNED && (!mlp || mlp->impl->in() != cgj::kFeatures))
+            throw std::runtime_error("the learned estimator needs a network with cgj_num_features() inputs");
+        auto& e     = ctx->impl->estimator();
+        e.mode      = mode;
+        e.mlp       = mode == cgj::EST_LEARNED ? mlp->impl : nullptr;
+        e.max_sigma = max_sigma;
+    });
+}
+
+int64_t cgj_plan_features(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                          uint64_t* sets, float* features, uint64_t cap) {
+    int64_t n  = -1;
+    int     rc = guarded(ctx, [&] {
+        auto                  views = views_of(inputs, num_inputs);
+        std::vector<uint64_t> rows;
+        std::vector<uint32_t> cols;
+        for (auto* t : views) {
+            rows.push_back(t->nrows);
+            cols.push_back(uint32_t(t->cols.size()));
+        }
+        cgj::Query            q = cgj::import_plan(*plan, rows, cols);
+        cgj::EstimateReport   rep;
+        cgj::QueryStats       st = ctx->impl->estimator().stats(*ctx->impl, q, views, rep);
+        std::vector<uint64_t> all;
+        if (!cgj::connected_subsets(q, 1u << 20, all)) throw std::runtime_error("too many connected relation sets");
+        uint64_t k = 0;
+        for (uint64_t set : all) {
+            if (!(set & (set - 1))) continue;
+            if (k < cap) {
+                sets[k] = set;
+                cgj::featurize(q, st, set, features + k * cgj::kFeatures);
+            }
+            ++k;
+        }
+        n = int64_t(k);
+    });
+    return rc == 0 ? n : -1;
+}
+
+int cgj_count_subset(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                     uint64_t relset, uint64_t* rows) {
+    return guarded(ctx, [&] { *rows = ctx->impl->count_subset(*plan, views_of(inputs, num_inputs), relset); });
+}
+
+int cgj_plan_joins(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                   uint64_t* build_sets, uint64_t* probe_sets, double* cards, uint32_t cap, uint32_t* num_joins) {
+    return guarded(ctx, [&] {
+        auto joins = ctx->impl->plan_tree(*plan, views_of(inputs, num_inputs));
+        *num_joins = uint32_t(joins.size());
+        for (uint32_t i = 0; i < joins.size() && i < cap; ++i) {
+            build_sets[i] = joins[i].build;
+            probe_sets[i] = joins[i].probe;
+            cards[i]      = joins[i].card;
+        }
     });
 }
 

@@ -11,12 +11,14 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
 #include <numeric>
 
+#include "estimator.hpp"
 #include "gen.hpp"
 #include "planner.hpp"
 
@@ -81,7 +83,12 @@ ScanCol DevColumn::view(uint32_t col_id) const {
 // ---------------------------------------------------------------------------
 // context, tables
 // ---------------------------------------------------------------------------
-Context::Context(int device) : device_(device) {
+uint64_t next_table_uid() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
+}
+
+Context::Context(int device) : device_(device), est_(std::make_unique<Estimator>()) {
     CGJ_HIP(hipSetDevice(device));
     CGJ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     hipDeviceProp_t prop;
@@ -93,6 +100,7 @@ Context::Context(int device) : device_(device) {
 
 Context::~Context() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    est_.reset(); // its network's buffers go back to the heap
     heap_->trim();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -878,25 +886,72 @@ double Context::copy_probe(uint64_t bytes, uint32_t reps) {
     return double(ms) / std::max<uint32_t>(reps, 1);
 }
 
-std::unique_ptr<DevResult> Context::run_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs,
-                                              bool count_only) {
+void Context::check_types(const Query& q, const std::vector<const DevTable*>& inputs) const {
+    for (auto& r : q.rels)
+        for (uint32_t a : r.attrs)
+            if (r.rows && inputs[r.table]->cols[q.attrs[a].col].type != q.attrs[a].type)
+                throw std::runtime_error("plan column type differs from the input column type");
+}
+
+static Query import_inputs(const cgj_plan& plan, const std::vector<const DevTable*>& inputs) {
     std::vector<uint64_t> rows;
     std::vector<uint32_t> cols;
     for (auto* t : inputs) {
         rows.push_back(t->nrows);
         cols.push_back(uint32_t(t->cols.size()));
     }
-    Query               q = import_plan(plan, rows, cols);
+    return import_plan(plan, rows, cols);
+}
+
+uint64_t Context::count_query(const Query& q, const std::vector<const DevTable*>& inputs) {
+    if (q.rels.size() == 1) return q.rels[0].rows;
     std::vector<double> base;
     for (auto& r : q.rels) base.push_back(double(r.rows));
     PlanTree tree = plan_joins(q, max_card_model(q, base));
-    for (auto& r : q.rels)
-        for (uint32_t a : r.attrs)
-            if (r.rows && inputs[r.table]->cols[q.attrs[a].col].type != q.attrs[a].type)
-                throw std::runtime_error("plan column type differs from the input column type");
+    Executor ex(*this, q, tree, inputs);
+    return ex.run(true)->nrows;
+}
+
+uint64_t Context::count_subset(const cgj_plan& plan, const std::vector<const DevTable*>& inputs, uint64_t relset) {
+    Query q = import_inputs(plan, inputs);
+    check_types(q, inputs);
+    return count_query(restrict_query(q, relset), inputs);
+}
+
+std::vector<Context::PlannedJoin> Context::plan_tree(const cgj_plan& plan, const std::vector<const DevTable*>& inputs) {
+    Query q = import_inputs(plan, inputs);
+    check_types(q, inputs);
+    EstimateReport           rep;
+    PlanTree                 tree = plan_joins(q, est_->model(*this, q, inputs, rep));
+    std::vector<PlannedJoin> out;
+    std::function<void(int)> rec = [&](int n) {
+        if (tree.leaf(n)) return;
+        const TreeNode& t = tree.nodes[n];
+        rec(t.build);
+        rec(t.probe);
+        out.push_back(PlannedJoin{tree.nodes[t.build].set, tree.nodes[t.probe].set, t.card});
+    };
+    rec(tree.root);
+    return out;
+}
+
+std::unique_ptr<DevResult> Context::run_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs,
+                                              bool count_only) {
+    Query q = import_inputs(plan, inputs);
+    check_types(q, inputs);
+    const auto     t0 = std::chrono::steady_clock::now();
+    EstimateReport rep;
+    PlanTree       tree = plan_joins(q, est_->model(*this, q, inputs, rep));
+    const double   ms_plan =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() - rep.ms_stats;
     Executor ex(*this, q, tree, inputs);
     auto     res = ex.run(count_only);
     stats                   = cgj_stats{};
+    stats.ms_plan           = ms_plan;
+    stats.ms_stats          = rep.ms_stats;
+    stats.est_sets          = rep.sets;
+    stats.est_fallbacks     = rep.fallbacks;
+    stats.est_stat_joins    = rep.stat_joins;
     stats.ms_build          = ex.ms_build;
     stats.ms_probe          = ex.ms_probe;
     stats.ms_total          = ex.ms_build + ex.ms_probe;

@@ -92,9 +92,13 @@ struct DevColumn {
     ScanCol view(uint32_t col_id) const;
 };
 
+// Process-wide distinct id of a device table (key of cached join statistics).
+uint64_t next_table_uid();
+
 struct DevTable {
     uint64_t               nrows = 0;
     std::vector<DevColumn> cols;
+    uint64_t               uid = next_table_uid();
 };
 
 // Output pages of a query, still in HBM: `pool` holds `pool_pages` page slots
@@ -124,6 +128,8 @@ struct PoolView {
 };
 
 class Context;
+class Estimator;
+struct Query;
 
 // Tunables of one hash join, chosen per (build rows, probe rows, key width).
 struct JoinTuning {
@@ -201,10 +207,28 @@ public:
 
     JoinTuning tune(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes) const;
     BatchTuner batch_tuner; // per size class settings of batched joins (kept across batches)
+
+    // Cardinality model of the join ordering (the max-of-inputs rule unless
+    // configured otherwise) and its cached join statistics.
+    Estimator& estimator() { return *est_; }
+    // Result rows of an imported query (any join order; no pages written).
+    uint64_t count_query(const Query& q, const std::vector<const DevTable*>& inputs);
+    // Result rows of the join of the connected relation set `relset` of a plan
+    // (relations numbered in the plan's depth-first scan order, left first).
+    uint64_t count_subset(const cgj_plan& plan, const std::vector<const DevTable*>& inputs, uint64_t relset);
+    // The join tree the planner picks for a plan under the current estimator:
+    // per join, the relation sets of its build side, its probe side, and the
+    // estimated cardinality of its result.
+    struct PlannedJoin {
+        uint64_t build, probe;
+        double   card;
+    };
+    std::vector<PlannedJoin> plan_tree(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
     uint32_t   cu_count() const { return cus_; }
     uint64_t   lds_bytes() const { return lds_; } // LDS per workgroup
 
 private:
+    void check_types(const Query& q, const std::vector<const DevTable*>& inputs) const;
     std::unique_ptr<DevResult> run_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs,
                                          bool count_only);
 
@@ -213,6 +237,7 @@ private:
     uint64_t    lds_    = 65536;
     hipStream_t stream_ = nullptr;
     std::shared_ptr<DeviceHeap> heap_ = std::make_shared<DeviceHeap>();
+    std::unique_ptr<Estimator>  est_;
 };
 
 } // namespace cgj

@@ -1,0 +1,130 @@
+// Learned cardinality estimation for join ordering (host side).
+//
+// The reference estimates a join's cardinality as the largest of its inputs
+// after sampling-based filter selectivities (engine/query/QueryPlan.cpp:66-159,
+// 719-746; engine/query/QueryGraph.cpp:53-68). Here a small MLP predicts the
+// log10 cardinality of a connected set of relations from features of the set
+// (relation sizes, exact pairwise join selectivities measured on the device,
+// shape), together with a confidence; below the confidence bar the planner
+// falls back to the independence estimate over the same measured pairs.
+// Estimates only steer the join order, so results are identical under every
+// estimator.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "engine.hpp"
+#include "mlp.hpp"
+#include "planner.hpp"
+
+namespace cgj {
+
+// x -> 256 -> 256 -> 2 network with its own FP32 training on the GPU
+// (forward, backward, Adam) and BF16-input MFMA inference.
+class Mlp {
+public:
+    Mlp(Context& ctx, uint32_t in, uint64_t seed);
+    uint32_t in() const { return in_; }
+    uint64_t num_params() const;
+    void     set_params(const float* p);
+    void     get_params(float* p) const;
+    // Mean loss over n rows and its gradient w.r.t. every parameter (FP32).
+    float    loss_grad(const float* x, const float* y, uint32_t n, float* grads);
+    // Adam over `epochs` passes of shuffled batches; returns the last epoch's mean loss.
+    float    train(const float* x, const float* y, uint32_t n, uint32_t epochs, uint32_t batch, float lr,
+                   uint64_t seed);
+    void     predict(const float* x, uint32_t n, float* out);     // BF16 MFMA, host arrays (n x 2)
+    void     predict_f32(const float* x, uint32_t n, float* out); // FP32 forward, host arrays
+
+private:
+    float forward(const float* xb, const float* yb, uint32_t m, bool backward);
+    void  ensure_rows(uint32_t m);
+    void  refresh_bf16();
+    uint64_t off_w1() const { return 0; }
+    uint64_t off_b1() const { return uint64_t(in_) * kMlpHidden; }
+    uint64_t off_w2() const { return off_b1() + kMlpHidden; }
+    uint64_t off_b2() const { return off_w2() + uint64_t(kMlpHidden) * kMlpHidden; }
+    uint64_t off_w3() const { return off_b2() + kMlpHidden; }
+    uint64_t off_b3() const { return off_w3() + uint64_t(kMlpHidden) * kMlpOut; }
+
+    Context& ctx_;
+    uint32_t in_;
+    DevBuf   params_, grads_, m_, v_, bf1_, bf2_, loss_;
+    DevBuf   a1_, a2_, out_, dout_, d2_, d1_;
+    uint32_t rows_   = 0;
+    bool     dirty_  = true;
+    uint64_t step_   = 0;
+};
+
+// Features of a connected relation set of a query (kFeatures floats).
+constexpr uint32_t kFeatures = 24;
+
+// Per-query statistics the features use: relation sizes and, per join class,
+// the exact 2-way join selectivity of every related relation pair.
+struct QueryStats {
+    struct Edge {
+        uint32_t a, b; // relations, a < b
+        uint32_t eq;   // join class
+        double   sel;  // |a join b| / (|a| |b|) on this class alone
+    };
+    std::vector<double> rows;
+    std::vector<Edge>   edges;
+};
+
+// Edges of `relset` the estimates use: per join class, the members inside the
+// set in relation order, consecutive pairs (a spanning chain of the class).
+std::vector<const QueryStats::Edge*> active_edges(const Query& q, const QueryStats& st, uint64_t relset);
+
+void featurize(const Query& q, const QueryStats& st, uint64_t relset, float* out);
+
+// log10 of the independence (System R) estimate: product of the relation
+// sizes and of the selectivities of the active edges.
+double log10_independence(const Query& q, const QueryStats& st, uint64_t relset);
+
+// Connected relation sets of the query graph, each once (EnumerateCsg of
+// Moerkotte & Neumann, VLDB 2006), singletons included; false when there are
+// more than `limit`.
+bool connected_subsets(const Query& q, uint32_t limit, std::vector<uint64_t>& out);
+
+enum : uint32_t { EST_MAX = 0, EST_INDEPENDENCE = 1, EST_LEARNED = 2 };
+
+struct EstimateReport {
+    uint32_t sets       = 0; // connected sets estimated (two or more relations)
+    uint32_t fallbacks  = 0; // of them, estimated by the independence rule (low confidence)
+    uint32_t stat_joins = 0; // 2-way counts run for the statistics catalog
+    double   ms_stats   = 0; // time in those counts
+    double   ms_model   = 0; // featurisation + inference
+};
+
+// Cardinality model of the join ordering.
+//   EST_MAX          the reference's rule: a join is as large as its largest input
+//   EST_INDEPENDENCE relation sizes x exact pairwise selectivities
+//   EST_LEARNED      the MLP's log10 estimate of every connected set, batched in
+//                    one BF16 MFMA inference; sets whose predicted spread
+//                    (sigma, log10 units) exceeds `max_sigma` fall back to the
+//                    independence rule
+// Pairwise selectivities come from exact 2-way counts on the device, cached
+// per (table, column) pair for the context's lifetime: tables of a shared
+// pool are joined by many plans.
+class Estimator {
+public:
+    uint32_t             mode      = EST_MAX;
+    std::shared_ptr<Mlp> mlp;
+    float                max_sigma = 0.5f;
+    uint32_t             max_sets  = 1u << 15;
+
+    CardFn     model(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep);
+    QueryStats stats(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep);
+    size_t     cached_pairs() const { return pair_rows_.size(); }
+
+private:
+    using PairKey = std::tuple<uint64_t, uint64_t, uint64_t, uint64_t>; // (uid, col) of each side, ordered
+    std::map<PairKey, uint64_t> pair_rows_;
+};
+
+} // namespace cgj

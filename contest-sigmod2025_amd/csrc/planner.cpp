@@ -149,6 +149,46 @@ Query import_plan(const cgj_plan& plan, const std::vector<uint64_t>& table_rows,
     return q;
 }
 
+Query restrict_query(const Query& q, uint64_t relset) {
+    const uint32_t n = uint32_t(q.rels.size());
+    if (relset == 0 || (n < 64 && (relset >> n) != 0)) throw std::runtime_error("relation set out of range");
+    // connected?
+    const auto adj  = q.adjacency();
+    uint64_t   seen = relset & (~relset + 1), frontier = seen;
+    while (frontier) {
+        uint64_t next = 0;
+        for (uint64_t t = frontier; t; t &= t - 1) next |= adj[__builtin_ctzll(t)];
+        frontier = next & relset & ~seen;
+        seen |= frontier;
+    }
+    if (seen != relset) throw std::runtime_error("relation set is not connected by join predicates");
+    Query                 s;
+    std::vector<int32_t>  relmap(n, -1), eqmap(q.eq_attrs.size(), -1);
+    for (uint32_t r = 0; r < n; ++r)
+        if ((relset >> r) & 1) {
+            relmap[r] = int32_t(s.rels.size());
+            QRel nr;
+            nr.table = q.rels[r].table;
+            nr.rows  = q.rels[r].rows;
+            s.rels.push_back(nr);
+        }
+    for (uint32_t a = 0; a < q.attrs.size(); ++a) {
+        const QAttr& at = q.attrs[a];
+        if (relmap[at.rel] < 0) continue;
+        if (eqmap[at.eq] < 0) {
+            eqmap[at.eq] = int32_t(s.eq_attrs.size());
+            s.eq_attrs.emplace_back();
+        }
+        const uint32_t na = uint32_t(s.attrs.size());
+        s.attrs.push_back(QAttr{uint32_t(relmap[at.rel]), at.col, at.type, uint32_t(eqmap[at.eq])});
+        s.eq_attrs[eqmap[at.eq]].push_back(na);
+        s.rels[relmap[at.rel]].attrs.push_back(na);
+        s.rels[relmap[at.rel]].eqs.push_back(uint32_t(eqmap[at.eq]));
+    }
+    for (auto& r : s.rels) std::sort(r.eqs.begin(), r.eqs.end());
+    return s;
+}
+
 CardFn max_card_model(const Query& q, const std::vector<double>& base_card) {
     (void)q;
     return [base_card](uint64_t set) {

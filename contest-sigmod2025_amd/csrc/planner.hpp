@@ -53,6 +53,10 @@ struct Query {
 Query import_plan(const cgj_plan& plan, const std::vector<uint64_t>& table_rows,
                   const std::vector<uint32_t>& table_cols);
 
+// The sub-query of a connected relation set: its relations (in order), their
+// scanned columns and the join classes among them; no output columns.
+Query restrict_query(const Query& q, uint64_t relset);
+
 // Join tree: leaves are relations; an inner node joins `build` (hashed) with
 // `probe` (streamed through the pipeline).
 struct TreeNode {

@@ -91,6 +91,12 @@ class Stats(C.Structure):
         ("ms_probe_partition", C.c_double),
         ("probe_kernel", C.c_uint32),
         ("probe_waves", C.c_uint32),
+        ("ms_plan", C.c_double),
+        ("ms_stats", C.c_double),
+        ("est_sets", C.c_uint32),
+        ("est_fallbacks", C.c_uint32),
+        ("est_stat_joins", C.c_uint32),
+        ("est_pad", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -127,6 +133,8 @@ class BatchStats(C.Structure):
 
 
 # exported symbols of include/gpujoin.h with their ctypes signatures
+_F32P = C.POINTER(C.c_float)
+
 SIGNATURES = {
     "cgj_context_create": (C.c_void_p, [C.c_int]),
     "cgj_context_destroy": (None, [C.c_void_p]),
@@ -164,7 +172,28 @@ SIGNATURES = {
     "cgj_batch_num_rows": (C.c_uint64, [C.c_void_p, C.c_uint32]),
     "cgj_batch_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(Result)]),
     "cgj_batch_free": (None, [C.c_void_p, C.c_void_p]),
+    "cgj_mlp_create": (C.c_void_p, [C.c_void_p, C.c_uint32, C.c_uint64]),
+    "cgj_mlp_num_params": (C.c_uint64, [C.c_void_p]),
+    "cgj_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, _F32P]),
+    "cgj_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, _F32P]),
+    "cgj_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, _F32P, C.c_uint32, _F32P, _F32P]),
+    "cgj_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, _F32P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                C.c_float, C.c_uint64, _F32P]),
+    "cgj_mlp_predict": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, C.c_uint32, _F32P]),
+    "cgj_mlp_predict_f32": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, C.c_uint32, _F32P]),
+    "cgj_mlp_free": (None, [C.c_void_p, C.c_void_p]),
+    "cgj_num_features": (C.c_uint32, []),
+    "cgj_set_estimator": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_float]),
+    "cgj_plan_features": (C.c_int64, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
+                                      C.POINTER(C.c_uint64), _F32P, C.c_uint64]),
+    "cgj_count_subset": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32, C.c_uint64,
+                                   C.POINTER(C.c_uint64)]),
+    "cgj_plan_joins": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
+                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_uint32,
+                                 C.POINTER(C.c_uint32)]),
 }
+
+EST_MAX, EST_INDEPENDENCE, EST_LEARNED = 0, 1, 2
 
 _lib = None
 
@@ -450,6 +479,93 @@ class Batch:
             pass
 
 
+def _f32(a, shape=None):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a, a.ctypes.data_as(_F32P)
+
+
+class Mlp:
+    """The cardinality estimator's network on the device (cgj_mlp_*): three
+    layers in -> 256 -> 256 -> 2, outputs (mu, log sigma)."""
+
+    def __init__(self, ctx: "Context", in_features: int, seed: int = 1):
+        self.ctx = ctx
+        self.in_features = in_features
+        self.handle = lib().cgj_mlp_create(ctx.handle, in_features, seed)
+        if not self.handle:
+            raise RuntimeError(ctx.error())
+        self.num_params = lib().cgj_mlp_num_params(self.handle)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.ctx.error())
+
+    def set_params(self, params):
+        a, p = _f32(params, (self.num_params,))
+        self._check(lib().cgj_mlp_set_params(self.ctx.handle, self.handle, p))
+
+    def get_params(self):
+        import numpy as np
+        out = np.empty(self.num_params, np.float32)
+        self._check(lib().cgj_mlp_get_params(self.ctx.handle, self.handle, out.ctypes.data_as(_F32P)))
+        return out
+
+    def loss_grad(self, x, y):
+        import numpy as np
+        xa, xp = _f32(x)
+        n = xa.shape[0]
+        if xa.ndim != 2 or xa.shape[1] != self.in_features:
+            raise ValueError("x must be n x in_features")
+        ya, yp = _f32(y, (n,))
+        loss = C.c_float()
+        g = np.empty(self.num_params, np.float32)
+        self._check(lib().cgj_mlp_loss_grad(self.ctx.handle, self.handle, xp, yp, n, C.byref(loss),
+                                            g.ctypes.data_as(_F32P)))
+        return loss.value, g
+
+    def train(self, x, y, epochs: int, batch: int, lr: float, seed: int = 1) -> float:
+        xa, xp = _f32(x)
+        n = xa.shape[0]
+        if xa.ndim != 2 or xa.shape[1] != self.in_features:
+            raise ValueError("x must be n x in_features")
+        ya, yp = _f32(y, (n,))
+        loss = C.c_float()
+        self._check(lib().cgj_mlp_train(self.ctx.handle, self.handle, xp, yp, n, epochs, batch, lr, seed,
+                                        C.byref(loss)))
+        return loss.value
+
+    def _predict(self, fn, x):
+        import numpy as np
+        xa, xp = _f32(x)
+        if xa.ndim != 2 or xa.shape[1] != self.in_features:
+            raise ValueError("x must be n x in_features")
+        out = np.empty((xa.shape[0], 2), np.float32)
+        self._check(fn(self.ctx.handle, self.handle, xp, xa.shape[0], out.ctypes.data_as(_F32P)))
+        return out
+
+    def predict(self, x):
+        """BF16 MFMA inference, FP32 accumulation."""
+        return self._predict(lib().cgj_mlp_predict, x)
+
+    def predict_f32(self, x):
+        """FP32 forward (the training path's)."""
+        return self._predict(lib().cgj_mlp_predict_f32, x)
+
+    def free(self):
+        if self.handle:
+            lib().cgj_mlp_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Context:
     """Mirror of Contest::build_context / execute / destroy_context."""
 
@@ -505,6 +621,62 @@ class Context:
     def batch(self, plans: Sequence[Plan], tables: Sequence[Sequence[DeviceTable]]) -> "Batch":
         """Prepares plans (each one join of two scans) for batched execution."""
         return Batch(self, plans, tables)
+
+    def mlp(self, in_features: int, seed: int = 1) -> "Mlp":
+        return Mlp(self, in_features, seed)
+
+    # -- cardinality estimation for join ordering --------------------------
+    def set_estimator(self, mode: int, mlp: Optional["Mlp"] = None, max_sigma: float = 0.5):
+        """EST_MAX (the reference's max-of-inputs rule, default),
+        EST_INDEPENDENCE, or EST_LEARNED with a network of num_features() inputs."""
+        h = mlp.handle if mlp is not None else None
+        if lib().cgj_set_estimator(self.handle, mode, h, max_sigma) != 0:
+            raise RuntimeError(self.error())
+
+    @staticmethod
+    def num_features() -> int:
+        return lib().cgj_num_features()
+
+    def _plan_args(self, plan: Plan, tables: Sequence[DeviceTable]):
+        m = _Marshal(plan, [])
+        arr = (C.c_void_p * max(len(tables), 1))(*[t.handle for t in tables])
+        return m, arr
+
+    def plan_features(self, plan: Plan, tables: Sequence[DeviceTable]):
+        """(sets, features) of every connected relation set of two or more
+        relations; relation r is the plan's r-th scan in depth-first order."""
+        import numpy as np
+        m, arr = self._plan_args(plan, tables)
+        n = lib().cgj_plan_features(self.handle, C.byref(m.plan), arr, len(tables), None, None, 0)
+        if n < 0:
+            raise RuntimeError(self.error())
+        sets = np.zeros(max(n, 1), np.uint64)
+        feats = np.zeros((max(n, 1), self.num_features()), np.float32)
+        n2 = lib().cgj_plan_features(self.handle, C.byref(m.plan), arr, len(tables),
+                                     sets.ctypes.data_as(C.POINTER(C.c_uint64)), feats.ctypes.data_as(_F32P), n)
+        if n2 != n:
+            raise RuntimeError(self.error())
+        return sets[:n], feats[:n]
+
+    def count_subset(self, plan: Plan, tables: Sequence[DeviceTable], relset: int) -> int:
+        """Exact result rows of the join of a connected relation set of the plan."""
+        m, arr = self._plan_args(plan, tables)
+        out = C.c_uint64()
+        if lib().cgj_count_subset(self.handle, C.byref(m.plan), arr, len(tables), relset, C.byref(out)) != 0:
+            raise RuntimeError(self.error())
+        return out.value
+
+    def plan_joins(self, plan: Plan, tables: Sequence[DeviceTable]):
+        """Join order under the current estimator: [(build set, probe set, estimated rows)], children first."""
+        m, arr = self._plan_args(plan, tables)
+        cap = 64
+        b = (C.c_uint64 * cap)()
+        pr = (C.c_uint64 * cap)()
+        cards = (C.c_double * cap)()
+        n = C.c_uint32()
+        if lib().cgj_plan_joins(self.handle, C.byref(m.plan), arr, len(tables), b, pr, cards, cap, C.byref(n)) != 0:
+            raise RuntimeError(self.error())
+        return [(b[i], pr[i], cards[i]) for i in range(min(n.value, cap))]
 
     def copy_probe(self, nbytes: int, reps: int = 5) -> float:
         """Mean time (ms) of a plain HBM-to-HBM copy of `nbytes` bytes."""

@@ -1,0 +1,148 @@
+"""Training data, training and evaluation of the learned cardinality estimator
+over a plan set whose plans share one pool of device tables.
+
+A training row is one connected relation set of a plan: its features
+(Context.plan_features) and the log10 of its exact cardinality
+(Context.count_subset, one count-only join pipeline per set). The network
+predicts (mu, log sigma) of that log10 cardinality.
+"""
+from __future__ import annotations
+
+import math
+import random
+from dataclasses import dataclass, field
+from typing import Dict, List, Sequence, Tuple
+
+import numpy as np
+
+from . import EST_INDEPENDENCE, EST_LEARNED, EST_MAX, Context, DeviceTable, Mlp
+
+
+@dataclass
+class SetSamples:
+    plan: int
+    sets: np.ndarray       # uint64 relation sets
+    features: np.ndarray   # n x num_features float32
+    rows: np.ndarray       # exact cardinalities (float64)
+
+
+def plan_inputs(pool_tables: Sequence[DeviceTable], pp) -> List[DeviceTable]:
+    return [pool_tables[t] for t in pp.tables]
+
+
+def collect(ctx: Context, planset, pool_tables: Sequence[DeviceTable], plan_ids: Sequence[int],
+            max_sets_per_plan: int = 48, seed: int = 7, counts: Dict = None) -> List[SetSamples]:
+    """Features and exact cardinalities of up to `max_sets_per_plan` connected
+    sets per plan (all sets of the full plan included). `counts` caches exact
+    counts by (plan, set) across calls."""
+    rng = random.Random(seed)
+    out = []
+    for pid in plan_ids:
+        pp = planset.plans[pid]
+        tabs = plan_inputs(pool_tables, pp)
+        sets, feats = ctx.plan_features(pp.plan, tabs)
+        full = (1 << len(pp.tables)) - 1
+        idx = list(range(len(sets)))
+        if len(idx) > max_sets_per_plan:
+            keep = [i for i in idx if int(sets[i]) == full]
+            rest = [i for i in idx if int(sets[i]) != full]
+            keep += rng.sample(rest, max_sets_per_plan - len(keep))
+            idx = sorted(keep)
+        rows = []
+        for i in idx:
+            key = (pid, int(sets[i]))
+            if counts is not None and key in counts:
+                c = counts[key]
+            else:
+                c = ctx.count_subset(pp.plan, tabs, int(sets[i]))
+                if counts is not None:
+                    counts[key] = c
+            rows.append(float(c))
+        out.append(SetSamples(pid, sets[idx], feats[idx], np.array(rows)))
+    return out
+
+
+def labels(rows: np.ndarray) -> np.ndarray:
+    return np.log10(np.maximum(rows, 1.0)).astype(np.float32)
+
+
+def stack(samples: Sequence[SetSamples]) -> Tuple[np.ndarray, np.ndarray]:
+    x = np.concatenate([s.features for s in samples]).astype(np.float32)
+    y = np.concatenate([labels(s.rows) for s in samples])
+    return x, y
+
+
+def train(ctx: Context, samples: Sequence[SetSamples], epochs: int = 200, batch: int = 256, lr: float = 1e-3,
+          seed: int = 1) -> Tuple[Mlp, float]:
+    x, y = stack(samples)
+    m = ctx.mlp(Context.num_features(), seed=seed)
+    loss = m.train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
+    return m, loss
+
+
+def qerror(est: np.ndarray, true: np.ndarray) -> np.ndarray:
+    e = np.maximum(np.asarray(est, dtype=np.float64), 1.0)
+    t = np.maximum(np.asarray(true, dtype=np.float64), 1.0)
+    return np.maximum(e / t, t / e)
+
+
+def estimates(samples: Sequence[SetSamples], planset, mode: int, mlp: Mlp = None,
+              max_sigma: float = 0.5) -> np.ndarray:
+    """Estimated cardinalities of the samples' sets under one model, computed
+    on the host from the features (they hold the same inputs the engine uses):
+    EST_MAX = largest relation, EST_INDEPENDENCE = feature 10 (x 8 = log10 of
+    the independence estimate), EST_LEARNED = the network (BF16 inference),
+    falling back to independence above `max_sigma`."""
+    x = np.concatenate([s.features for s in samples])
+    indep = 10.0 ** (x[:, 10].astype(np.float64) * 8.0)
+    if mode == EST_MAX:
+        return 10.0 ** (x[:, 11].astype(np.float64) * 8.0)
+    if mode == EST_INDEPENDENCE:
+        return indep
+    out = mlp.predict(x)
+    mu, s = out[:, 0].astype(np.float64), out[:, 1].astype(np.float64)
+    learned = 10.0 ** np.minimum(mu, 300.0)
+    return np.where(np.exp(s) <= max_sigma, learned, indep)
+
+
+def true_cost(ctx: Context, planset, pool_tables, pid: int, counts: Dict) -> float:
+    """Cost of the join order the context's current estimator picks, priced
+    with exact cardinalities: per join, result rows + 2 x build rows (the
+    planner's own cost function, engine/query/QueryGraph.cpp:46-50 restated
+    for a hash join whose table is written and read once more)."""
+    pp = planset.plans[pid]
+    tabs = plan_inputs(pool_tables, pp)
+
+    def card(s):
+        if s & (s - 1) == 0:
+            return float(tabs[s.bit_length() - 1].num_rows)
+        key = (pid, s)
+        if key not in counts:
+            counts[key] = ctx.count_subset(pp.plan, tabs, s)
+        return float(counts[key])
+
+    cost = 0.0
+    for b, p, _ in ctx.plan_joins(pp.plan, tabs):
+        cost += card(b | p) + 2.0 * card(b)
+    return cost
+
+
+@dataclass
+class Evaluation:
+    qerror_median: Dict[str, float] = field(default_factory=dict)
+    qerror_p90: Dict[str, float] = field(default_factory=dict)
+    cost: Dict[str, float] = field(default_factory=dict)
+
+
+def evaluate(ctx: Context, planset, pool_tables, samples: Sequence[SetSamples], mlp: Mlp, counts: Dict,
+             max_sigma: float = 0.5) -> Evaluation:
+    ev = Evaluation()
+    true = np.concatenate([s.rows for s in samples])
+    for name, mode in (("max", EST_MAX), ("independence", EST_INDEPENDENCE), ("learned", EST_LEARNED)):
+        q = qerror(estimates(samples, planset, mode, mlp, max_sigma), true)
+        ev.qerror_median[name] = float(np.median(q))
+        ev.qerror_p90[name] = float(np.percentile(q, 90))
+        ctx.set_estimator(mode, mlp if mode == EST_LEARNED else None, max_sigma)
+        ev.cost[name] = sum(true_cost(ctx, planset, pool_tables, s.plan, counts) for s in samples)
+    ctx.set_estimator(EST_MAX)
+    return ev

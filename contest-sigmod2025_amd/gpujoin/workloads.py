@@ -172,5 +172,142 @@ def c4(scale: float = 1.0, njobs: int = 256) -> BatchWorkload:
                                      f"({total_b} rows), probe 10x build ({total_p} rows), 5% misses")
 
 
+@dataclass
+class PoolPlan:
+    plan: Plan
+    tables: List[int]  # pool table of each plan input (scan base_table_id i -> pool table tables[i])
+    shape: str         # "chain", "star" or "tree"
+    edges: List[tuple] = None  # (source member, FK column, target member): source.FK = target.key
+
+    def subplan(self, relset: int) -> "PoolPlan":
+        """The plan of a connected set of members (bit k = member k, the plan's
+        k-th scan): the same joins restricted to those members. Members join in
+        insertion order, and every member's parent edge comes from an earlier
+        member, so a connected set keeps each non-first member's parent."""
+        keep = [k for k in range(len(self.tables)) if (relset >> k) & 1]
+        idx = {k: i for i, k in enumerate(keep)}
+        edges = [(idx[a], c, idx[b]) for a, c, b in self.edges if a in idx and b in idx]
+        if len(edges) != len(keep) - 1:
+            raise ValueError("relation set is not connected")
+        return PoolPlan(_c5_plan(len(keep), edges), [self.tables[k] for k in keep], self.shape, edges)
+
+
+def _c5_plan(nmembers: int, edges) -> Plan:
+    """Left-deep contest plan over members 0..n-1 joined by `edges` in order;
+    output: member 0's key and the last joined member's key (member 0's key
+    alone for a single member)."""
+    p = Plan()
+    # columns each member scans: its key (joined or output) and its FK columns used
+    scans = {k: [0] for k in range(nmembers)}
+    for sp, c, _ in edges:
+        scans[sp].append(c)
+    for k in scans:
+        scans[k] = sorted(set(scans[k]))
+    nodes = [p.new_scan_node(k, [(c, INT32) for c in scans[k]]) for k in range(nmembers)]
+    pos = {(0, c): i for i, c in enumerate(scans[0])}
+    width = len(scans[0])
+    cur = nodes[0]
+    for e, (sp, c, tp) in enumerate(edges):
+        right_cols = scans[tp]
+        la = pos[(sp, c)]
+        ra = right_cols.index(0)
+        if e == len(edges) - 1:
+            out = [(pos[(0, 0)], INT32), (width + ra, INT32)]
+        else:
+            out = [(i, INT32) for i in range(width + len(right_cols))]
+        cur = p.new_join_node(False, cur, nodes[tp], la, ra, out)
+        for i, c2 in enumerate(right_cols):
+            pos[(tp, c2)] = width + i
+        width += len(right_cols)
+    p.root = cur
+    return p
+
+
+@dataclass
+class PlanSet:
+    name: str
+    seed: int
+    pool: List[TableSpec]  # shared by all plans
+    plans: List[PoolPlan]
+    description: str
+
+
+C5_FKS = 11  # foreign-key columns per pool table (a star of 12 tables needs 11)
+
+
+def c5(scale: float = 1.0, nplans: int = 2000, pool: int = 32, seed: int = 0x5EED0005) -> PlanSet:
+    """`nplans` seeded join plans of 3-12 tables (chains, stars and mixed
+    trees) over one shared pool of `pool` tables. Pool table i has
+    log-uniform [10K, 40M] x `scale` rows; column 0 is its key: the table holds
+    a log-uniform [1e-4, 1] fraction (its pre-filter selectivity) of a key
+    domain, as unique INT32 keys. Columns 1..11 are INT32 foreign keys into
+    the key domains of other pool tables, Zipf-distributed over key rank with
+    s ~ U[0, 1.5]. Every join of a plan is foreign key = key of a table not yet
+    in the plan, so a plan's result has at most as many rows as its first
+    table. Plans output the first table's key and the last table's key."""
+    rng = random.Random(seed)
+    specs = []
+    refs = []  # refs[i][c] = pool table that FK column c+1 of table i references
+    lo, hi = math.log(10_000), math.log(40_000_000)
+    domains = []
+    for i in range(pool):
+        n = max(int(round(math.exp(rng.uniform(lo, hi)) * scale)), 16)
+        sel = math.exp(rng.uniform(math.log(1e-4), 0.0))
+        d = min(max(int(math.ceil(n / sel)), n), 1 << 30)
+        domains.append(d)
+        specs.append((n, d))
+    for i in range(pool):
+        others = [j for j in range(pool) if j != i]
+        refs.append([rng.choice(others) for _ in range(C5_FKS)])
+    for i, (n, d) in enumerate(specs):
+        cols = [GenColumn(type=INT32, kind=GEN_PERM_KEY, stream=5000 + 16 * i, domain=d)]
+        for c in range(C5_FKS):
+            cols.append(GenColumn(type=INT32, kind=GEN_FK_ZIPF, stream=5001 + 16 * i + c, domain=domains[refs[i][c]],
+                                  zipf_s=rng.uniform(0.0, 1.5)))
+        specs[i] = TableSpec(f"pool{i}", n, cols)
+    pool_specs = specs
+
+    def one_plan(prng):
+        ntab = prng.randint(3, 12)
+        shape = prng.choice(["chain", "star", "tree"])
+        for _ in range(100):
+            start = prng.randrange(pool)
+            members = [start]
+            edges = []  # (source position, fk column, target position)
+            used = set()
+            ok = True
+            while len(members) < ntab:
+                if shape == "chain":
+                    srcs = [len(members) - 1]
+                elif shape == "star":
+                    srcs = [0]
+                else:
+                    srcs = list(range(len(members)))
+                cands = [(sp, c) for sp in srcs for c in range(1, C5_FKS + 1)
+                         if (sp, c) not in used and refs[members[sp]][c - 1] not in members]
+                if not cands:
+                    ok = False
+                    break
+                sp, c = prng.choice(cands)
+                used.add((sp, c))
+                members.append(refs[members[sp]][c - 1])
+                edges.append((sp, c, len(members) - 1))
+            if ok:
+                return members, edges, shape
+        raise RuntimeError("could not draw a plan from the pool")
+
+    prng = random.Random(seed + 1)
+    plans = []
+    for _ in range(nplans):
+        members, edges, shape = one_plan(prng)
+        plans.append(PoolPlan(_c5_plan(len(members), edges), members, shape, edges))
+    total = sum(t.rows for t in pool_specs)
+    return PlanSet("c5", seed, pool_specs, plans,
+                   f"{nplans} plans of 3-12 tables (chain/star/tree) over a pool of {pool} tables "
+                   f"({total} rows, log-uniform [10K, 40M] x {scale:g}), Zipf FKs s~U[0,1.5], "
+                   f"pre-filter selectivity log-uniform [1e-4, 1]")
+
+
 WORKLOADS = {"c1": c1, "c2": c2, "c3": c3}
 BATCH_WORKLOADS = {"c4": c4}
+PLAN_SETS = {"c5": c5}

@@ -116,6 +116,13 @@ typedef struct {
     double   ms_probe_partition;/* HIP-event time of the probe side's radix partitioning */
     uint32_t probe_kernel;      /* output pipeline ran on: 0 the general kernel, 1 the lean one-probe kernel */
     uint32_t probe_waves;       /* waves requested for the output pipeline */
+    double   ms_plan;           /* host time of cardinality estimation + join ordering,
+                                   statistics counts excluded */
+    double   ms_stats;          /* time of the 2-way counts run for the statistics catalog */
+    uint32_t est_sets;          /* connected relation sets estimated by the network */
+    uint32_t est_fallbacks;     /* ... of them, by the independence rule (low confidence) */
+    uint32_t est_stat_joins;    /* 2-way counts run (pairs not yet in the catalog) */
+    uint32_t est_pad;
 } cgj_stats;
 
 /* Execution options of a context (0 = success, -1 = unknown name):
@@ -216,6 +223,65 @@ int        cgj_batch_info(const cgj_batch* b, cgj_batch_stats* out);
 uint64_t   cgj_batch_num_rows(const cgj_batch* b, uint32_t plan);
 int        cgj_batch_download(cgj_context* ctx, cgj_batch* b, uint32_t plan, cgj_result* out);
 void       cgj_batch_free(cgj_context* ctx, cgj_batch* b);
+
+/* ---- learned cardinality estimator ----
+ * A 3-layer MLP (in features -> 256 -> 256 -> 2, ReLU) that predicts the
+ * log10 cardinality of a connected set of relations and the log of its
+ * standard deviation. It stands in for the reference's sampling estimate
+ * (engine/query/QueryPlan.cpp:66-159, 719-746; QueryGraph.cpp:53-68) in join
+ * ordering. Training runs on the GPU (FP32 forward, backward and Adam);
+ * inference takes BF16 inputs with FP32 accumulation on the matrix cores.
+ * Parameters are one flat FP32 array: W1 (in x 256), b1, W2 (256 x 256), b2,
+ * W3 (256 x 2), b3; row-major, input index first. Host arrays throughout. */
+typedef struct cgj_mlp cgj_mlp;
+
+cgj_mlp* cgj_mlp_create(cgj_context* ctx, uint32_t in_features, uint64_t seed);
+uint64_t cgj_mlp_num_params(const cgj_mlp* m);
+int      cgj_mlp_set_params(cgj_context* ctx, cgj_mlp* m, const float* params);
+int      cgj_mlp_get_params(cgj_context* ctx, const cgj_mlp* m, float* params);
+/* Mean heteroscedastic Gaussian loss, 0.5 (y - mu)^2 exp(-2 s) + s, over the n
+ * rows of x (n x in) against y, and its gradient (num_params floats). */
+int      cgj_mlp_loss_grad(cgj_context* ctx, cgj_mlp* m, const float* x, const float* y, uint32_t n, float* loss,
+                           float* grads);
+int      cgj_mlp_train(cgj_context* ctx, cgj_mlp* m, const float* x, const float* y, uint32_t n, uint32_t epochs,
+                       uint32_t batch, float lr, uint64_t seed, float* final_loss);
+/* out: n x 2 (mu, log sigma). _predict is the BF16 MFMA path, _predict_f32 the FP32 forward. */
+int      cgj_mlp_predict(cgj_context* ctx, cgj_mlp* m, const float* x, uint32_t n, float* out);
+int      cgj_mlp_predict_f32(cgj_context* ctx, cgj_mlp* m, const float* x, uint32_t n, float* out);
+void     cgj_mlp_free(cgj_context* ctx, cgj_mlp* m);
+
+/* ---- cardinality estimation for join ordering ----
+ * The reference orders joins with DPccp under the rule "a join is as large as
+ * its largest input" (engine/query/QueryGraph.cpp:53-68, fed by the sampled
+ * base cardinalities of engine/query/QueryPlan.cpp:66-159, 719-746): mode
+ * CGJ_EST_MAX, the default. CGJ_EST_INDEPENDENCE multiplies relation sizes by
+ * exact pairwise join selectivities; CGJ_EST_LEARNED asks the network (which
+ * must have cgj_num_features() inputs) for the log10 cardinality and its
+ * spread of every connected relation set in one batched inference and falls
+ * back to the independence rule where the spread exceeds `max_sigma` (log10
+ * units). Pairwise selectivities come from exact 2-way counts on the device,
+ * cached per (table, column) pair for the context's lifetime. The estimator
+ * only changes the join order: results are identical in every mode. The
+ * context keeps a reference to the network, so its handle may be freed. */
+enum { CGJ_EST_MAX = 0, CGJ_EST_INDEPENDENCE = 1, CGJ_EST_LEARNED = 2 };
+uint32_t cgj_num_features(void);
+int      cgj_set_estimator(cgj_context* ctx, uint32_t mode, cgj_mlp* mlp, float max_sigma);
+/* Relations of a plan are numbered in its depth-first scan order (left
+ * input first). Writes, for up to `cap` connected sets of two or more
+ * relations, the set (bit r = relation r) and its cgj_num_features() features;
+ * returns the number of such sets, or -1 on error. */
+int64_t  cgj_plan_features(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                           uint64_t* sets, float* features, uint64_t cap);
+/* Exact result rows of the join of a connected relation set of the plan. */
+int      cgj_count_subset(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                          uint64_t relset, uint64_t* rows);
+/* The join order the planner picks under the current estimator: per join
+ * (children first), the relation sets of the build and probe sides and the
+ * estimated result cardinality. *num_joins = number of joins (at most `cap`
+ * are written). */
+int      cgj_plan_joins(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                        uint64_t* build_sets, uint64_t* probe_sets, double* cards, uint32_t cap,
+                        uint32_t* num_joins);
 
 /* ---- operator-level entry points ---- */
 
