@@ -71,6 +71,103 @@ def load_traffic(name):
     return None
 
 
+def run_c5(args):
+    """Plan set (3-12-table chains/stars/trees over a shared table pool):
+    trains the estimator on the first `train_plans` plans' connected sets
+    (exact counts), then executes the next `plans` plans device-resident, once
+    planned with the reference's max-of-inputs rule and once with the learned
+    estimator. A step is one pass over the evaluated plans; planning
+    (estimation + DP) is inside the timed region, collecting pairwise join
+    statistics is timed separately (it is done once per table pair of the pool)."""
+    import numpy as np
+    import gpujoin as gj
+    from gpujoin import estimation as est
+    from gpujoin import workloads
+
+    ps = workloads.c5(args.scale, nplans=args.train_plans + args.plans)
+    ctx = gj.Context(0)
+    t0 = time.perf_counter()
+    tabs = [ctx.generate(ps.seed, t.rows, t.columns) for t in ps.pool]
+    log(f"c5 pool: {len(tabs)} tables, {sum(t.rows for t in ps.pool)} rows in {time.perf_counter() - t0:.1f} s")
+    counts = {}
+    t0 = time.perf_counter()
+    train_s = []
+    for lo in range(0, args.train_plans, 25):
+        train_s += est.collect(ctx, ps, tabs, range(lo, min(lo + 25, args.train_plans)), counts=counts)
+        log(f"training sets: {sum(len(s.rows) for s in train_s)} from {min(lo + 25, args.train_plans)} plans")
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mlp, loss = est.train(ctx, train_s, epochs=args.epochs)
+    t_train = time.perf_counter() - t0
+    log(f"trained in {t_train:.1f} s, loss {loss:.4f}")
+    eval_ids = list(range(args.train_plans, args.train_plans + args.plans))
+    held = est.collect(ctx, ps, tabs, eval_ids[:min(100, len(eval_ids))], counts=counts, seed=8)
+    true = np.concatenate([s.rows for s in held])
+    q = {}
+    for name, mode in (("max", gj.EST_MAX), ("independence", gj.EST_INDEPENDENCE), ("learned", gj.EST_LEARNED)):
+        qe = est.qerror(est.estimates(held, ps, mode, mlp), true)
+        q[name] = {"median": round(float(np.median(qe)), 3), "p90": round(float(np.percentile(qe, 90)), 3)}
+    log(f"held-out q-error: {q}")
+
+    results = {}
+    for name, mode in (("max", gj.EST_MAX), ("learned", gj.EST_LEARNED)):
+        ctx.set_estimator(mode, mlp if mode == gj.EST_LEARNED else None)
+        # untimed: fill the statistics catalog with every table pair the
+        # evaluated plans join (learned mode), then warm up on a few plans
+        t0 = time.perf_counter()
+        if mode == gj.EST_LEARNED:
+            for pid in eval_ids:
+                ctx.plan_joins(ps.plans[pid].plan, est.plan_inputs(tabs, ps.plans[pid]))
+        s_stats = time.perf_counter() - t0
+        for pid in eval_ids[:args.warmup * 20]:
+            ctx.execute_device(ps.plans[pid].plan, est.plan_inputs(tabs, ps.plans[pid])).free()
+        t_rows = 0
+        ms_plan = 0.0
+        per_step = []
+        for step in range(args.steps):
+            t_start = time.perf_counter()
+            for k, pid in enumerate(eval_ids):
+                pp = ps.plans[pid]
+                r = ctx.execute_device(pp.plan, est.plan_inputs(tabs, pp))
+                st = ctx.stats()
+                ms_plan += st["ms_plan"]
+                t_rows += r.num_rows
+                r.free()
+                if k % 250 == 249:
+                    log(f"{name} step {step}: {k + 1} plans")
+            per_step.append(time.perf_counter() - t_start)
+        results[name] = {"s_per_step": min(per_step), "ms_plan_per_step": ms_plan / args.steps,
+                         "rows_per_step": t_rows // args.steps, "s_statistics_untimed": round(s_stats, 2)}
+        log(f"{name}: {results[name]}")
+    ctx.set_estimator(gj.EST_MAX)
+    s_learned = results["learned"]["s_per_step"]
+    line = {
+        "metric": "c5 plan set: plans/s end-to-end (planning + device-resident execution), learned estimator",
+        "value": round(args.plans / s_learned, 2),
+        "unit": "plans/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s_learned * 1000.0, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 joins; fp32/bf16 estimator",
+        "data": "synthetic",
+        "config": {"workload": f"c5: {ps.description}; evaluated plans {args.plans}, training plans {args.train_plans}"},
+        "speedup_vs_max_rule": round(results["max"]["s_per_step"] / s_learned, 3),
+        "max_rule": results["max"],
+        "learned": results["learned"],
+        "qerror_held_out": q,
+        "training": {"sets": int(sum(len(s.rows) for s in train_s)), "s_collect": round(t_collect, 1),
+                     "s_train": round(t_train, 1), "epochs": args.epochs, "final_loss": round(loss, 4)},
+    }
+    print(json.dumps(line), flush=True)
+    for t in tabs:
+        t.free()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,17 +176,28 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="execution option (cgj_set_option), repeatable")
+    ap.add_argument("--plans", type=int, default=2000, help="c5: evaluated plans")
+    ap.add_argument("--train-plans", type=int, default=150, help="c5: training plans")
+    ap.add_argument("--epochs", type=int, default=150, help="c5: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the join pipeline on a single MI355X "
               f"(--gpus {args.gpus} is not supported; sharding is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c5":
+        return run_c5(args)
+
     import gpujoin as gj
     from gpujoin import workloads
 
     wl = workloads.WORKLOADS[args.workload](args.scale)
     ctx = gj.Context(0)
+    for o in args.option:
+        name, value = o.split("=", 1)
+        ctx.set_option(name, float(value))
     t0 = time.perf_counter()
     tables = []
     for t in wl.tables:

@@ -259,6 +259,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "lean_kernel") options.lean_kernel = v != 0;
     else if (name == "build_regions_log") options.build_regions_log = v >= 1 && v <= 14 ? uint32_t(v) : 14;
     else if (name == "lean_waves_per_cu") options.lean_waves_per_cu = v >= 4 && v <= 64 ? uint32_t(v) : 32;
+    else if (name == "aligned_pairs") options.aligned_pairs = v != 0;
     else return false;
     return true;
 }
@@ -483,7 +484,7 @@ uint64_t Executor::launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& 
     CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
     if (a.nmorsels) {
         if (use_lean(a, nslots))
-            CGJ_HIP(launch_probe1(a, nslots, uint32_t(waves), ctx_.stream()));
+            CGJ_HIP(launch_probe1(a, nslots, uint32_t(waves), ctx_.options.aligned_pairs, ctx_.stream()));
         else
             CGJ_HIP(launch_pipeline(a, nslots, uint32_t(waves), ctx_.stream()));
     }

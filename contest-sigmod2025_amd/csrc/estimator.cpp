@@ -228,6 +228,10 @@ double log10_independence(const Query& q, const QueryStats& st, uint64_t relset)
     return l;
 }
 
+double independence_base(const Query& q, const QueryStats& st, uint64_t relset) {
+    return std::max(log10_independence(q, st, relset), 0.0);
+}
+
 void featurize(const Query& q, const QueryStats& st, uint64_t relset, float* f) {
     std::fill(f, f + kFeatures, 0.f);
     std::vector<uint32_t> rels;
@@ -402,12 +406,14 @@ CardFn Estimator::model(Context& ctx, const Query& q, const std::vector<const De
         std::vector<float> x(multi.size() * kFeatures), y(multi.size() * kMlpOut);
         for (size_t i = 0; i < multi.size(); ++i) featurize(q, *st, multi[i], &x[i * kFeatures]);
         mlp->predict(x.data(), uint32_t(multi.size()), y.data());
-        // sigma = exp(s) in log10 units
+        // The network predicts a correction (mu) to the independence estimate
+        // in log10 units, and its spread sigma = exp(s).
         const float max_s = std::log(std::max(max_sigma, 1e-6f));
         for (size_t i = 0; i < multi.size(); ++i) {
-            double l = y[2 * i];
+            const double base_l = independence_base(q, *st, multi[i]);
+            double       l      = base_l + y[2 * i];
             if (!(y[2 * i + 1] <= max_s) || !std::isfinite(l)) {
-                l = log10_independence(q, *st, multi[i]);
+                l = base_l;
                 ++rep.fallbacks;
             }
             (*est)[multi[i]] = std::pow(10.0, std::min(l, 300.0));

@@ -2,11 +2,11 @@
 //
 // The reference estimates a join's cardinality as the largest of its inputs
 // after sampling-based filter selectivities (engine/query/QueryPlan.cpp:66-159,
-// 719-746; engine/query/QueryGraph.cpp:53-68). Here a small MLP predicts the
-// log10 cardinality of a connected set of relations from features of the set
-// (relation sizes, exact pairwise join selectivities measured on the device,
-// shape), together with a confidence; below the confidence bar the planner
-// falls back to the independence estimate over the same measured pairs.
+// 719-746; engine/query/QueryGraph.cpp:53-68). Here a small MLP corrects the
+// independence estimate of a connected set of relations (log10 residual) from
+// features of the set (relation sizes, exact pairwise join selectivities
+// measured on the device, shape), together with a confidence; below the
+// confidence bar the planner keeps the uncorrected independence estimate.
 // Estimates only steer the join order, so results are identical under every
 // estimator.
 #pragma once
@@ -86,6 +86,10 @@ void featurize(const Query& q, const QueryStats& st, uint64_t relset, float* out
 // sizes and of the selectivities of the active edges.
 double log10_independence(const Query& q, const QueryStats& st, uint64_t relset);
 
+// The learned estimate's base: the independence estimate in log10 units,
+// at least 0 (one row). The network predicts log10 |set| minus this base.
+double independence_base(const Query& q, const QueryStats& st, uint64_t relset);
+
 // Connected relation sets of the query graph, each once (EnumerateCsg of
 // Moerkotte & Neumann, VLDB 2006), singletons included; false when there are
 // more than `limit`.
@@ -104,10 +108,10 @@ struct EstimateReport {
 // Cardinality model of the join ordering.
 //   EST_MAX          the reference's rule: a join is as large as its largest input
 //   EST_INDEPENDENCE relation sizes x exact pairwise selectivities
-//   EST_LEARNED      the MLP's log10 estimate of every connected set, batched in
-//                    one BF16 MFMA inference; sets whose predicted spread
-//                    (sigma, log10 units) exceeds `max_sigma` fall back to the
-//                    independence rule
+//   EST_LEARNED      the independence estimate corrected by the MLP's log10
+//                    residual for every connected set, batched in one BF16 MFMA
+//                    inference; sets whose predicted spread (sigma, log10
+//                    units) exceeds `max_sigma` keep the independence estimate
 // Pairwise selectivities come from exact 2-way counts on the device, cached
 // per (table, column) pair for the context's lifetime: tables of a shared
 // pool are joined by many plans.

@@ -1185,29 +1185,33 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                 // halve the dependent hash-table round trips.
                 bool     have = false, searching = live;
                 uint64_t hslot = 0, k1 = 0;
+                const uint64_t  odd   = a.aligned_pairs ? 1 : 0; // uniform
                 while (__ballot(searching)) {
                     if (searching) {
-                        const uint64_t q1 = (q & ~rmask) | ((q + 1) & rmask);
-                        const uint64_t q2 = (q1 & ~rmask) | ((q1 + 1) & rmask);
+                        // slots qa (ignored when `skip`) and qb, then continue at qn
+                        const uint64_t qa   = q & ~odd;
+                        const bool     skip = qa != q;
+                        const uint64_t qb   = (qa & ~rmask) | ((qa + 1) & rmask);
+                        const uint64_t qn   = (qb & ~rmask) | ((qb + 1) & rmask);
                         uint64_t       ka, wa, kb, wb;
-                        ht_entry2(ht, words, q, ka, wa);
-                        ht_entry2(ht, words, q1, kb, wb);
-                        if (ka == key) {
+                        ht_entry2(ht, words, qa, ka, wa);
+                        ht_entry2(ht, words, qb, kb, wb);
+                        if (!skip && ka == key) {
                             have  = true;
-                            hslot = q;
+                            hslot = qa;
                             k1    = wa;
-                            q     = q1;
-                        } else if (ka == kEmpty) {
+                            q     = qb;
+                        } else if (!skip && ka == kEmpty) {
                             live = false;
                         } else if (kb == key) {
                             have  = true;
-                            hslot = q1;
+                            hslot = qb;
                             k1    = wb;
-                            q     = q2;
+                            q     = qn;
                         } else if (kb == kEmpty) {
                             live = false;
                         } else {
-                            q = q2;
+                            q = qn;
                         }
                         searching = live && !have;
                     }
@@ -1480,7 +1484,7 @@ bool probe1_applicable(const PipeArgs& a, uint32_t slots) {
     return true;
 }
 
-hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s) {
+hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, bool aligned_pairs, hipStream_t s) {
     uint32_t blocks = (grid_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks == 0) blocks = 1;
     if (!probe1_applicable(a, slots)) return hipErrorInvalidValue;
@@ -1524,6 +1528,7 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
     l.dir        = a.dir;
     l.counters   = a.counters;
     l.page_chunk = kPageChunk;
+    l.aligned_pairs = aligned_pairs && P.words <= 2 ? 1u : 0u;
     // With the dynamic morsel schedule a grid larger than one resident wave
     // set only adds a tail: cap it at what the kernel's occupancy fits.
     static const void* const kernels[kLeanOut + 1] = {

@@ -127,7 +127,10 @@ struct LeanArgs {
     uint32_t*       dir;
     PipeCounters*   counters;
     uint32_t        page_chunk; // pool pages reserved per atomic (1..kPageChunk)
-    uint32_t        pad;
+    // Probe steps read the 2-slot group {q & ~1, q | 1} holding the next slot q
+    // (slot q & ~1 ignored when q is odd) instead of {q, q + 1}: with 16-byte
+    // entries a step then never spans two 64-byte lines. Entries of <= 2 words.
+    uint32_t        aligned_pairs;
 };
 
 // ---- batched joins: one persistent launch over a device work queue ----
@@ -229,7 +232,7 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
 hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s);
 // Lean variant for one probe without residual predicates and a count or
 // fixed-width page target (<= 4 scanned, <= 4 output columns, <= 8 slots).
-hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s);
+hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, bool aligned_pairs, hipStream_t s);
 bool       probe1_applicable(const PipeArgs& a, uint32_t slots);
 uint32_t   pipeline_waves_per_block();
 

@@ -4,7 +4,10 @@ over a plan set whose plans share one pool of device tables.
 A training row is one connected relation set of a plan: its features
 (Context.plan_features) and the log10 of its exact cardinality
 (Context.count_subset, one count-only join pipeline per set). The network
-predicts (mu, log sigma) of that log10 cardinality.
+predicts (mu, log sigma) of the residual of that log10 cardinality over the
+independence estimate (feature 10 x 8, at least 0): the engine adds mu to
+the independence estimate, and keeps the independence estimate where sigma
+exceeds the confidence bar.
 """
 from __future__ import annotations
 
@@ -62,13 +65,18 @@ def collect(ctx: Context, planset, pool_tables: Sequence[DeviceTable], plan_ids:
     return out
 
 
-def labels(rows: np.ndarray) -> np.ndarray:
-    return np.log10(np.maximum(rows, 1.0)).astype(np.float32)
+def base_log10(features: np.ndarray) -> np.ndarray:
+    """The independence estimate in log10 units, at least 0 (the engine's base)."""
+    return np.maximum(features[:, 10].astype(np.float64) * 8.0, 0.0)
+
+
+def labels(rows: np.ndarray, features: np.ndarray) -> np.ndarray:
+    return (np.log10(np.maximum(rows, 1.0)) - base_log10(features)).astype(np.float32)
 
 
 def stack(samples: Sequence[SetSamples]) -> Tuple[np.ndarray, np.ndarray]:
     x = np.concatenate([s.features for s in samples]).astype(np.float32)
-    y = np.concatenate([labels(s.rows) for s in samples])
+    y = np.concatenate([labels(s.rows, s.features) for s in samples])
     return x, y
 
 
@@ -91,8 +99,9 @@ def estimates(samples: Sequence[SetSamples], planset, mode: int, mlp: Mlp = None
     """Estimated cardinalities of the samples' sets under one model, computed
     on the host from the features (they hold the same inputs the engine uses):
     EST_MAX = largest relation, EST_INDEPENDENCE = feature 10 (x 8 = log10 of
-    the independence estimate), EST_LEARNED = the network (BF16 inference),
-    falling back to independence above `max_sigma`."""
+    the independence estimate), EST_LEARNED = that estimate corrected by the
+    network's residual (BF16 inference), kept uncorrected where the network's
+    sigma exceeds `max_sigma`."""
     x = np.concatenate([s.features for s in samples])
     indep = 10.0 ** (x[:, 10].astype(np.float64) * 8.0)
     if mode == EST_MAX:
@@ -101,8 +110,9 @@ def estimates(samples: Sequence[SetSamples], planset, mode: int, mlp: Mlp = None
         return indep
     out = mlp.predict(x)
     mu, s = out[:, 0].astype(np.float64), out[:, 1].astype(np.float64)
-    learned = 10.0 ** np.minimum(mu, 300.0)
-    return np.where(np.exp(s) <= max_sigma, learned, indep)
+    base = base_log10(x)
+    learned = 10.0 ** np.minimum(base + mu, 300.0)
+    return np.where(np.exp(s) <= max_sigma, learned, 10.0 ** base)
 
 
 def true_cost(ctx: Context, planset, pool_tables, pid: int, counts: Dict) -> float:

@@ -132,7 +132,9 @@ typedef struct {
  *   "bloom_bits": bloom bits per key (-1 = automatic, 0 = none);
  *   "partitioned_build": radix-partitioned LDS build (-1 automatic, 0 off, 1 on);
  *   "lean_kernel": 1 (default) runs one-probe pipelines on the lean kernel, 0 on the general one;
- *   "lean_waves_per_cu": grid of the lean kernel in waves per compute unit (4..64, default 32). */
+ *   "lean_waves_per_cu": grid of the lean kernel in waves per compute unit (4..64, default 32);
+ *   "aligned_pairs": the lean kernel's probe steps read 2-slot groups aligned to two
+ *       slots (1) or the next two slots (0, default). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
@@ -255,11 +257,11 @@ void     cgj_mlp_free(cgj_context* ctx, cgj_mlp* m);
  * its largest input" (engine/query/QueryGraph.cpp:53-68, fed by the sampled
  * base cardinalities of engine/query/QueryPlan.cpp:66-159, 719-746): mode
  * CGJ_EST_MAX, the default. CGJ_EST_INDEPENDENCE multiplies relation sizes by
- * exact pairwise join selectivities; CGJ_EST_LEARNED asks the network (which
- * must have cgj_num_features() inputs) for the log10 cardinality and its
- * spread of every connected relation set in one batched inference and falls
- * back to the independence rule where the spread exceeds `max_sigma` (log10
- * units). Pairwise selectivities come from exact 2-way counts on the device,
+ * exact pairwise join selectivities; CGJ_EST_LEARNED adds the network's output
+ * mu (a log10 correction; the network must have cgj_num_features() inputs) to
+ * the independence estimate (log10, at least 0) of every connected relation
+ * set, all sets in one batched inference, and keeps the independence estimate
+ * where the predicted spread sigma exceeds `max_sigma` (log10 units). Pairwise selectivities come from exact 2-way counts on the device,
  * cached per (table, column) pair for the context's lifetime. The estimator
  * only changes the join order: results are identical in every mode. The
  * context keeps a reference to the network, so its handle may be freed. */

@@ -269,6 +269,25 @@ def test_lean_kernel_shapes(ctx, seed):
     assert ctx.count_device(plan, tables) == len(exp)
 
 
+@pytest.mark.parametrize("seed", range(0, 18, 2))
+def test_lean_kernel_aligned_pair_probes(ctx, seed):
+    """Probe steps over 2-slot groups aligned to two slots (the slot before an
+    odd start slot is read but never decides the lookup): the oracle's rows."""
+    plan = _lean_shape_plan(seed)
+    say(f"lean aligned pairs seed={seed}")
+    types = [t for _, t in plan.nodes[plan.root].output_attrs]
+    exp = check_table(oracle.execute(plan), types)
+    ctx.set_option("aligned_pairs", 1)
+    try:
+        got = check_table(ctx.execute(plan), types)
+        tables = [ctx.upload(t) for t in plan.inputs]
+        count = ctx.count_device(plan, tables)
+    finally:
+        ctx.set_option("aligned_pairs", 0)
+    assert got == exp
+    assert count == len(exp)
+
+
 def test_lean_kernel_all_null_int32_rows_cap(ctx):
     """An all-NULL INT32 output column costs one bit per row: pages close at the
     4096-row bitmap capacity instead of the bit budget."""
