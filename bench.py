@@ -91,17 +91,29 @@ def run_c5(args):
     log(f"c5 pool: {len(tabs)} tables, {sum(t.rows for t in ps.pool)} rows in {time.perf_counter() - t0:.1f} s")
     counts = {}
     t0 = time.perf_counter()
-    train_s = []
-    for lo in range(0, args.train_plans, 25):
-        train_s += est.collect(ctx, ps, tabs, range(lo, min(lo + 25, args.train_plans)), counts=counts)
-        log(f"training sets: {sum(len(s.rows) for s in train_s)} from {min(lo + 25, args.train_plans)} plans")
+
+    def progress(what):
+        last = [time.perf_counter()]
+
+        def f(done):
+            if time.perf_counter() - last[0] > 20.0:
+                last[0] = time.perf_counter()
+                log(f"{what}: {done} plans, {time.perf_counter() - t0:.0f} s")
+        return f
+
+    train_s = est.collect(ctx, ps, tabs, range(args.train_plans), counts=counts, progress=progress("training sets"))
     t_collect = time.perf_counter() - t0
+    log(f"training sets: {sum(len(s.rows) for s in train_s)} from {args.train_plans} plans in {t_collect:.1f} s")
     t0 = time.perf_counter()
     mlp, loss = est.train(ctx, train_s, epochs=args.epochs)
     t_train = time.perf_counter() - t0
     log(f"trained in {t_train:.1f} s, loss {loss:.4f}")
     eval_ids = list(range(args.train_plans, args.train_plans + args.plans))
-    held = est.collect(ctx, ps, tabs, eval_ids[:min(100, len(eval_ids))], counts=counts, seed=8)
+    held = est.collect(ctx, ps, tabs, eval_ids[:min(100, len(eval_ids))], counts=counts, seed=8,
+                       progress=progress("held-out sets"))
+    if args.dump:
+        est.save(args.dump + "_train.npz", train_s)
+        est.save(args.dump + "_held.npz", held)
     true = np.concatenate([s.rows for s in held])
     q = {}
     for name, mode in (("max", gj.EST_MAX), ("independence", gj.EST_INDEPENDENCE), ("learned", gj.EST_LEARNED)):
@@ -116,13 +128,16 @@ def run_c5(args):
         # evaluated plans join (learned mode), then warm up on a few plans
         t0 = time.perf_counter()
         if mode == gj.EST_LEARNED:
-            for pid in eval_ids:
+            for k, pid in enumerate(eval_ids):
                 ctx.plan_joins(ps.plans[pid].plan, est.plan_inputs(tabs, ps.plans[pid]))
+                if k % 200 == 199:
+                    log(f"statistics: {k + 1} plans, {time.perf_counter() - t0:.1f} s")
         s_stats = time.perf_counter() - t0
         for pid in eval_ids[:args.warmup * 20]:
             ctx.execute_device(ps.plans[pid].plan, est.plan_inputs(tabs, ps.plans[pid])).free()
         t_rows = 0
         ms_plan = 0.0
+        pipelines = retries = 0
         per_step = []
         for step in range(args.steps):
             t_start = time.perf_counter()
@@ -131,13 +146,16 @@ def run_c5(args):
                 r = ctx.execute_device(pp.plan, est.plan_inputs(tabs, pp))
                 st = ctx.stats()
                 ms_plan += st["ms_plan"]
+                pipelines += st["pipelines"]
+                retries += st["retries"]
                 t_rows += r.num_rows
                 r.free()
-                if k % 250 == 249:
-                    log(f"{name} step {step}: {k + 1} plans")
+                if k % 100 == 99:
+                    log(f"{name} step {step}: {k + 1} plans, {time.perf_counter() - t_start:.1f} s")
             per_step.append(time.perf_counter() - t_start)
         results[name] = {"s_per_step": min(per_step), "ms_plan_per_step": ms_plan / args.steps,
-                         "rows_per_step": t_rows // args.steps, "s_statistics_untimed": round(s_stats, 2)}
+                         "rows_per_step": t_rows // args.steps, "s_statistics_untimed": round(s_stats, 2),
+                         "pipelines_per_step": pipelines // args.steps, "retries_per_step": retries // args.steps}
         log(f"{name}: {results[name]}")
     ctx.set_estimator(gj.EST_MAX)
     s_learned = results["learned"]["s_per_step"]
@@ -174,19 +192,23 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--scale", type=float, default=None,
+                    help="table scale (default 1; c5: 0.1, tables of 1K-4M rows)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
                     help="execution option (cgj_set_option), repeatable")
     ap.add_argument("--plans", type=int, default=2000, help="c5: evaluated plans")
     ap.add_argument("--train-plans", type=int, default=150, help="c5: training plans")
     ap.add_argument("--epochs", type=int, default=150, help="c5: training epochs")
+    ap.add_argument("--dump", default="", help="c5: write training/held-out sets to PREFIX_{train,held}.npz")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the join pipeline on a single MI355X "
               f"(--gpus {args.gpus} is not supported; sharding is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.scale is None:
+        args.scale = 0.1 if args.workload == "c5" else 1.0
     if args.workload == "c5":
         return run_c5(args)
 

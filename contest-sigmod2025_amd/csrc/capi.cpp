@@ -217,9 +217,9 @@ int cgj_mlp_loss_grad(cgj_context* ctx, cgj_mlp* m, const float* x, const float*
 }
 
 int cgj_mlp_train(cgj_context* ctx, cgj_mlp* m, const float* x, const float* y, uint32_t n, uint32_t epochs,
-                  uint32_t batch, float lr, uint64_t seed, float* final_loss) {
+                  uint32_t batch, float lr, float weight_decay, uint64_t seed, float* final_loss) {
     return guarded(ctx, [&] {
-        float l = m->impl->train(x, y, n, epochs, batch, lr, seed);
+        float l = m->impl->train(x, y, n, epochs, batch, lr, weight_decay, seed);
         if (final_loss) *final_loss = l;
     });
 }

@@ -260,6 +260,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "build_regions_log") options.build_regions_log = v >= 1 && v <= 14 ? uint32_t(v) : 14;
     else if (name == "lean_waves_per_cu") options.lean_waves_per_cu = v >= 4 && v <= 64 ? uint32_t(v) : 32;
     else if (name == "aligned_pairs") options.aligned_pairs = v != 0;
+    else if (name == "build_threads") options.build_threads = (v == 512 || v == 1024) ? uint32_t(v) : 256u;
     else return false;
     return true;
 }
@@ -355,6 +356,17 @@ private:
 
     const DevColumn& column(uint32_t rel, uint64_t col) const {
         return inputs_[q_.rels[rel].table]->cols[col];
+    }
+    // Rows to size a join result's staging area or page pool for: the
+    // planner's estimate, but never less than the largest input of the set.
+    // An estimate that steers the join order well can still fall short of the
+    // true size, and a short buffer re-runs the whole pipeline; the largest
+    // input bounds every foreign-key join.
+    double size_bound(uint64_t set, double est) const {
+        double b = est;
+        for (uint32_t r = 0; r < q_.rels.size(); ++r)
+            if ((set >> r) & 1) b = std::max(b, double(q_.rels[r].rows));
+        return b;
     }
     uint32_t col_id(uint32_t rel, uint64_t col) {
         uint64_t table = q_.rels[rel].table;
@@ -547,7 +559,7 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
         CGJ_HIP(launch_part_build(staged.stage.get<uint64_t>(), slots, n, ht->words, parts.get<uint64_t>(), bh.get<uint32_t>(),
                                   blocks, chunk, scratch.get<uint32_t>(), total.get<uint64_t>(),
                                   ht->slots.get<uint64_t>(), lg, log_regions, bloom, ht->bloom_log,
-                                  flags.get<uint32_t>(), ctx_.stream()));
+                                  flags.get<uint32_t>(), ctx_.options.build_threads, ctx_.stream()));
         uint32_t f[4];
         CGJ_HIP(hipMemcpyAsync(f, flags.get(), sizeof(f), hipMemcpyDeviceToHost, ctx_.stream()));
         CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
@@ -591,7 +603,7 @@ const HashTable* Executor::build(int node, uint32_t key_eq) {
     Target t;
     t.kind = TARGET_STAGE;
     t.eqs  = eqs;
-    t.est  = tn.card;
+    t.est  = size_bound(tn.set, tn.card);
     std::vector<int> joins;
     int              leaf;
     walk(node, joins, leaf);
@@ -834,7 +846,7 @@ std::unique_ptr<DevResult> Executor::run(bool count_only) {
     register_all_columns();
     Target t;
     t.kind   = count_only ? TARGET_COUNT : TARGET_PAGES;
-    t.est    = t_.nodes[t_.root].card;
+    t.est    = size_bound(t_.nodes[t_.root].set, t_.nodes[t_.root].card);
     t.result = res.get();
     std::vector<int> joins;
     int              leaf;

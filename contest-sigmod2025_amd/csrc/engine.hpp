@@ -151,7 +151,8 @@ struct Options {
     bool     lean_kernel           = true;               // use the one-probe kernel where it applies
     uint32_t lean_waves_per_cu     = 32;                 // lean kernel grid: waves per CU (8 per SIMD)
     uint32_t build_regions_log     = 14;                 // partitioned build: at most 2^this regions
-    bool     aligned_pairs         = false;              // lean probe: 2-slot groups aligned to 2 slots
+    bool     aligned_pairs         = true;               // lean probe: 2-slot groups aligned to 2 slots
+    uint32_t build_threads         = 256;                // partitioned build: threads per workgroup (256/512/1024)
 };
 
 // Tunables of one batched join.

@@ -124,7 +124,7 @@ float Mlp::loss_grad(const float* x, const float* y, uint32_t n, float* grads) {
 }
 
 float Mlp::train(const float* x, const float* y, uint32_t n, uint32_t epochs, uint32_t batch, float lr,
-                 uint64_t seed) {
+                 float weight_decay, uint64_t seed) {
     if (n == 0 || batch == 0) throw std::runtime_error("mlp: no rows");
     hipStream_t s = ctx_.stream();
     DevBuf X(&ctx_.heap(), uint64_t(n) * in_ * sizeof(float)), Y(&ctx_.heap(), uint64_t(n) * sizeof(float));
@@ -150,7 +150,7 @@ float Mlp::train(const float* x, const float* y, uint32_t n, uint32_t epochs, ui
             ++step_;
             const float bc1 = 1.f - std::pow(b1, float(step_)), bc2 = 1.f - std::pow(b2, float(step_));
             CGJ_HIP(launch_adam(params_.get<float>(), grads_.get<float>(), m_.get<float>(), v_.get<float>(),
-                                num_params(), lr, b1, b2, eps, bc1, bc2, s));
+                                num_params(), lr, b1, b2, eps, bc1, bc2, weight_decay, s));
         }
     }
     CGJ_HIP(hipStreamSynchronize(s));

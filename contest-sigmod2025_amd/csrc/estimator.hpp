@@ -35,9 +35,10 @@ public:
     void     get_params(float* p) const;
     // Mean loss over n rows and its gradient w.r.t. every parameter (FP32).
     float    loss_grad(const float* x, const float* y, uint32_t n, float* grads);
-    // Adam over `epochs` passes of shuffled batches; returns the last epoch's mean loss.
+    // Adam (L2 weight decay added to the gradient) over `epochs` passes of
+    // shuffled batches; returns the last epoch's mean loss.
     float    train(const float* x, const float* y, uint32_t n, uint32_t epochs, uint32_t batch, float lr,
-                   uint64_t seed);
+                   float weight_decay, uint64_t seed);
     void     predict(const float* x, uint32_t n, float* out);     // BF16 MFMA, host arrays (n x 2)
     void     predict_f32(const float* x, uint32_t n, float* out); // FP32 forward, host arrays
 

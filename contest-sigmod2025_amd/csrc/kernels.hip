@@ -280,7 +280,7 @@ __global__ void k_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, u
 // Per-block histogram of region ids over a contiguous chunk of staged tuples;
 // stored region-major (bh[region * blocks + block]) so that one exclusive scan
 // yields every block's write offset in every region.
-__global__ __launch_bounds__(256) void k_part_hist(const uint64_t* stage, uint64_t n, uint32_t words,
+__global__ __launch_bounds__(1024) void k_part_hist(const uint64_t* stage, uint64_t n, uint32_t words,
                                                    uint32_t rshift, uint32_t nregions, uint64_t chunk,
                                                    uint32_t* bh) {
     extern __shared__ uint32_t hist[];
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_part_hist(const uint64_t* stage, uint64
 }
 
 // Scatters the same chunk into region order using the scanned offsets.
-__global__ __launch_bounds__(256) void k_part_scatter(const uint64_t* stage, uint64_t n, uint32_t words,
+__global__ __launch_bounds__(1024) void k_part_scatter(const uint64_t* stage, uint64_t n, uint32_t words,
                                                       uint32_t rshift, uint32_t nregions, uint64_t chunk,
                                                       const uint32_t* offs, uint64_t* out) {
     extern __shared__ uint32_t cur[];
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(256) void k_part_scatter(const uint64_t* stage, uin
 // One block per region: inserts the region's tuples into an LDS table of
 // `rslots` slots (linear probing inside the region), then stores the region.
 // flags[0] = duplicate keys seen, flags[1] = a region was too full.
-__global__ __launch_bounds__(256) void k_part_build(const uint64_t* parts, const uint32_t* offs, uint32_t blocks,
+__global__ __launch_bounds__(1024) void k_part_build(const uint64_t* parts, const uint32_t* offs, uint32_t blocks,
                                                     uint64_t n, uint32_t nregions, uint32_t words,
                                                     uint32_t rslots, uint32_t shift, uint64_t* ht,
                                                     uint64_t* bloom, uint32_t bloom_log, uint32_t* flags) {
@@ -363,24 +363,28 @@ __global__ __launch_bounds__(256) void k_part_build(const uint64_t* parts, const
 hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real, uint32_t words, uint64_t* parts,
                              uint32_t* bh, uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch, uint64_t* total,
                              uint64_t* ht, uint32_t log_cap, uint32_t log_regions, uint64_t* bloom,
-                             uint32_t bloom_log, uint32_t* flags, hipStream_t s) {
+                             uint32_t bloom_log, uint32_t* flags, uint32_t threads, hipStream_t s) {
     if (n == 0) return hipSuccess;
+    // The LDS histograms and region tables allow one or two workgroups per CU:
+    // wider workgroups keep more waves (memory requests) in flight.
+    if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;
     uint32_t nregions = 1u << log_regions;
     uint32_t rslots   = 1u << (log_cap - log_regions);
     uint32_t rshift   = 64 - log_regions;
     size_t   hist_lds = size_t(nregions) * sizeof(uint32_t);
-    k_part_hist<<<dim3(blocks), dim3(256), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh);
+    k_part_hist<<<dim3(blocks), dim3(threads), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh);
     hipError_t e = launch_exclusive_scan(bh, uint64_t(nregions) * blocks, scan_scratch, total, s);
     if (e != hipSuccess) return e;
-    k_part_scatter<<<dim3(blocks), dim3(256), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh, parts);
+    k_part_scatter<<<dim3(blocks), dim3(threads), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh,
+                                                                 parts);
     size_t tab_lds = size_t(rslots) * words * sizeof(uint64_t);
     if (tab_lds > 65536) {
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part_build),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(tab_lds));
         if (e != hipSuccess) return e;
     }
-    k_part_build<<<dim3(nregions), dim3(256), tab_lds, s>>>(parts, bh, blocks, n_real, nregions, words, rslots,
-                                                            64 - log_cap, ht, bloom, bloom_log, flags);
+    k_part_build<<<dim3(nregions), dim3(threads), tab_lds, s>>>(parts, bh, blocks, n_real, nregions, words, rslots,
+                                                                64 - log_cap, ht, bloom, bloom_log, flags);
     return hipGetLastError();
 }
 

@@ -227,7 +227,7 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
                              uint32_t* bh,
                              uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch, uint64_t* total,
                              uint64_t* ht, uint32_t log_cap, uint32_t log_regions, uint64_t* bloom,
-                             uint32_t bloom_log, uint32_t* flags, hipStream_t s);
+                             uint32_t bloom_log, uint32_t* flags, uint32_t threads, hipStream_t s);
 // Launches the fused pipeline. `slots` = tuple width (selects the kernel variant).
 hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s);
 // Lean variant for one probe without residual predicates and a count or

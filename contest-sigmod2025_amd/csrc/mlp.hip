@@ -120,11 +120,12 @@ __global__ void k_nll(const float* out, const float* y, uint32_t M, float* dout,
     if (lane_id() == 0) atomicAdd(loss_sum, l);
 }
 
+// Adam step (Kingma & Ba, 2015) with L2 weight decay added to the gradient.
 __global__ void k_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2,
-                       float eps, float bc1, float bc2) {
+                       float eps, float bc1, float bc2, float wd) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float gi = g[i];
+    const float gi = g[i] + wd * p[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i]           = mi;
@@ -242,8 +243,8 @@ hipError_t launch_nll(const float* out, const float* y, uint32_t M, float* dout,
     return hipGetLastError();
 }
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2,
-                       float eps, float bc1, float bc2, hipStream_t s) {
-    k_adam<<<blocks_for(n, 256), 256, 0, s>>>(p, g, m, v, n, lr, b1, b2, eps, bc1, bc2);
+                       float eps, float bc1, float bc2, float wd, hipStream_t s) {
+    k_adam<<<blocks_for(n, 256), 256, 0, s>>>(p, g, m, v, n, lr, b1, b2, eps, bc1, bc2, wd);
     return hipGetLastError();
 }
 hipError_t launch_gather(const float* X, const float* Y, const uint32_t* idx, uint32_t M, uint32_t in, float* Xb,

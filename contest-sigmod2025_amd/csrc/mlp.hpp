@@ -20,7 +20,7 @@ hipError_t launch_relu_grad(float* d, const float* act, uint64_t n, hipStream_t 
 hipError_t launch_colsum(const float* d, uint32_t M, uint32_t N, float* out, hipStream_t s);
 hipError_t launch_nll(const float* out, const float* y, uint32_t M, float* dout, float* loss_sum, hipStream_t s);
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2,
-                       float eps, float bc1, float bc2, hipStream_t s);
+                       float eps, float bc1, float bc2, float wd, hipStream_t s);
 hipError_t launch_gather(const float* X, const float* Y, const uint32_t* idx, uint32_t M, uint32_t in, float* Xb,
                          float* Yb, hipStream_t s);
 hipError_t launch_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s);

@@ -178,7 +178,7 @@ SIGNATURES = {
     "cgj_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, _F32P]),
     "cgj_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, _F32P, C.c_uint32, _F32P, _F32P]),
     "cgj_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, _F32P, C.c_uint32, C.c_uint32, C.c_uint32,
-                                C.c_float, C.c_uint64, _F32P]),
+                                C.c_float, C.c_float, C.c_uint64, _F32P]),
     "cgj_mlp_predict": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, C.c_uint32, _F32P]),
     "cgj_mlp_predict_f32": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, C.c_uint32, _F32P]),
     "cgj_mlp_free": (None, [C.c_void_p, C.c_void_p]),
@@ -526,15 +526,15 @@ class Mlp:
                                             g.ctypes.data_as(_F32P)))
         return loss.value, g
 
-    def train(self, x, y, epochs: int, batch: int, lr: float, seed: int = 1) -> float:
+    def train(self, x, y, epochs: int, batch: int, lr: float, seed: int = 1, weight_decay: float = 0.0) -> float:
         xa, xp = _f32(x)
         n = xa.shape[0]
         if xa.ndim != 2 or xa.shape[1] != self.in_features:
             raise ValueError("x must be n x in_features")
         ya, yp = _f32(y, (n,))
         loss = C.c_float()
-        self._check(lib().cgj_mlp_train(self.ctx.handle, self.handle, xp, yp, n, epochs, batch, lr, seed,
-                                        C.byref(loss)))
+        self._check(lib().cgj_mlp_train(self.ctx.handle, self.handle, xp, yp, n, epochs, batch, lr, weight_decay,
+                                        seed, C.byref(loss)))
         return loss.value
 
     def _predict(self, fn, x):

@@ -34,13 +34,16 @@ def plan_inputs(pool_tables: Sequence[DeviceTable], pp) -> List[DeviceTable]:
 
 
 def collect(ctx: Context, planset, pool_tables: Sequence[DeviceTable], plan_ids: Sequence[int],
-            max_sets_per_plan: int = 48, seed: int = 7, counts: Dict = None) -> List[SetSamples]:
+            max_sets_per_plan: int = 48, seed: int = 7, counts: Dict = None, progress=None) -> List[SetSamples]:
     """Features and exact cardinalities of up to `max_sets_per_plan` connected
     sets per plan (all sets of the full plan included). `counts` caches exact
-    counts by (plan, set) across calls."""
+    counts by (plan, set) across calls; `progress(plans_done)` is called after
+    every plan."""
     rng = random.Random(seed)
     out = []
-    for pid in plan_ids:
+    for done, pid in enumerate(plan_ids):
+        if progress is not None and done:
+            progress(done)
         pp = planset.plans[pid]
         tabs = plan_inputs(pool_tables, pp)
         sets, feats = ctx.plan_features(pp.plan, tabs)
@@ -65,6 +68,13 @@ def collect(ctx: Context, planset, pool_tables: Sequence[DeviceTable], plan_ids:
     return out
 
 
+def save(path: str, samples: Sequence[SetSamples]):
+    """Writes samples as arrays (plan, set, features, rows) to an .npz file."""
+    np.savez(path, plan=np.concatenate([np.full(len(s.rows), s.plan) for s in samples]),
+             sets=np.concatenate([s.sets for s in samples]), features=np.concatenate([s.features for s in samples]),
+             rows=np.concatenate([s.rows for s in samples]))
+
+
 def base_log10(features: np.ndarray) -> np.ndarray:
     """The independence estimate in log10 units, at least 0 (the engine's base)."""
     return np.maximum(features[:, 10].astype(np.float64) * 8.0, 0.0)
@@ -80,11 +90,13 @@ def stack(samples: Sequence[SetSamples]) -> Tuple[np.ndarray, np.ndarray]:
     return x, y
 
 
-def train(ctx: Context, samples: Sequence[SetSamples], epochs: int = 200, batch: int = 256, lr: float = 1e-3,
-          seed: int = 1) -> Tuple[Mlp, float]:
+def train(ctx: Context, samples: Sequence[SetSamples], epochs: int = 150, batch: int = 256, lr: float = 1e-3,
+          weight_decay: float = 1e-4, seed: int = 1) -> Tuple[Mlp, float]:
+    """Trains the residual network. The defaults won a float64 sweep over
+    c5 sets (held-out q-error p90 equal to the independence rule's, p99 lower)."""
     x, y = stack(samples)
     m = ctx.mlp(Context.num_features(), seed=seed)
-    loss = m.train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
+    loss = m.train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed + 1, weight_decay=weight_decay)
     return m, loss
 
 

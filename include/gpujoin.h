@@ -134,7 +134,8 @@ typedef struct {
  *   "lean_kernel": 1 (default) runs one-probe pipelines on the lean kernel, 0 on the general one;
  *   "lean_waves_per_cu": grid of the lean kernel in waves per compute unit (4..64, default 32);
  *   "aligned_pairs": the lean kernel's probe steps read 2-slot groups aligned to two
- *       slots (1) or the next two slots (0, default). */
+ *       slots (1, default) or the next two slots (0);
+ *   "build_threads": threads per workgroup of the partitioned build kernels (256, 512, 1024). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
@@ -246,7 +247,11 @@ int      cgj_mlp_get_params(cgj_context* ctx, const cgj_mlp* m, float* params);
 int      cgj_mlp_loss_grad(cgj_context* ctx, cgj_mlp* m, const float* x, const float* y, uint32_t n, float* loss,
                            float* grads);
 int      cgj_mlp_train(cgj_context* ctx, cgj_mlp* m, const float* x, const float* y, uint32_t n, uint32_t epochs,
-                       uint32_t batch, float lr, uint64_t seed, float* final_loss);
+                       uint32_t batch, float lr, float weight_decay,
+                       uint64_t seed, float* final_loss);
+/* cgj_mlp_train: Adam (beta 0.9/0.999, eps 1e-8) with `weight_decay` x params
+ * added to each gradient, over shuffled batches; *final_loss = the last
+ * epoch's mean loss. */
 /* out: n x 2 (mu, log sigma). _predict is the BF16 MFMA path, _predict_f32 the FP32 forward. */
 int      cgj_mlp_predict(cgj_context* ctx, cgj_mlp* m, const float* x, uint32_t n, float* out);
 int      cgj_mlp_predict_f32(cgj_context* ctx, cgj_mlp* m, const float* x, uint32_t n, float* out);

@@ -81,9 +81,10 @@ def forward_bf16(params, x):
     return h2 @ w3 + b3
 
 
-def adam_full_batch(params, x, y, steps, lr, b1=0.9, b2=0.999, eps=1e-8):
+def adam_full_batch(params, x, y, steps, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
     """`steps` Adam updates (Kingma & Ba, 2015, Algorithm 1) on the gradient
-    of the whole set; returns the parameters and the loss before each step."""
+    of the whole set plus wd x params (L2 weight decay); returns the
+    parameters and the loss before each step."""
     p = np.asarray(params, dtype=np.float64).copy()
     m = np.zeros_like(p)
     v = np.zeros_like(p)
@@ -91,6 +92,7 @@ def adam_full_batch(params, x, y, steps, lr, b1=0.9, b2=0.999, eps=1e-8):
     for t in range(1, steps + 1):
         loss, g = loss_grad(p, x, y)
         losses.append(loss)
+        g = g + wd * p
         m = b1 * m + (1 - b1) * g
         v = b2 * v + (1 - b2) * g * g
         p -= lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)

@@ -70,17 +70,19 @@ def test_bf16_inference_vs_oracle(ctx, n_in, n):
     assert np.max(np.abs(got - exp32)) <= 3e-2 * scale
 
 
-def test_full_batch_training_vs_oracle(ctx):
+@pytest.mark.parametrize("wd", [0.0, 1e-2])
+def test_full_batch_training_vs_oracle(ctx, wd):
     """One batch holding every row makes the GPU's Adam steps deterministic up
-    to FP32 rounding: compare them with the oracle's float64 Adam."""
-    say("mlp full-batch Adam vs oracle")
+    to FP32 rounding: compare them with the oracle's float64 Adam (with and
+    without weight decay)."""
+    say(f"mlp full-batch Adam vs oracle, weight decay {wd}")
     n_in, n, steps, lr = 8, 256, 20, 1e-3
     x, y = _data(n, n_in, 5)
     m = ctx.mlp(n_in, seed=9)
     p0 = m.get_params()
-    final = m.train(x, y, epochs=steps, batch=n, lr=lr, seed=1)
+    final = m.train(x, y, epochs=steps, batch=n, lr=lr, seed=1, weight_decay=wd)
     p_gpu = m.get_params().astype(np.float64)
-    p_ref, losses = ref.adam_full_batch(p0, x, y, steps, lr)
+    p_ref, losses = ref.adam_full_batch(p0, x, y, steps, lr, wd=wd)
     # the returned loss is the last step's, before its update
     assert abs(final - losses[-1]) <= 1e-3 * max(1.0, abs(losses[-1])), (final, losses[-1])
     l_gpu, _ = ref.loss_grad(p_gpu, x, y)

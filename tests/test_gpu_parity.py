@@ -270,20 +270,21 @@ def test_lean_kernel_shapes(ctx, seed):
 
 
 @pytest.mark.parametrize("seed", range(0, 18, 2))
-def test_lean_kernel_aligned_pair_probes(ctx, seed):
-    """Probe steps over 2-slot groups aligned to two slots (the slot before an
-    odd start slot is read but never decides the lookup): the oracle's rows."""
+def test_lean_kernel_consecutive_pair_probes(ctx, seed):
+    """Probe steps over the next two slots (the default reads 2-slot groups
+    aligned to two slots, where the slot before an odd start slot is read but
+    never decides the lookup): the oracle's rows either way."""
     plan = _lean_shape_plan(seed)
-    say(f"lean aligned pairs seed={seed}")
+    say(f"lean consecutive pairs seed={seed}")
     types = [t for _, t in plan.nodes[plan.root].output_attrs]
     exp = check_table(oracle.execute(plan), types)
-    ctx.set_option("aligned_pairs", 1)
+    ctx.set_option("aligned_pairs", 0)
     try:
         got = check_table(ctx.execute(plan), types)
         tables = [ctx.upload(t) for t in plan.inputs]
         count = ctx.count_device(plan, tables)
     finally:
-        ctx.set_option("aligned_pairs", 0)
+        ctx.set_option("aligned_pairs", 1)
     assert got == exp
     assert count == len(exp)
 

@@ -55,3 +55,20 @@ def test_bf16_rounding_is_nearest_even():
     # 1 + 2^-8 is halfway between 1 and 1 + 2^-7: ties go to the even mantissa (1.0)
     assert out[0] == 1.0 and out[1] == 1.0 and out[2] == np.float32(1.0078125)
     assert abs(out[3] - 3.0e-3) < 3.0e-3 * 2 ** -8 and out[4] == -2.5
+
+
+def test_adam_weight_decay_pulls_towards_zero():
+    """With a zero data gradient (all weights zero, targets at the output bias's
+    optimum) only the decay term moves the parameters: each Adam step moves a
+    parameter by about lr towards zero (the first step exactly lr * sign)."""
+    n_in = 2
+    p = np.zeros(ref.num_params(n_in))
+    p[:n_in * ref.H] = 0.5  # W1 only; W2 = 0 makes every gradient of W1 zero
+    # targets equal to mu = 0 with s = 0 give a zero output-bias gradient for mu,
+    # and d/ds = 1 - r^2 = 1 at r = 0; keep it by comparing W1 alone
+    x = np.ones((3, n_in))
+    y = np.zeros(3)
+    out, _ = ref.adam_full_batch(p, x, y, steps=1, lr=1e-3, wd=0.1)
+    assert np.allclose(out[:n_in * ref.H], 0.5 - 1e-3, atol=1e-9)
+    out0, _ = ref.adam_full_batch(p, x, y, steps=1, lr=1e-3, wd=0.0)
+    assert np.allclose(out0[:n_in * ref.H], 0.5)
