@@ -71,6 +71,33 @@ def load_traffic(name):
     return None
 
 
+def c5_cpu_baseline(ps, eval_ids, target_seconds=15.0):
+    """The CPU oracle (one thread) executing evaluated plans in their given
+    join order until ~15 s of work; host generation of the pool tables is not
+    timed."""
+    import oracle
+    host = {}
+    t_run = 0.0
+    n = 0
+    for pid in eval_ids:
+        pp = ps.plans[pid]
+        for t in pp.tables:
+            if t not in host:
+                host[t] = ps.pool[t].host_table(ps.seed)
+        plan = pp.plan
+        plan.inputs = [host[t] for t in pp.tables]
+        t0 = time.perf_counter()
+        oracle.execute(plan)
+        t_run += time.perf_counter() - t0
+        plan.inputs = []
+        n += 1
+        if t_run >= target_seconds:
+            break
+    log(f"cpu baseline: {n} c5 plans in {t_run:.2f} s")
+    return {"value": n / t_run, "unit": "plans/s", "cores": 1, "kind": "port",
+            "sample": f"the first {n} evaluated plans, CPU oracle execute() in each plan's given join order"}
+
+
 def run_c5(args):
     """Plan set (3-12-table chains/stars/trees over a shared table pool):
     trains the estimator on the first `train_plans` plans' connected sets
@@ -138,6 +165,7 @@ def run_c5(args):
         t_rows = 0
         ms_plan = 0.0
         pipelines = retries = 0
+        ms_build = ms_probe = 0.0
         per_step = []
         for step in range(args.steps):
             t_start = time.perf_counter()
@@ -147,6 +175,8 @@ def run_c5(args):
                 st = ctx.stats()
                 ms_plan += st["ms_plan"]
                 pipelines += st["pipelines"]
+                ms_build += st["ms_build"]
+                ms_probe += st["ms_probe"]
                 retries += st["retries"]
                 t_rows += r.num_rows
                 r.free()
@@ -155,7 +185,9 @@ def run_c5(args):
             per_step.append(time.perf_counter() - t_start)
         results[name] = {"s_per_step": min(per_step), "ms_plan_per_step": ms_plan / args.steps,
                          "rows_per_step": t_rows // args.steps, "s_statistics_untimed": round(s_stats, 2),
-                         "pipelines_per_step": pipelines // args.steps, "retries_per_step": retries // args.steps}
+                         "pipelines_per_step": pipelines // args.steps, "retries_per_step": retries // args.steps,
+                         "ms_device_build_per_step": round(ms_build / args.steps, 1),
+                         "ms_device_probe_per_step": round(ms_probe / args.steps, 1)}
         log(f"{name}: {results[name]}")
     ctx.set_estimator(gj.EST_MAX)
     s_learned = results["learned"]["s_per_step"]
@@ -180,10 +212,12 @@ def run_c5(args):
         "training": {"sets": int(sum(len(s.rows) for s in train_s)), "s_collect": round(t_collect, 1),
                      "s_train": round(t_train, 1), "epochs": args.epochs, "final_loss": round(loss, 4)},
     }
-    print(json.dumps(line), flush=True)
     for t in tabs:
         t.free()
     ctx.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = c5_cpu_baseline(ps, eval_ids)
+    print(json.dumps(line), flush=True)
 
 
 def main():

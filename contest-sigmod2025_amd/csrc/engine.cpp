@@ -27,15 +27,22 @@ namespace cgj {
 // ---------------------------------------------------------------------------
 // device heap
 // ---------------------------------------------------------------------------
+// Size classes: 4 KiB steps up to 1 MiB, then four classes per power of two
+// (2^k x 1, 1.25, 1.5, 1.75; at most 25% over the request), so buffers of
+// varying sizes (hash tables, staging areas, page pools of successive
+// queries) map to few classes and are served from the cache after warm-up
+// instead of by hipMalloc.
 size_t DeviceHeap::round(size_t b) {
     if (b <= (1u << 20)) return (b + 4095) & ~size_t(4095);
-    return (b + (2u << 20) - 1) & ~((size_t(2) << 20) - 1);
+    uint32_t k = 63 - uint32_t(__builtin_clzll(b)); // 2^k <= b < 2^(k+1)
+    size_t   q = size_t(1) << (k - 2);              // class step: a quarter of 2^k
+    return (b + q - 1) & ~(q - 1);
 }
 
 void* DeviceHeap::alloc(size_t bytes) {
     size_t r  = round(bytes);
     auto   it = free_.lower_bound(r);
-    if (it != free_.end() && it->first <= r + r / 4) {
+    if (it != free_.end() && it->first <= r + r / 2) {
         void* p = it->second;
         live_[p] = it->first;
         free_.erase(it);
@@ -260,7 +267,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "build_regions_log") options.build_regions_log = v >= 1 && v <= 14 ? uint32_t(v) : 14;
     else if (name == "lean_waves_per_cu") options.lean_waves_per_cu = v >= 4 && v <= 64 ? uint32_t(v) : 32;
     else if (name == "aligned_pairs") options.aligned_pairs = v != 0;
-    else if (name == "build_threads") options.build_threads = (v == 512 || v == 1024) ? uint32_t(v) : 256u;
+    else if (name == "build_threads") options.build_threads = (v == 256 || v == 512) ? uint32_t(v) : 1024u;
     else return false;
     return true;
 }
@@ -381,7 +388,13 @@ private:
     }
     void register_all_columns();
     void walk(int node, std::vector<int>& joins, int& leaf) const;
-    const HashTable* build(int node, uint32_t key_eq);
+    // Builds the hash table of join-tree node `node` on class `key_eq`;
+    // `probe_rows` = rows expected to probe it (tunes load factor and bloom).
+    const HashTable* build(int node, uint32_t key_eq, uint64_t probe_rows);
+    uint64_t         probe_rows_of(const TreeNode& join) const {
+        const TreeNode& p = t_.nodes[join.probe];
+        return uint64_t(std::min(size_bound(p.set, p.card), 1.8e19));
+    }
     const HashTable* make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est);
     void pipeline(Driver d, std::vector<ProbeStep> probes, Target& t);
     uint64_t launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms);
@@ -594,7 +607,7 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
     return tables_.back().get();
 }
 
-const HashTable* Executor::build(int node, uint32_t key_eq) {
+const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows) {
     const TreeNode&       tn = t_.nodes[node];
     std::vector<uint32_t> eqs{key_eq};
     for (uint32_t e : q_.needed(tn.set))
@@ -614,7 +627,7 @@ const HashTable* Executor::build(int node, uint32_t key_eq) {
         const TreeNode& bn  = t_.nodes[jn.build];
         auto            sh  = q_.shared(bn.set, cur);
         if (sh.empty()) throw std::runtime_error("join without a predicate");
-        const HashTable* ht = build(jn.build, sh.front());
+        const HashTable* ht = build(jn.build, sh.front(), probe_rows_of(jn));
         if (!ht) return nullptr;
         probes.push_back(ProbeStep{ht, sh.front(), bn.set});
         cur |= bn.set;
@@ -624,7 +637,7 @@ const HashTable* Executor::build(int node, uint32_t key_eq) {
     d.set = 1ull << leaf;
     pipeline(d, probes, t);
     if (t.count == 0) return nullptr; // empty build side: the inner join is empty
-    return make_table(t, eqs, uint64_t(t_.nodes[t_.root].card));
+    return make_table(t, eqs, probe_rows);
 }
 
 void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
@@ -858,7 +871,7 @@ std::unique_ptr<DevResult> Executor::run(bool count_only) {
         const TreeNode& bn = t_.nodes[jn.build];
         auto            sh = q_.shared(bn.set, cur);
         if (sh.empty()) throw std::runtime_error("join without a predicate");
-        const HashTable* ht = build(jn.build, sh.front());
+        const HashTable* ht = build(jn.build, sh.front(), probe_rows_of(jn));
         if (!ht) return res; // empty build side
         probes.push_back(ProbeStep{ht, sh.front(), bn.set});
         cur |= bn.set;

@@ -152,7 +152,10 @@ struct Options {
     uint32_t lean_waves_per_cu     = 32;                 // lean kernel grid: waves per CU (8 per SIMD)
     uint32_t build_regions_log     = 14;                 // partitioned build: at most 2^this regions
     bool     aligned_pairs         = true;               // lean probe: 2-slot groups aligned to 2 slots
-    uint32_t build_threads         = 256;                // partitioned build: threads per workgroup (256/512/1024)
+    // Partitioned build: threads per workgroup (256/512/1024). Its LDS tables
+    // allow one or two workgroups per CU, so wider ones keep more loads in
+    // flight (c2 build 4.8 -> 3.9 ms at 1024).
+    uint32_t build_threads         = 1024;
 };
 
 // Tunables of one batched join.

@@ -12,6 +12,8 @@
 //    pages, or a tuple count.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "gen.hpp"
 #include "kernels.hpp"
 
@@ -379,9 +381,17 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
                                                                  parts);
     size_t tab_lds = size_t(rslots) * words * sizeof(uint64_t);
     if (tab_lds > 65536) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part_build),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(tab_lds));
-        if (e != hipSuccess) return e;
+        // raise the kernel's dynamic LDS limit once per device, to the most any region needs
+        constexpr int             kMaxDev = 16;
+        static std::atomic<size_t> lds_set[kMaxDev];
+        int                        dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev ||
+            lds_set[dev].load(std::memory_order_relaxed) < tab_lds) {
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part_build),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(tab_lds));
+            if (e != hipSuccess) return e;
+            if (dev >= 0 && dev < kMaxDev) lds_set[dev].store(tab_lds, std::memory_order_relaxed);
+        }
     }
     k_part_build<<<dim3(nregions), dim3(threads), tab_lds, s>>>(parts, bh, blocks, n_real, nregions, words, rslots,
                                                                 64 - log_cap, ht, bloom, bloom_log, flags);
@@ -1541,13 +1551,23 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
         reinterpret_cast<const void*>(&k_lean<4>)};
     const uint32_t no = l.nout < uint32_t(kLeanOut) ? l.nout : uint32_t(kLeanOut);
     if (!a.xcd_order) {
-        int dev = 0, cus = 0, per_cu = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[no], 256, 0) == hipSuccess && per_cu > 0 &&
-            cus > 0) {
-            const uint32_t resident = uint32_t(per_cu) * uint32_t(cus);
-            if (blocks > resident) blocks = resident;
+        // resident workgroups per device and kernel, queried once (the
+        // occupancy query costs more host time than a small launch)
+        constexpr int              kMaxDev = 16;
+        static std::atomic<uint32_t> resident_cache[kMaxDev][kLeanOut + 1];
+        int                          dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kMaxDev) {
+            uint32_t resident = resident_cache[dev][no].load(std::memory_order_relaxed);
+            if (resident == 0) {
+                int cus = 0, per_cu = 0;
+                if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+                    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[no], 256, 0) == hipSuccess &&
+                    per_cu > 0 && cus > 0) {
+                    resident = uint32_t(per_cu) * uint32_t(cus);
+                    resident_cache[dev][no].store(resident, std::memory_order_relaxed);
+                }
+            }
+            if (resident && blocks > resident) blocks = resident;
         }
     }
     switch (no) {

@@ -135,7 +135,8 @@ typedef struct {
  *   "lean_waves_per_cu": grid of the lean kernel in waves per compute unit (4..64, default 32);
  *   "aligned_pairs": the lean kernel's probe steps read 2-slot groups aligned to two
  *       slots (1, default) or the next two slots (0);
- *   "build_threads": threads per workgroup of the partitioned build kernels (256, 512, 1024). */
+ *   "build_threads": threads per workgroup of the partitioned build kernels (256, 512,
+ *       1024 = default). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
