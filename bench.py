@@ -71,6 +71,93 @@ def load_traffic(name):
     return None
 
 
+def pcie_inclusive(ctx, name, scale, reps=3):
+    """Host-to-host rate of the contest entry point: each call uploads the
+    scanned columns' pages, runs the query and downloads the result into host
+    pages (one operator-new block per page), as Contest::execute does."""
+    from gpujoin import workloads
+    wl = workloads.WORKLOADS[name](scale)
+    plan = wl.make_plan()
+    for t in wl.tables:
+        plan.new_input(t.host_table(wl.seed))
+    tuples = sum(t.rows for t in wl.tables)
+    res = ctx.execute(plan)  # warm-up
+    rows = res.num_rows
+    del res
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = ctx.execute(plan)
+        times.append(time.perf_counter() - t0)
+        del res
+    t = min(times)
+    log(f"pcie-inclusive: {name} at scale {scale}: {t * 1000:.1f} ms for {tuples} tuples")
+    return {"value": round(tuples / t / 1e9, 4), "unit": "Gtuples/s", "scale": scale, "tuples": tuples,
+            "output_rows": rows, "ms": round(t * 1000.0, 2),
+            "path": "host pages in, host pages out (upload, execute, download); min of 3"}
+
+
+def run_c4(args):
+    """256 independent INT32 joins (build log-uniform [1K, 2M] rows, probe 10x),
+    each one execute_device call on device-resident tables: the one-launch-per-
+    join path. (The batched path, one persistent launch over a device work
+    queue, is disabled in this build.) A step is one pass over all joins."""
+    import gpujoin as gj
+    from gpujoin import workloads
+
+    bw = workloads.c4(args.scale)
+    ctx = gj.Context(0)
+    for o in args.option:
+        name, value = o.split("=", 1)
+        ctx.set_option(name, float(value))
+    t0 = time.perf_counter()
+    jobs = []
+    for w in bw.jobs:
+        jobs.append((w.make_plan(), [ctx.generate(w.seed, t.rows, t.columns) for t in w.tables]))
+    tuples = sum(t.rows for w in bw.jobs for t in w.tables)
+    log(f"c4: {len(jobs)} joins, {tuples} tuples generated in {time.perf_counter() - t0:.1f} s")
+    for _ in range(args.warmup):
+        for plan, tabs in jobs:
+            ctx.execute_device(plan, tabs).free()
+    per_step = []
+    ms_dev = 0.0
+    rows = 0
+    for step in range(args.steps):
+        t_start = time.perf_counter()
+        for plan, tabs in jobs:
+            r = ctx.execute_device(plan, tabs)
+            st = ctx.stats()
+            ms_dev += st["ms_total"]
+            rows += r.num_rows
+            r.free()
+        per_step.append(time.perf_counter() - t_start)
+        log(f"c4 step {step}: {per_step[-1] * 1000:.1f} ms")
+    s_step = sum(per_step) / len(per_step)
+    line = {
+        "metric": "c4 batched joins: Gtuples/s over 256 independent joins (one launch per join path)",
+        "value": round(tuples / s_step / 1e9, 4),
+        "unit": "Gtuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s_step * 1000.0, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"c4: {bw.description}", "path": "one execute_device per join (batched path disabled)"},
+        "joins_per_s": round(len(jobs) / s_step, 1),
+        "ms_device_per_step": round(ms_dev / args.steps, 2),
+        "output_rows_per_step": rows // args.steps,
+    }
+    print(json.dumps(line), flush=True)
+    for _, tabs in jobs:
+        for t in tabs:
+            t.free()
+    ctx.close()
+
+
 def c5_cpu_baseline(ps, eval_ids, target_seconds=15.0):
     """The CPU oracle (one thread) executing evaluated plans in their given
     join order until ~15 s of work; host generation of the pool tables is not
@@ -235,6 +322,8 @@ def main():
     ap.add_argument("--train-plans", type=int, default=150, help="c5: training plans")
     ap.add_argument("--epochs", type=int, default=150, help="c5: training epochs")
     ap.add_argument("--dump", default="", help="c5: write training/held-out sets to PREFIX_{train,held}.npz")
+    ap.add_argument("--pcie", type=float, default=0.0, metavar="SCALE",
+                    help="also time the host-pages-in/host-pages-out path at this table scale")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the join pipeline on a single MI355X "
@@ -245,6 +334,8 @@ def main():
         args.scale = 0.1 if args.workload == "c5" else 1.0
     if args.workload == "c5":
         return run_c5(args)
+    if args.workload == "c4":
+        return run_c4(args)
 
     import gpujoin as gj
     from gpujoin import workloads
@@ -336,11 +427,13 @@ def main():
             "copy_gbs": round(copy_gbs, 1),
         },
     }
+    for t in tables:
+        t.free()
+    if args.pcie:
+        line["pcie_inclusive"] = pcie_inclusive(ctx, args.workload, args.pcie)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl.name)
     print(json.dumps(line), flush=True)
-    for t in tables:
-        t.free()
     ctx.close()
 
 
