@@ -268,6 +268,8 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "lean_waves_per_cu") options.lean_waves_per_cu = v >= 4 && v <= 64 ? uint32_t(v) : 32;
     else if (name == "aligned_pairs") options.aligned_pairs = v != 0;
     else if (name == "build_threads") options.build_threads = (v == 256 || v == 512) ? uint32_t(v) : 1024u;
+    else if (name == "group_walk") options.group_walk = v >= 0 && v < 1e6 ? uint32_t(v) : 32u;
+    else if (name == "force_grouped") options.force_grouped = v != 0;
     else return false;
     return true;
 }
@@ -288,6 +290,11 @@ uint32_t log2_ceil(uint64_t x) {
 struct HashTable {
     DevBuf                slots;
     DevBuf                bloom;
+    // Grouped layout (keys that repeat many times): `slots` holds 2-word
+    // entries [key, start << 24 | count]; words 1.. of the key's tuples are
+    // rows start.. of `payload` (words - 1 words each).
+    bool                  grouped = false;
+    DevBuf                payload;
     uint64_t              cap       = 0;
     uint64_t              rmask     = 0; // linear probing wraps inside regions of rmask + 1 slots
     uint32_t              shift     = 64;
@@ -558,8 +565,18 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
     if (lg > rs_log + max_rlog) rs_log = lg - max_rlog;
     bool partitioned = tn.partitioned && n >= 65536 && lg > rs_log &&
                        (uint64_t(1) << rs_log) * ht->words * 8 <= ctx_.lds_bytes();
-    bool done        = false;
-    if (partitioned) {
+    // With grouping on, an insert that passes `walk_cap` slots holding its own
+    // key (a key repeated that often) abandons the plain layout: the table is
+    // rebuilt grouped instead of finishing a quadratic insert.
+    const bool     can_group = ctx_.options.group_walk > 0;
+    const uint32_t walk_cap  = can_group ? ctx_.options.group_walk : 0;
+    bool           done = false, group = ctx_.options.force_grouped;
+    auto           read_flags = [&](uint32_t* f) {
+        CGJ_HIP(hipMemcpyAsync(f, flags.get(), 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx_.stream()));
+        CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
+        CGJ_HIP(hipMemsetAsync(flags.get(), 0, 4 * sizeof(uint32_t), ctx_.stream()));
+    };
+    if (partitioned && !group) {
         uint32_t log_regions = lg - rs_log;
         uint64_t nregions    = 1ull << log_regions;
         uint64_t slots       = staged.slots;
@@ -572,28 +589,75 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
         CGJ_HIP(launch_part_build(staged.stage.get<uint64_t>(), slots, n, ht->words, parts.get<uint64_t>(), bh.get<uint32_t>(),
                                   blocks, chunk, scratch.get<uint32_t>(), total.get<uint64_t>(),
                                   ht->slots.get<uint64_t>(), lg, log_regions, bloom, ht->bloom_log,
-                                  flags.get<uint32_t>(), ctx_.options.build_threads, ctx_.stream()));
+                                  flags.get<uint32_t>(), ctx_.options.build_threads, walk_cap, ctx_.stream()));
         uint32_t f[4];
-        CGJ_HIP(hipMemcpyAsync(f, flags.get(), sizeof(f), hipMemcpyDeviceToHost, ctx_.stream()));
-        CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
-        if (!f[1]) {
+        read_flags(f);
+        if (!f[1] && !f[2]) {
             done       = true;
             ht->rmask  = (1ull << rs_log) - 1;
             ht->unique = f[0] == 0;
         } else {
-            ++retries; // a region overflowed (heavy duplicates): use global inserts
-            CGJ_HIP(hipMemsetAsync(flags.get(), 0, 4 * sizeof(uint32_t), ctx_.stream()));
-            if (bloom) CGJ_HIP(launch_fill_u64(bloom, 1ull << ht->bloom_log, 0, ctx_.stream()));
+            ++retries; // a region overflowed or an insert walked too far (heavy duplicates)
+            group = can_group;
+        }
+    }
+    if (!done && !group) {
+        CGJ_HIP(launch_fill_u64(ht->slots.get<uint64_t>(), ht->cap * ht->words, kEmpty, ctx_.stream()));
+        CGJ_HIP(launch_ht_insert(staged.stage.get<uint64_t>(), staged.slots, ht->words, ht->slots.get<uint64_t>(), ht->shift,
+                                 ht->cap - 1, bloom, ht->bloom_log, flags.get<uint32_t>(), walk_cap, ctx_.stream()));
+        uint32_t f[4];
+        read_flags(f);
+        if (!f[2]) {
+            done       = true;
+            ht->rmask  = ht->cap - 1;
+            ht->unique = f[0] == 0;
+        } else {
+            ++retries;
+            group = true;
         }
     }
     if (!done) {
-        CGJ_HIP(launch_fill_u64(ht->slots.get<uint64_t>(), ht->cap * ht->words, kEmpty, ctx_.stream()));
-        CGJ_HIP(launch_ht_insert(staged.stage.get<uint64_t>(), staged.slots, ht->words, ht->slots.get<uint64_t>(), ht->shift,
-                                 ht->cap - 1, bloom, ht->bloom_log, flags.get<uint32_t>(), ctx_.stream()));
-        uint32_t f[4];
-        CGJ_HIP(hipMemcpyAsync(f, flags.get(), sizeof(f), hipMemcpyDeviceToHost, ctx_.stream()));
-        CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
-        ht->unique = f[0] == 0;
+        // Grouped build: regions of up to 8K two-word slots built in LDS (keys,
+        // counts, cursors: 16 bytes per slot). Capacity follows the tuple
+        // count, an upper bound on the distinct keys; a region whose distinct
+        // keys still overflow doubles the capacity.
+        const uint32_t threads = ctx_.options.build_threads;
+        for (int attempt = 0;; ++attempt) {
+            uint32_t glog = 13;
+            while (glog > 6 && group_build_lds(1u << glog, threads) > ctx_.lds_bytes()) --glog;
+            if (glog > lg) glog = lg;
+            if (lg > glog + ctx_.options.build_regions_log) glog = lg - ctx_.options.build_regions_log;
+            if (group_build_lds(1u << glog, threads) > ctx_.lds_bytes())
+                throw std::runtime_error("grouped hash table: region does not fit the LDS");
+            const uint32_t log_regions = lg - glog;
+            const uint64_t nregions    = 1ull << log_regions;
+            const uint64_t slots       = staged.slots;
+            const uint32_t blocks      = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(512, (slots + 16383) / 16384)));
+            const uint64_t chunk       = (slots + blocks - 1) / blocks;
+            ht->cap     = 1ull << lg;
+            ht->shift   = 64 - lg;
+            ht->rmask   = (1ull << glog) - 1;
+            ht->slots   = DevBuf(&ctx_.heap(), ht->cap * 2 * sizeof(uint64_t));
+            ht->payload = DevBuf(&ctx_.heap(), std::max<uint64_t>(1, n * (ht->words - 1)) * sizeof(uint64_t));
+            DevBuf bh(&ctx_.heap(), nregions * blocks * sizeof(uint32_t));
+            DevBuf scratch(&ctx_.heap(), scan_scratch_words(nregions * blocks) * sizeof(uint32_t));
+            DevBuf total(&ctx_.heap(), sizeof(uint64_t));
+            DevBuf parts(&ctx_.heap(), n * ht->words * sizeof(uint64_t));
+            if (bloom) CGJ_HIP(launch_fill_u64(bloom, 1ull << ht->bloom_log, 0, ctx_.stream()));
+            CGJ_HIP(launch_group_build(staged.stage.get<uint64_t>(), slots, n, ht->words, parts.get<uint64_t>(),
+                                       bh.get<uint32_t>(), blocks, chunk, scratch.get<uint32_t>(), total.get<uint64_t>(),
+                                       ht->slots.get<uint64_t>(), ht->payload.get<uint64_t>(), lg, log_regions, bloom,
+                                       ht->bloom_log, flags.get<uint32_t>(), threads, ctx_.stream()));
+            uint32_t f[4];
+            read_flags(f);
+            if (f[3]) throw std::runtime_error("grouped hash table: a key repeats 2^24 or more times");
+            if (!f[1]) break;
+            if (attempt >= 3) throw std::runtime_error("grouped hash table: regions overflow");
+            ++retries;
+            ++lg;
+        }
+        ht->grouped = true;
+        ht->unique  = false;
     }
     CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
     CGJ_HIP(hipEventSynchronize(ev1_));
@@ -721,10 +785,13 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         if (ks == slot.end()) throw std::runtime_error("probe key is not available in the pipeline");
         P.ht        = ht->slots.get<uint64_t>();
         P.shift     = ht->shift;
-        P.words     = ht->words;
+        P.words     = ht->grouped ? 2 : ht->words;
         P.mask      = ht->rmask;
         P.key_slot  = ks->second;
         P.unique    = ht->unique ? 1 : 0;
+        P.grouped   = ht->grouped ? 1 : 0;
+        P.pwords    = ht->words - 1;
+        P.payload   = ht->grouped ? ht->payload.get<uint64_t>() : nullptr;
         P.bloom     = ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr;
         P.bloom_log = ht->bloom_log;
         P.early     = ks->second < scanned ? 1 : 0;

@@ -156,6 +156,11 @@ struct Options {
     // allow one or two workgroups per CU, so wider ones keep more loads in
     // flight (c2 build 4.8 -> 3.9 ms at 1024).
     uint32_t build_threads         = 1024;
+    // Grouped tables: an insert that passes this many slots holding its own
+    // key rebuilds the table grouped by key (0 = never group);
+    // `force_grouped` groups every table (tests).
+    uint32_t group_walk            = 32;
+    bool     force_grouped         = false;
 };
 
 // Tunables of one batched join.

@@ -245,8 +245,11 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // hash-table insert (open addressing, linear probing, entries [key, attrs...])
 // ---------------------------------------------------------------------------
+// flags[0] = duplicate keys seen, flags[2] = an insert passed `walk_cap`
+// slots holding its own key (0 = no cap) and stopped: the table is then
+// incomplete and gets rebuilt with grouped duplicates.
 __global__ void k_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,
-                            uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* dup_flag) {
+                            uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* flags, uint32_t walk_cap) {
     uint64_t i      = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (; i < n; i += stride) {
@@ -255,20 +258,30 @@ __global__ void k_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, u
         if (key == kEmpty) continue; // unused staging slot
         uint64_t        h   = hash_key(key);
         uint64_t        pos = h >> shift;
-        bool            dup = false;
-        for (;;) {
+        bool            dup = false, stop = false;
+        for (uint32_t same = 0;;) {
             unsigned long long prev =
                 atomicCAS(reinterpret_cast<unsigned long long*>(ht + pos * words), kEmpty, key);
             if (prev == kEmpty) break;
-            dup |= prev == key;
+            if (prev == key) {
+                dup = true;
+                if (walk_cap && ++same >= walk_cap) {
+                    stop = true;
+                    break;
+                }
+            }
             pos = (pos + 1) & mask;
+        }
+        if (dup) flags[0] = 1;
+        if (stop) {
+            flags[2] = 1;
+            continue;
         }
         for (uint32_t w = 1; w < words; ++w) ht[pos * words + w] = t[w];
         if (bloom) {
             uint64_t h2 = hash_bloom(h);
             atomicOr(reinterpret_cast<unsigned long long*>(bloom + bloom_word(h2, bloom_log)), bloom_bits(h2));
         }
-        if (dup) *dup_flag = 1;
     }
 }
 
@@ -293,7 +306,7 @@ __global__ __launch_bounds__(1024) void k_part_hist(const uint64_t* stage, uint6
     for (uint64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
         uint64_t key = stage[i * words];
         if (key == kEmpty) continue; // unused staging slot
-        atomicAdd(&hist[uint32_t(hash_key(key) >> rshift)], 1u);
+        atomicAdd(&hist[rshift >= 64 ? 0u : uint32_t(hash_key(key) >> rshift)], 1u);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nregions; i += blockDim.x) bh[uint64_t(i) * gridDim.x + blockIdx.x] = hist[i];
@@ -311,7 +324,7 @@ __global__ __launch_bounds__(1024) void k_part_scatter(const uint64_t* stage, ui
     for (uint64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
         const uint64_t* t = stage + i * words;
         if (t[0] == kEmpty) continue; // unused staging slot
-        uint32_t r   = uint32_t(hash_key(t[0]) >> rshift);
+        uint32_t r   = rshift >= 64 ? 0u : uint32_t(hash_key(t[0]) >> rshift);
         uint32_t pos = atomicAdd(&cur[r], 1u);
         uint64_t*       d   = out + uint64_t(pos) * words;
         for (uint32_t w = 0; w < words; ++w) d[w] = t[w];
@@ -320,11 +333,14 @@ __global__ __launch_bounds__(1024) void k_part_scatter(const uint64_t* stage, ui
 
 // One block per region: inserts the region's tuples into an LDS table of
 // `rslots` slots (linear probing inside the region), then stores the region.
-// flags[0] = duplicate keys seen, flags[1] = a region was too full.
+// flags[0] = duplicate keys seen, flags[1] = a region was too full,
+// flags[2] = an insert passed `walk_cap` slots holding its own key (0 = no
+// cap) and the region was abandoned (heavy duplicates: rebuilt grouped).
 __global__ __launch_bounds__(1024) void k_part_build(const uint64_t* parts, const uint32_t* offs, uint32_t blocks,
                                                     uint64_t n, uint32_t nregions, uint32_t words,
                                                     uint32_t rslots, uint32_t shift, uint64_t* ht,
-                                                    uint64_t* bloom, uint32_t bloom_log, uint32_t* flags) {
+                                                    uint64_t* bloom, uint32_t bloom_log, uint32_t* flags,
+                                                    uint32_t walk_cap) {
     extern __shared__ uint64_t tab[];
     uint32_t r     = blockIdx.x;
     uint64_t start = offs[uint64_t(r) * blocks];
@@ -337,20 +353,27 @@ __global__ __launch_bounds__(1024) void k_part_build(const uint64_t* parts, cons
     }
     for (uint32_t i = threadIdx.x; i < rslots * words; i += blockDim.x) tab[i] = kEmpty;
     __syncthreads();
-    bool     dup   = false;
+    bool     dup   = false, stop = false;
     uint32_t rmask = rslots - 1;
-    for (uint64_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+    for (uint64_t i = start + threadIdx.x; i < end && !stop; i += blockDim.x) {
         const uint64_t* t   = parts + i * words;
         uint64_t        key = t[0];
         uint64_t        h   = hash_key(key);
         uint32_t        pos = uint32_t(h >> shift) & rmask;
-        for (;;) {
+        for (uint32_t same = 0;;) {
             unsigned long long prev =
                 atomicCAS(reinterpret_cast<unsigned long long*>(tab + uint64_t(pos) * words), kEmpty, key);
             if (prev == kEmpty) break;
-            dup |= prev == key;
+            if (prev == key) {
+                dup = true;
+                if (walk_cap && ++same >= walk_cap) {
+                    stop = true;
+                    break;
+                }
+            }
             pos = (pos + 1) & rmask;
         }
+        if (stop) break;
         for (uint32_t w = 1; w < words; ++w) tab[uint64_t(pos) * words + w] = t[w];
         if (bloom) {
             uint64_t h2 = hash_bloom(h);
@@ -358,6 +381,7 @@ __global__ __launch_bounds__(1024) void k_part_build(const uint64_t* parts, cons
         }
     }
     if (dup) flags[0] = 1;
+    if (stop) flags[2] = 1;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < rslots * words; i += blockDim.x) dst[i] = tab[i];
 }
@@ -365,7 +389,8 @@ __global__ __launch_bounds__(1024) void k_part_build(const uint64_t* parts, cons
 hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real, uint32_t words, uint64_t* parts,
                              uint32_t* bh, uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch, uint64_t* total,
                              uint64_t* ht, uint32_t log_cap, uint32_t log_regions, uint64_t* bloom,
-                             uint32_t bloom_log, uint32_t* flags, uint32_t threads, hipStream_t s) {
+                             uint32_t bloom_log, uint32_t* flags, uint32_t threads, uint32_t walk_cap,
+                             hipStream_t s) {
     if (n == 0) return hipSuccess;
     // The LDS histograms and region tables allow one or two workgroups per CU:
     // wider workgroups keep more waves (memory requests) in flight.
@@ -394,7 +419,151 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
         }
     }
     k_part_build<<<dim3(nregions), dim3(threads), tab_lds, s>>>(parts, bh, blocks, n_real, nregions, words, rslots,
-                                                                64 - log_cap, ht, bloom, bloom_log, flags);
+                                                                64 - log_cap, ht, bloom, bloom_log, flags, walk_cap);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// grouped build: a table whose keys repeat many times keeps each distinct key
+// once, as [key, start << 24 | count], and the key's tuples (words 1.. of the
+// staged tuple) contiguously in a payload array. Probe sequences then stay
+// as short as with unique keys, and a probe that matches walks its group.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kGroupCountBits = 24;
+
+// Exclusive scan of cnt[0..n) into out[0..n) by one workgroup; `tmp` holds
+// blockDim.x words of LDS.
+__device__ inline void block_exclusive_scan(const uint32_t* cnt, uint32_t* out, uint32_t n, uint32_t* tmp) {
+    const uint32_t T = blockDim.x, t = threadIdx.x;
+    const uint32_t per = (n + T - 1) / T;
+    const uint32_t b0 = t * per, b1 = b0 + per < n ? b0 + per : n;
+    uint32_t       local = 0;
+    for (uint32_t j = b0; j < b1; ++j) local += cnt[j];
+    tmp[t] = local;
+    __syncthreads();
+    for (uint32_t d = 1; d < T; d <<= 1) { // Hillis-Steele inclusive scan
+        const uint32_t v = t >= d ? tmp[t - d] : 0u;
+        __syncthreads();
+        tmp[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = tmp[t] - local;
+    for (uint32_t j = b0; j < b1; ++j) {
+        const uint32_t c = cnt[j];
+        out[j]           = run;
+        run += c;
+    }
+    __syncthreads();
+}
+
+// One workgroup per region of `rslots` slots (tuples already in region order
+// in `parts`, region r starting at offs[r * blocks]). LDS: keys, counts and
+// cursors of the region's slots, plus the scan's scratch.
+// flags[1] = a region's distinct keys did not fit, flags[3] = a group
+// exceeded the count field.
+__global__ __launch_bounds__(1024) void k_group_build(const uint64_t* parts, const uint32_t* offs, uint32_t blocks,
+                                                     uint64_t n, uint32_t nregions, uint32_t words, uint32_t rslots,
+                                                     uint32_t shift, uint64_t* ht, uint64_t* payload,
+                                                     uint64_t* bloom, uint32_t bloom_log, uint32_t* flags) {
+    extern __shared__ uint64_t lds[];
+    uint64_t*      keys  = lds;
+    uint32_t*      cnt   = reinterpret_cast<uint32_t*>(lds + rslots);
+    uint32_t*      cur   = cnt + rslots;
+    uint32_t*      tmp   = cur + rslots;
+    const uint32_t r     = blockIdx.x;
+    const uint64_t start = offs[uint64_t(r) * blocks];
+    const uint64_t end   = r + 1 < nregions ? offs[uint64_t(r + 1) * blocks] : n;
+    const uint32_t rmask = rslots - 1;
+    const uint32_t pw    = words - 1;
+    for (uint32_t i = threadIdx.x; i < rslots; i += blockDim.x) {
+        keys[i] = kEmpty;
+        cnt[i]  = 0;
+    }
+    __syncthreads();
+    bool over = false;
+    for (uint64_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint64_t key = parts[i * words];
+        const uint64_t h   = hash_key(key);
+        uint32_t       pos = uint32_t(h >> shift) & rmask;
+        uint32_t       steps = 0;
+        for (;; ++steps) {
+            if (steps >= rslots) {
+                over = true;
+                break;
+            }
+            const unsigned long long prev =
+                atomicCAS(reinterpret_cast<unsigned long long*>(keys + pos), kEmpty, key);
+            if (prev == kEmpty || prev == key) {
+                atomicAdd(cnt + pos, 1u);
+                break;
+            }
+            pos = (pos + 1) & rmask;
+        }
+        if (bloom && steps < rslots) {
+            const uint64_t h2 = hash_bloom(h);
+            atomicOr(reinterpret_cast<unsigned long long*>(bloom + bloom_word(h2, bloom_log)), bloom_bits(h2));
+        }
+    }
+    if (over) flags[1] = 1;
+    __syncthreads();
+    block_exclusive_scan(cnt, cur, rslots, tmp);
+    uint64_t* dst = ht + uint64_t(r) * rslots * 2;
+    bool      big = false;
+    for (uint32_t i = threadIdx.x; i < rslots; i += blockDim.x) {
+        const uint64_t k = keys[i];
+        big              = big || cnt[i] >= (1u << kGroupCountBits);
+        dst[2 * uint64_t(i)]     = k;
+        dst[2 * uint64_t(i) + 1] = k == kEmpty ? 0ull : ((start + cur[i]) << kGroupCountBits) | cnt[i];
+    }
+    if (big) flags[3] = 1;
+    __syncthreads();
+    if (pw == 0 || over) return; // key-only tuples: the counts are the table
+    for (uint64_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint64_t* t   = parts + i * words;
+        const uint64_t  key = t[0];
+        uint32_t        pos = uint32_t(hash_key(key) >> shift) & rmask;
+        while (keys[pos] != key) pos = (pos + 1) & rmask; // inserted above: always found
+        const uint32_t q = atomicAdd(cur + pos, 1u);
+        uint64_t*      d = payload + (start + q) * pw;
+        for (uint32_t w = 0; w < pw; ++w) d[w] = t[1 + w];
+    }
+}
+
+size_t group_build_lds(uint32_t rslots, uint32_t threads) {
+    return size_t(rslots) * 16 + size_t(threads) * 4;
+}
+
+hipError_t launch_group_build(const uint64_t* stage, uint64_t n, uint64_t n_real, uint32_t words, uint64_t* parts,
+                              uint32_t* bh, uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch,
+                              uint64_t* total, uint64_t* ht, uint64_t* payload, uint32_t log_cap,
+                              uint32_t log_regions, uint64_t* bloom, uint32_t bloom_log, uint32_t* flags,
+                              uint32_t threads, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;
+    const uint32_t nregions = 1u << log_regions;
+    const uint32_t rslots   = 1u << (log_cap - log_regions);
+    const uint32_t rshift   = 64 - log_regions; // 64: one region (index 0)
+    const size_t   hist_lds = size_t(nregions) * sizeof(uint32_t);
+    k_part_hist<<<dim3(blocks), dim3(threads), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh);
+    hipError_t e = launch_exclusive_scan(bh, uint64_t(nregions) * blocks, scan_scratch, total, s);
+    if (e != hipSuccess) return e;
+    k_part_scatter<<<dim3(blocks), dim3(threads), hist_lds, s>>>(stage, n, words, rshift, nregions, chunk, bh,
+                                                                 parts);
+    const size_t lds = group_build_lds(rslots, threads);
+    if (lds > 65536) {
+        constexpr int              kMaxDev = 16;
+        static std::atomic<size_t> lds_set[kMaxDev];
+        int                        dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev ||
+            lds_set[dev].load(std::memory_order_relaxed) < lds) {
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_group_build),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if (e != hipSuccess) return e;
+            if (dev >= 0 && dev < kMaxDev) lds_set[dev].store(lds, std::memory_order_relaxed);
+        }
+    }
+    k_group_build<<<dim3(nregions), dim3(threads), lds, s>>>(parts, bh, blocks, n_real, nregions, words, rslots,
+                                                             64 - log_cap, ht, payload, bloom, bloom_log, flags);
     return hipGetLastError();
 }
 
@@ -460,12 +629,12 @@ hipError_t launch_radix_partition(const RadixArgs& a, uint32_t wgs, uint32_t* sc
 }
 
 hipError_t launch_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,
-                            uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* dup_flag,
+                            uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* flags, uint32_t walk_cap,
                             hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 32768) blocks = 32768;
-    k_ht_insert<<<dim3(blocks), dim3(256), 0, s>>>(stage, n, words, ht, shift, mask, bloom, bloom_log, dup_flag);
+    k_ht_insert<<<dim3(blocks), dim3(256), 0, s>>>(stage, n, words, ht, shift, mask, bloom, bloom_log, flags, walk_cap);
     return hipGetLastError();
 }
 
@@ -830,14 +999,19 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                 }
             }
 
-            // depth-first enumeration of probe matches, one tuple per lane per round
+            // depth-first enumeration of probe matches, one tuple per lane per
+            // round. Per level: pos = next slot to inspect, or (grouped table,
+            // rem > 0) the next payload row of the matched key's group.
             uint32_t level = 0;
             uint64_t pos[kMaxProbes];
             uint64_t pkey[kMaxProbes];
+            uint32_t rem[kMaxProbes];
+            uint64_t mult = 1; // tuples this lane's match stands for (count target)
 #pragma unroll
             for (int k = 0; k < kMaxProbes; ++k) {
                 pos[k]  = ~0ull;
                 pkey[k] = kNull;
+                rem[k]  = 0;
             }
             if (alive && a.nprobes > 0) {
                 pkey[0] = sget(s, a.probes[0].key_slot);
@@ -856,7 +1030,39 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                             const ProbeSpec& P     = a.probes[k];
                             bool             found = false;
                             uint64_t         p     = pos[k];
-                            if (p != ~0ull) {
+                            if (P.grouped) {
+                                if (rem[k] == 0 && p != ~0ull) {
+                                    // the key's slot (keys are unique in a grouped table)
+                                    for (;;) {
+                                        const uint64_t* e  = P.ht + 2 * p;
+                                        const uint64_t  kw = e[0];
+                                        if (kw == kEmpty) {
+                                            p = ~0ull;
+                                            break;
+                                        }
+                                        if (kw == pkey[k]) {
+                                            const uint64_t g = e[1];
+                                            p                = g >> kGroupCountBits;
+                                            rem[k]           = uint32_t(g & ((1u << kGroupCountBits) - 1));
+                                            break;
+                                        }
+                                        p = (p & ~P.mask) | ((p + 1) & P.mask);
+                                    }
+                                }
+                                // next row of the group that passes the residual equalities
+                                while (rem[k] > 0) {
+                                    const uint64_t* t  = P.payload + p * P.pwords;
+                                    bool            ok = true;
+                                    for (uint32_t c = 0; c < P.nchecks; ++c)
+                                        ok = ok && t[P.check_word[c] - 1] == sget(s, P.check_slot[c]);
+                                    if (ok) {
+                                        found = true;
+                                        break;
+                                    }
+                                    ++p;
+                                    --rem[k];
+                                }
+                            } else if (p != ~0ull) {
                                 for (;;) {
                                     const uint64_t* e  = P.ht + p * P.words;
                                     uint64_t        kw = e[0];
@@ -873,10 +1079,22 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                                     p = (p & ~P.mask) | ((p + 1) & P.mask);
                                 }
                             }
-                            if (found) {
+                            if (found && P.grouped) {
+                                const uint64_t* t = P.payload + p * P.pwords;
+                                for (uint32_t u = 0; u < P.nattrs; ++u) sset(s, P.attr_slot[u], t[P.attr_word[u] - 1]);
+                                if (a.target == TARGET_COUNT && P.nchecks == 0 && uint32_t(k) + 1 == a.nprobes) {
+                                    mult   = rem[k]; // the rest of the group, counted at once
+                                    rem[k] = 0;
+                                } else {
+                                    --rem[k];
+                                }
+                                pos[k] = rem[k] ? p + 1 : ~0ull;
+                            } else if (found) {
                                 const uint64_t* e = P.ht + p * P.words;
                                 for (uint32_t t = 0; t < P.nattrs; ++t) sset(s, P.attr_slot[t], e[P.attr_word[t]]);
                                 pos[k] = P.unique ? ~0ull : ((p & ~P.mask) | ((p + 1) & P.mask));
+                            }
+                            if (found) {
                                 if (uint32_t(k) + 1 == a.nprobes) {
                                     have = true;
                                 } else if (k + 1 < kMaxProbes) {
@@ -914,7 +1132,10 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                         stage_put(a, ws, have, rank, cnt, words, a.stage_words);
                         local_count += cnt;
                     } else if (a.target == TARGET_COUNT) {
-                        local_count += cnt;
+                        uint64_t m = have ? mult : 0;
+#pragma unroll
+                        for (int d = 32; d >= 1; d >>= 1) m += __shfl_xor(m, d, 64);
+                        local_count += m;
                     } else {
                         local_count += cnt;
                         for (uint32_t c = 0; c < a.nout; ++c) {
@@ -930,6 +1151,7 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                         }
                     }
                 }
+                mult = 1;
                 if (!__ballot(alive)) break;
             }
         }
@@ -1482,7 +1704,8 @@ hipError_t launch_batch(const BatchJob* jobs, const BatchTask* tasks, uint64_t n
 // the key's), and the target is a count or at most four fixed-width columns.
 bool probe1_applicable(const PipeArgs& a, uint32_t slots) {
     (void)slots;
-    if (a.nprobes != 1 || a.probes[0].nchecks != 0 || a.nscan > uint32_t(kMaxScan)) return false;
+    if (a.nprobes != 1 || a.probes[0].nchecks != 0 || a.probes[0].grouped || a.nscan > uint32_t(kMaxScan))
+        return false;
     const ProbeSpec& P = a.probes[0];
     if (P.key_slot >= a.nscan || (a.not_null & ~(1u << P.key_slot)) != 0) return false;
     if (a.target == TARGET_COUNT) return true;

@@ -30,6 +30,12 @@ struct ProbeSpec {
     const uint64_t* bloom;     // blocked bloom filter (nullptr = none)
     uint32_t        bloom_log; // log2(words)
     uint32_t        early;     // key is a scanned column: filter before probing
+    // Grouped table: ht holds 2-word slots [key, start << 24 | count] and the
+    // matching tuples' words 1.. (attr_word / check_word - 1) are payload rows
+    // start .. start + count - 1 of `pwords` words.
+    uint32_t        grouped;
+    uint32_t        pwords;
+    const uint64_t* payload;
 };
 
 struct OutSpec {
@@ -214,8 +220,10 @@ size_t     prepare_scratch_words(uint64_t npages);
 hipError_t launch_decode(const ScanCol& c, uint64_t nrows, uint64_t* out, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, uint64_t n16, hipStream_t s);
+// flags[0] = duplicate keys, flags[2] = an insert passed walk_cap slots
+// holding its own key (0 = no cap) and stopped (the table is incomplete).
 hipError_t launch_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,
-                            uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* dup_flag,
+                            uint64_t mask, uint64_t* bloom, uint32_t bloom_log, uint32_t* flags, uint32_t walk_cap,
                             hipStream_t s);
 size_t     scan_scratch_words(uint64_t n);
 hipError_t launch_exclusive_scan(uint32_t* data, uint64_t n, uint32_t* scratch, uint64_t* total, hipStream_t s);
@@ -227,7 +235,18 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
                              uint32_t* bh,
                              uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch, uint64_t* total,
                              uint64_t* ht, uint32_t log_cap, uint32_t log_regions, uint64_t* bloom,
-                             uint32_t bloom_log, uint32_t* flags, uint32_t threads, hipStream_t s);
+                             uint32_t bloom_log, uint32_t* flags, uint32_t threads, uint32_t walk_cap,
+                             hipStream_t s);
+// Grouped table of 2^log_cap two-word slots [key, start << 24 | count] in
+// 2^log_regions regions, with the key's tuples (staged words 1..) in `payload`
+// grouped by key; same partition buffers as launch_part_build. flags[1] = a
+// region's distinct keys overflowed it, flags[3] = a group exceeded 2^24 - 1.
+hipError_t launch_group_build(const uint64_t* stage, uint64_t n, uint64_t n_real, uint32_t words, uint64_t* parts,
+                              uint32_t* bh, uint32_t blocks, uint64_t chunk, uint32_t* scan_scratch,
+                              uint64_t* total, uint64_t* ht, uint64_t* payload, uint32_t log_cap,
+                              uint32_t log_regions, uint64_t* bloom, uint32_t bloom_log, uint32_t* flags,
+                              uint32_t threads, hipStream_t s);
+size_t     group_build_lds(uint32_t rslots, uint32_t threads);
 // Launches the fused pipeline. `slots` = tuple width (selects the kernel variant).
 hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s);
 // Lean variant for one probe without residual predicates and a count or

@@ -136,7 +136,10 @@ typedef struct {
  *   "aligned_pairs": the lean kernel's probe steps read 2-slot groups aligned to two
  *       slots (1, default) or the next two slots (0);
  *   "build_threads": threads per workgroup of the partitioned build kernels (256, 512,
- *       1024 = default). */
+ *       1024 = default);
+ *   "group_walk": a hash-table insert that passes this many slots holding its own key
+ *       (a heavily repeated key) rebuilds the table grouped by key (default 32; 0 = never);
+ *   "force_grouped": build every hash table grouped by key (1) or only when needed (0). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */

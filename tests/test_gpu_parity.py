@@ -48,6 +48,89 @@ def test_random_plans_vs_oracle(ctx, seed):
     assert got == exp
 
 
+@pytest.fixture
+def grouped_ctx(ctx):
+    """The context with every hash table built grouped by key."""
+    ctx.set_option("force_grouped", 1)
+    yield ctx
+    ctx.set_option("force_grouped", 0)
+
+
+@pytest.mark.parametrize("seed", range(0, 60, 3))
+def test_random_plans_grouped_tables_vs_oracle(grouped_ctx, seed):
+    """The random plans again, every hash table grouped by key (one slot per
+    distinct key, the key's tuples in a payload array): residual equalities,
+    probe chains, counts and every output type."""
+    rng = random.Random(1000 + seed)
+    ntables = rng.randint(1, 6)
+    max_rows = rng.choice([5, 50, 400, 3000])
+    key_type = rng.choice([INT32, INT64])
+    domain = max(rng.choice([4, 30, 1000]), max_rows if ntables >= 3 else max_rows // 8, 4)
+    plan = random_plan(rng, ntables=ntables, max_rows=max_rows, key_type=key_type, domain=domain,
+                       long_strings=seed % 7 == 0)
+    say(f"grouped random plan seed={seed} tables={len(plan.inputs)}")
+    types = [t for _, t in plan.nodes[plan.root].output_attrs]
+    got = check_table(grouped_ctx.execute(plan), types)
+    exp = check_table(oracle.execute(plan), types)
+    assert got == exp
+    tables = [grouped_ctx.upload(t) for t in plan.inputs]
+    assert grouped_ctx.count_device(plan, tables) == len(exp)
+
+
+def _hot_key_plan(n_build, n_probe, hot, seed):
+    """Build side of `n_build` rows where key 7 takes a `hot` share of the
+    rows; probe side with a uniform key over the same domain."""
+    rng = np.random.default_rng(seed)
+    domain = 5000
+    bk = rng.integers(0, domain, n_build).astype(np.int64)
+    bk[rng.random(n_build) < hot] = 7
+    bv = rng.integers(0, 1 << 30, n_build)
+    pk = rng.integers(0, domain, n_probe).astype(np.int64)
+    pv = rng.integers(0, 1 << 30, n_probe)
+    # the planner hashes the smaller input: callers make it the build side
+    return _join_plan([(INT64, bk, None), (INT32, bv, None)], [(INT64, pk, None), (INT32, pv, None)],
+                      0, 0, [(1, INT32), (3, INT32)])
+
+
+@pytest.mark.parametrize("n_build,n_probe,hot", [(40_000, 300_000, 0.5), (200_000, 400_000, 0.3)])
+def test_heavily_repeated_build_key_takes_grouped_table(ctx, n_build, n_probe, hot):
+    """A key repeated tens of thousands of times on the (smaller, hashed)
+    build side, below and above the partitioned build's size threshold: the
+    plain insert stops at the walk cap and the table is rebuilt grouped; same
+    rows as the oracle and an exact count (the count target adds whole groups
+    at once)."""
+    plan = _hot_key_plan(n_build, n_probe, hot, 5)
+    say(f"hot build key {n_build} x {n_probe}, share {hot}")
+    got = ctx.execute(plan)
+    st = ctx.stats()
+    exp = oracle.execute(plan)
+    assert got.num_rows == exp.num_rows
+    assert st["retries"] >= 1, st  # the plain build was abandoned
+    for a, b_ in zip(sorted_rows(slots_of(got)), sorted_rows(slots_of(exp))):
+        assert np.array_equal(a, b_)
+    tables = [ctx.upload(t) for t in plan.inputs]
+    assert ctx.count_device(plan, tables) == exp.num_rows
+
+
+def test_grouped_and_plain_tables_agree_on_a_star_plan(ctx):
+    """A c5 star plan (foreign keys with Zipf skew up to s = 1.5) run with
+    grouped tables and with plain ones: identical rows."""
+    from gpujoin import workloads as wl
+    from gpujoin import estimation as est
+    ps = wl.c5(scale=0.002, nplans=40)
+    pp = next(p for p in ps.plans if p.shape == "star" and len(p.tables) >= 5)
+    tabs = [ctx.generate(ps.seed, ps.pool[t].rows, ps.pool[t].columns) for t in pp.tables]
+    types = [t for _, t in pp.plan.nodes[pp.plan.root].output_attrs]
+    say(f"star plan over {len(pp.tables)} tables, grouped vs plain")
+    plain = check_table(ctx.execute_device(pp.plan, tabs).download(), types)
+    ctx.set_option("force_grouped", 1)
+    try:
+        grouped = check_table(ctx.execute_device(pp.plan, tabs).download(), types)
+    finally:
+        ctx.set_option("force_grouped", 0)
+    assert plain == grouped
+
+
 def slots_of(table):
     """Fixed-width result columns as uint64 slot arrays (NULL = all ones)."""
     out = []
