@@ -430,6 +430,7 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
 // as short as with unique keys, and a probe that matches walks its group.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kGroupCountBits = 24;
+constexpr uint32_t kGroupPosBits   = 64 - kGroupCountBits; // payload row index bits
 
 // Exclusive scan of cnt[0..n) into out[0..n) by one workgroup; `tmp` holds
 // blockDim.x words of LDS.
@@ -1000,18 +1001,17 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
             }
 
             // depth-first enumeration of probe matches, one tuple per lane per
-            // round. Per level: pos = next slot to inspect, or (grouped table,
-            // rem > 0) the next payload row of the matched key's group.
+            // round. Per level, pos = ~0 (no more matches), the next slot to
+            // inspect, or (grouped table) the next payload row of the matched
+            // key's group in bits 0..39 with the rows left in bits 40..63.
             uint32_t level = 0;
             uint64_t pos[kMaxProbes];
             uint64_t pkey[kMaxProbes];
-            uint32_t rem[kMaxProbes];
-            uint64_t mult = 1; // tuples this lane's match stands for (count target)
+            uint32_t mult = 1; // tuples this lane's match stands for (count target)
 #pragma unroll
             for (int k = 0; k < kMaxProbes; ++k) {
                 pos[k]  = ~0ull;
                 pkey[k] = kNull;
-                rem[k]  = 0;
             }
             if (alive && a.nprobes > 0) {
                 pkey[0] = sget(s, a.probes[0].key_slot);
@@ -1030,27 +1030,27 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                             const ProbeSpec& P     = a.probes[k];
                             bool             found = false;
                             uint64_t         p     = pos[k];
-                            if (P.grouped) {
-                                if (rem[k] == 0 && p != ~0ull) {
+                            uint32_t         rem   = 0; // grouped: rows left in the group, from row p
+                            if (P.grouped && p != ~0ull) {
+                                rem = uint32_t(p >> kGroupPosBits);
+                                p &= (1ull << kGroupPosBits) - 1;
+                                if (rem == 0) {
                                     // the key's slot (keys are unique in a grouped table)
                                     for (;;) {
                                         const uint64_t* e  = P.ht + 2 * p;
                                         const uint64_t  kw = e[0];
-                                        if (kw == kEmpty) {
-                                            p = ~0ull;
-                                            break;
-                                        }
+                                        if (kw == kEmpty) break;
                                         if (kw == pkey[k]) {
                                             const uint64_t g = e[1];
                                             p                = g >> kGroupCountBits;
-                                            rem[k]           = uint32_t(g & ((1u << kGroupCountBits) - 1));
+                                            rem              = uint32_t(g & ((1u << kGroupCountBits) - 1));
                                             break;
                                         }
                                         p = (p & ~P.mask) | ((p + 1) & P.mask);
                                     }
                                 }
                                 // next row of the group that passes the residual equalities
-                                while (rem[k] > 0) {
+                                while (rem > 0) {
                                     const uint64_t* t  = P.payload + p * P.pwords;
                                     bool            ok = true;
                                     for (uint32_t c = 0; c < P.nchecks; ++c)
@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                                         break;
                                     }
                                     ++p;
-                                    --rem[k];
+                                    --rem;
                                 }
                             } else if (p != ~0ull) {
                                 for (;;) {
@@ -1083,12 +1083,12 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                                 const uint64_t* t = P.payload + p * P.pwords;
                                 for (uint32_t u = 0; u < P.nattrs; ++u) sset(s, P.attr_slot[u], t[P.attr_word[u] - 1]);
                                 if (a.target == TARGET_COUNT && P.nchecks == 0 && uint32_t(k) + 1 == a.nprobes) {
-                                    mult   = rem[k]; // the rest of the group, counted at once
-                                    rem[k] = 0;
+                                    mult = rem; // the rest of the group, counted at once
+                                    rem  = 0;
                                 } else {
-                                    --rem[k];
+                                    --rem;
                                 }
-                                pos[k] = rem[k] ? p + 1 : ~0ull;
+                                pos[k] = rem ? ((p + 1) | (uint64_t(rem) << kGroupPosBits)) : ~0ull;
                             } else if (found) {
                                 const uint64_t* e = P.ht + p * P.words;
                                 for (uint32_t t = 0; t < P.nattrs; ++t) sset(s, P.attr_slot[t], e[P.attr_word[t]]);
@@ -1132,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                         stage_put(a, ws, have, rank, cnt, words, a.stage_words);
                         local_count += cnt;
                     } else if (a.target == TARGET_COUNT) {
-                        uint64_t m = have ? mult : 0;
+                        uint64_t m = have ? uint64_t(mult) : 0;
 #pragma unroll
                         for (int d = 32; d >= 1; d >>= 1) m += __shfl_xor(m, d, 64);
                         local_count += m;
