@@ -406,12 +406,24 @@ private:
     void pipeline(Driver d, std::vector<ProbeStep> probes, Target& t);
     uint64_t launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms);
     uint64_t grid_waves(uint64_t nrows, bool xcd_order = false, uint32_t per_cu = 16) const;
+    // Rows per morsel: up to kMorselRows, fewer when that leaves fewer
+    // morsels than the grid's `per_cu` waves per CU (a mid-size input would
+    // otherwise run on a handful of waves).
+    uint32_t morsel_rows(uint64_t nrows, bool xcd_order, uint32_t per_cu) const {
+        if (xcd_order) return kMorselRows;
+        const uint64_t target = std::max<uint64_t>(1, uint64_t(ctx_.cu_count()) * per_cu);
+        const uint64_t m      = ((nrows + target - 1) / target + 63) / 64 * 64;
+        return uint32_t(std::min<uint64_t>(std::max<uint64_t>(m, 64), kMorselRows));
+    }
+    uint32_t per_cu_for(const PipeArgs& a, uint32_t nslots) const {
+        return use_lean(a, nslots) ? ctx_.options.lean_waves_per_cu : 16;
+    }
     // Whether the pipeline runs on the lean one-probe kernel, and its grid.
     bool     use_lean(const PipeArgs& a, uint32_t nslots) const {
         return ctx_.options.lean_kernel && probe1_applicable(a, nslots);
     }
     uint64_t waves_for(const PipeArgs& a, uint32_t nslots, uint64_t nrows) const {
-        return grid_waves(nrows, a.xcd_order != 0, use_lean(a, nslots) ? ctx_.options.lean_waves_per_cu : 16);
+        return grid_waves(nrows, a.xcd_order != 0, per_cu_for(a, nslots));
     }
     const Tuples* partition_driver(uint32_t rel, uint32_t key_eq, uint64_t table_bytes);
     static constexpr uint32_t kMorselRows = 4096;
@@ -452,7 +464,8 @@ void Executor::register_all_columns() {
 }
 
 uint64_t Executor::grid_waves(uint64_t nrows, bool xcd_order, uint32_t per_cu) const {
-    uint64_t morsels = (nrows + kMorselRows - 1) / kMorselRows;
+    const uint64_t mr      = morsel_rows(nrows, xcd_order, per_cu);
+    uint64_t       morsels = (nrows + mr - 1) / mr;
     uint64_t waves   = std::min<uint64_t>(std::max<uint64_t>(morsels, 1), uint64_t(ctx_.cu_count()) * per_cu);
     uint32_t per     = pipeline_waves_per_block() * (xcd_order ? 8 : 1);
     return (waves + per - 1) / per * per;
@@ -506,7 +519,7 @@ const Tuples* Executor::partition_driver(uint32_t rel, uint32_t key_eq, uint64_t
 
 uint64_t Executor::launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms) {
     a.counters    = counters_.get<PipeCounters>();
-    a.morsel_rows = kMorselRows;
+    a.morsel_rows = morsel_rows(nrows, a.xcd_order != 0, per_cu_for(a, nslots));
     a.nrows       = nrows;
     a.nmorsels    = (nrows + a.morsel_rows - 1) / a.morsel_rows;
     a.col_base    = col_base_.get<const uint8_t* const>();

@@ -194,38 +194,134 @@ void Mlp::predict_f32(const float* x, uint32_t n, float* out) {
 // ---------------------------------------------------------------------------
 // features
 // ---------------------------------------------------------------------------
-std::vector<const QueryStats::Edge*> active_edges(const Query& q, const QueryStats& st, uint64_t relset) {
-    std::vector<const QueryStats::Edge*> out;
+// Per-query structures for featurising many sets: the join classes with their
+// member relations (sorted) and their measured edges, the log sizes.
+SetFeaturizer::SetFeaturizer(const Query& q, const QueryStats& st) : st_(st) {
+    nrels_ = uint32_t(q.rels.size());
+    for (uint32_t r = 0; r < nrels_; ++r) lrows_.push_back(std::log10(std::max(st.rows[r], 1.0)));
+    for (uint32_t r = 0; r < nrels_; ++r) empty_ |= st.rows[r] <= 0 ? (1ull << r) : 0;
     for (uint32_t e = 0; e < q.eq_attrs.size(); ++e) {
         if (!q.is_join_eq(e)) continue;
-        std::vector<uint32_t> members;
-        for (uint32_t a : q.eq_attrs[e]) {
-            const uint32_t r = q.attrs[a].rel;
-            if ((relset >> r) & 1) members.push_back(r);
-        }
-        std::sort(members.begin(), members.end());
-        for (size_t i = 1; i < members.size(); ++i)
-            for (const auto& ed : st.edges)
-                if (ed.eq == e && ed.a == members[i - 1] && ed.b == members[i]) {
-                    out.push_back(&ed);
-                    break;
-                }
+        Cls c;
+        for (uint32_t a : q.eq_attrs[e]) c.members.push_back(q.attrs[a].rel);
+        std::sort(c.members.begin(), c.members.end());
+        for (uint32_t r : c.members) c.mask |= 1ull << r;
+        for (const auto& ed : st.edges)
+            if (ed.eq == e) c.edges.push_back(&ed);
+        classes_.push_back(std::move(c));
     }
-    return out;
 }
 
-static double log10_rows(double r) { return std::log10(std::max(r, 1.0)); }
-static double log10_sel(double s) { return std::log10(std::max(s, 1e-30)); }
+const QueryStats::Edge* SetFeaturizer::edge(const Cls& c, uint32_t a, uint32_t b) const {
+    for (const auto* ed : c.edges)
+        if (ed->a == a && ed->b == b) return ed;
+    return nullptr;
+}
+
+uint32_t SetFeaturizer::active(uint64_t set, const QueryStats::Edge** out) const {
+    uint32_t n = 0;
+    for (const Cls& c : classes_) {
+        const uint64_t m = c.mask & set;
+        if (!(m & (m - 1))) continue; // fewer than two members inside the set
+        if (c.members.size() == 2) {
+            if (!c.edges.empty()) out[n++] = c.edges[0];
+            continue;
+        }
+        // consecutive members inside the set, in relation order
+        int32_t prev = -1;
+        for (uint32_t r : c.members) {
+            if (!((set >> r) & 1)) continue;
+            if (prev >= 0)
+                if (const auto* ed = edge(c, uint32_t(prev), r)) out[n++] = ed;
+            prev = int32_t(r);
+        }
+    }
+    return n;
+}
+
+double SetFeaturizer::log10_independence(uint64_t set) const {
+    if (set & empty_) return -30.0; // an empty input empties the join
+    const QueryStats::Edge* ed[64];
+    double                  l = 0;
+    for (uint64_t t = set; t; t &= t - 1) l += lrows_[__builtin_ctzll(t)];
+    const uint32_t n = active(set, ed);
+    for (uint32_t i = 0; i < n; ++i) l += std::log10(std::max(ed[i]->sel, 1e-30));
+    return l;
+}
+
+void SetFeaturizer::featurize(uint64_t set, float* f) const {
+    std::fill(f, f + kFeatures, 0.f);
+    const QueryStats::Edge* ed[64];
+    const uint32_t          ne = active(set, ed);
+    double                  lr[64], ls[64];
+    uint32_t                nr = 0;
+    double                  sum_lr = 0, max_lr = -1e300, min_lr = 1e300;
+    for (uint64_t t = set; t; t &= t - 1) {
+        const double v = lrows_[__builtin_ctzll(t)];
+        lr[nr++]       = v;
+        sum_lr += v;
+        max_lr = std::max(max_lr, v);
+        min_lr = std::min(min_lr, v);
+    }
+    uint8_t degree[64] = {0};
+    double  sum_ls = 0, max_ls = -1e300, min_ls = 1e300, fan_big = 0, fan_small = 0;
+    for (uint32_t i = 0; i < ne; ++i) {
+        const double l  = std::log10(std::max(ed[i]->sel, 1e-30));
+        const double la = lrows_[ed[i]->a], lb = lrows_[ed[i]->b];
+        ls[i]           = l;
+        sum_ls += l;
+        max_ls = std::max(max_ls, l);
+        min_ls = std::min(min_ls, l);
+        fan_big += l + std::max(la, lb);   // log matches per row of the smaller side
+        fan_small += l + std::min(la, lb); // ... of the larger side
+        ++degree[ed[i]->a];
+        ++degree[ed[i]->b];
+    }
+    const double n = double(nr), e = double(ne);
+    f[0]           = float(n / 12.0);
+    f[1]           = float(e / 12.0);
+    if (nr) {
+        f[2] = float(sum_lr / 32.0);
+        f[3] = float(max_lr / 8.0);
+        f[4] = float(min_lr / 8.0);
+        f[5] = float(sum_lr / n / 8.0);
+    }
+    if (ne) {
+        f[6] = float(sum_ls / 32.0);
+        f[7] = float(max_ls / 8.0);
+        f[8] = float(min_ls / 8.0);
+        f[9] = float(sum_ls / e / 8.0);
+    }
+    const double indep = (set & empty_) ? -30.0 : sum_lr + sum_ls;
+    f[10]              = float(std::max(indep, -8.0) / 8.0); // independence estimate
+    f[11]              = nr ? float(max_lr / 8.0) : 0.f;     // max-of-inputs rule
+    uint32_t maxdeg = 0, leaves = 0;
+    for (uint64_t t = set; t; t &= t - 1) {
+        const uint32_t d = degree[__builtin_ctzll(t)];
+        maxdeg           = std::max(maxdeg, d);
+        leaves += d == 1 ? 1 : 0;
+    }
+    f[12] = maxdeg >= 3 ? 1.f : 0.f;
+    f[13] = float(maxdeg / 12.0);
+    f[14] = float(leaves / 12.0);
+    std::sort(lr, lr + nr, [](double x, double y) { return x > y; });
+    for (uint32_t i = 0; i < 3 && i < nr; ++i) f[15 + i] = float(lr[i] / 8.0);
+    std::sort(ls, ls + ne);
+    for (uint32_t i = 0; i < 3 && i < ne; ++i) f[18 + i] = float(ls[i] / 8.0);
+    f[21] = float(fan_big / 16.0);
+    f[22] = float(fan_small / 16.0);
+    f[23] = 1.f;
+}
+
+std::vector<const QueryStats::Edge*> active_edges(const Query& q, const QueryStats& st, uint64_t relset) {
+    SetFeaturizer           fz(q, st);
+    const QueryStats::Edge* ed[64];
+    const uint32_t          n = fz.active(relset, ed);
+    return std::vector<const QueryStats::Edge*>(ed, ed + n);
+}
 
 double log10_independence(const Query& q, const QueryStats& st, uint64_t relset) {
-    double l = 0;
-    for (uint32_t r = 0; r < q.rels.size(); ++r)
-        if ((relset >> r) & 1) {
-            if (st.rows[r] <= 0) return -30.0; // an empty input empties the join
-            l += log10_rows(st.rows[r]);
-        }
-    for (const auto* e : active_edges(q, st, relset)) l += log10_sel(e->sel);
-    return l;
+    return SetFeaturizer(q, st).log10_independence(relset);
 }
 
 double independence_base(const Query& q, const QueryStats& st, uint64_t relset) {
@@ -233,58 +329,7 @@ double independence_base(const Query& q, const QueryStats& st, uint64_t relset) 
 }
 
 void featurize(const Query& q, const QueryStats& st, uint64_t relset, float* f) {
-    std::fill(f, f + kFeatures, 0.f);
-    std::vector<uint32_t> rels;
-    for (uint32_t r = 0; r < q.rels.size(); ++r)
-        if ((relset >> r) & 1) rels.push_back(r);
-    std::vector<double> lr;
-    for (uint32_t r : rels) lr.push_back(log10_rows(st.rows[r]));
-    std::vector<double>   ls, fan_big, fan_small;
-    std::vector<uint32_t> degree(q.rels.size(), 0);
-    for (const auto* ed : active_edges(q, st, relset)) {
-        const double l  = log10_sel(ed->sel);
-        const double la = log10_rows(st.rows[ed->a]), lb = log10_rows(st.rows[ed->b]);
-        ls.push_back(l);
-        fan_big.push_back(l + std::max(la, lb));   // log matches per row of the smaller side
-        fan_small.push_back(l + std::min(la, lb)); // ... of the larger side
-        ++degree[ed->a];
-        ++degree[ed->b];
-    }
-    auto sum = [](const std::vector<double>& v) { return std::accumulate(v.begin(), v.end(), 0.0); };
-    const double n = double(rels.size()), e = double(ls.size());
-    f[0]           = float(n / 12.0);
-    f[1]           = float(e / 12.0);
-    if (!lr.empty()) {
-        f[2] = float(sum(lr) / 32.0);
-        f[3] = float(*std::max_element(lr.begin(), lr.end()) / 8.0);
-        f[4] = float(*std::min_element(lr.begin(), lr.end()) / 8.0);
-        f[5] = float(sum(lr) / n / 8.0);
-    }
-    if (!ls.empty()) {
-        f[6] = float(sum(ls) / 32.0);
-        f[7] = float(*std::max_element(ls.begin(), ls.end()) / 8.0);
-        f[8] = float(*std::min_element(ls.begin(), ls.end()) / 8.0);
-        f[9] = float(sum(ls) / e / 8.0);
-    }
-    f[10] = float(std::max(log10_independence(q, st, relset), -8.0) / 8.0); // independence estimate
-    f[11] = lr.empty() ? 0.f : float(*std::max_element(lr.begin(), lr.end()) / 8.0); // max-of-inputs rule
-    uint32_t maxdeg = 0, leaves = 0;
-    for (uint32_t r : rels) {
-        maxdeg = std::max(maxdeg, degree[r]);
-        leaves += degree[r] == 1 ? 1 : 0;
-    }
-    f[12] = maxdeg >= 3 ? 1.f : 0.f;
-    f[13] = float(maxdeg / 12.0);
-    f[14] = float(leaves / 12.0);
-    std::vector<double> srt = lr;
-    std::sort(srt.rbegin(), srt.rend());
-    for (uint32_t i = 0; i < 3 && i < srt.size(); ++i) f[15 + i] = float(srt[i] / 8.0);
-    std::vector<double> sls = ls;
-    std::sort(sls.begin(), sls.end());
-    for (uint32_t i = 0; i < 3 && i < sls.size(); ++i) f[18 + i] = float(sls[i] / 8.0);
-    f[21] = float(sum(fan_big) / 16.0);
-    f[22] = float(sum(fan_small) / 16.0);
-    f[23] = 1.f;
+    SetFeaturizer(q, st).featurize(relset, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -392,7 +437,8 @@ CardFn Estimator::model(Context& ctx, const Query& q, const std::vector<const De
     if (mode == EST_MAX || q.rels.size() < 2) return max_card_model(q, base);
     auto st = std::make_shared<QueryStats>(stats(ctx, q, in, rep));
     if (mode == EST_INDEPENDENCE) {
-        return [st, &q](uint64_t set) { return std::pow(10.0, log10_independence(q, *st, set)); };
+        auto fz = std::make_shared<SetFeaturizer>(q, *st);
+        return [st, fz](uint64_t set) { return std::pow(10.0, fz->log10_independence(set)); };
     }
     if (!mlp || mlp->in() != kFeatures) throw std::runtime_error("learned estimator without a network of matching width");
     const auto            t0 = std::chrono::steady_clock::now();
@@ -404,13 +450,14 @@ CardFn Estimator::model(Context& ctx, const Query& q, const std::vector<const De
             if (s & (s - 1)) multi.push_back(s);
         rep.sets = uint32_t(multi.size());
         std::vector<float> x(multi.size() * kFeatures), y(multi.size() * kMlpOut);
-        for (size_t i = 0; i < multi.size(); ++i) featurize(q, *st, multi[i], &x[i * kFeatures]);
+        const SetFeaturizer fz(q, *st);
+        for (size_t i = 0; i < multi.size(); ++i) fz.featurize(multi[i], &x[i * kFeatures]);
         mlp->predict(x.data(), uint32_t(multi.size()), y.data());
         // The network predicts a correction (mu) to the independence estimate
         // in log10 units, and its spread sigma = exp(s).
         const float max_s = std::log(std::max(max_sigma, 1e-6f));
         for (size_t i = 0; i < multi.size(); ++i) {
-            const double base_l = independence_base(q, *st, multi[i]);
+            const double base_l = std::max(fz.log10_independence(multi[i]), 0.0);
             double       l      = base_l + y[2 * i];
             if (!(y[2 * i + 1] <= max_s) || !std::isfinite(l)) {
                 l = base_l;
@@ -420,11 +467,12 @@ CardFn Estimator::model(Context& ctx, const Query& q, const std::vector<const De
         }
     }
     rep.ms_model += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return [st, est, base, &q](uint64_t set) {
+    auto fz = std::make_shared<SetFeaturizer>(q, *st);
+    return [st, est, base, fz](uint64_t set) {
         if (!(set & (set - 1))) return base[__builtin_ctzll(set)];
         auto it = est->find(set);
         if (it != est->end()) return it->second;
-        return std::pow(10.0, log10_independence(q, *st, set)); // more sets than the cap
+        return std::pow(10.0, fz->log10_independence(set)); // more sets than the cap
     };
 }
 

@@ -77,6 +77,30 @@ struct QueryStats {
     std::vector<Edge>   edges;
 };
 
+// Featurisation of many sets of one query: the join classes, their members
+// and measured edges, and the log sizes, prepared once. No allocation per set.
+class SetFeaturizer {
+public:
+    SetFeaturizer(const Query& q, const QueryStats& st);
+    // The edges the estimates use (see active_edges); returns their number (<= 64).
+    uint32_t active(uint64_t set, const QueryStats::Edge** out) const;
+    double   log10_independence(uint64_t set) const;
+    void     featurize(uint64_t set, float* out) const; // kFeatures floats
+
+private:
+    struct Cls {
+        std::vector<uint32_t>                members; // relations, sorted
+        uint64_t                             mask = 0;
+        std::vector<const QueryStats::Edge*> edges;
+    };
+    const QueryStats::Edge* edge(const Cls& c, uint32_t a, uint32_t b) const;
+    const QueryStats&       st_;
+    uint32_t                nrels_ = 0;
+    uint64_t                empty_ = 0; // relations without rows
+    std::vector<double>     lrows_;
+    std::vector<Cls>        classes_;
+};
+
 // Edges of `relset` the estimates use: per join class, the members inside the
 // set in relation order, consecutive pairs (a spanning chain of the class).
 std::vector<const QueryStats::Edge*> active_edges(const Query& q, const QueryStats& st, uint64_t relset);

@@ -896,7 +896,10 @@ __device__ inline void stage_put(const PipeArgs& a, WaveState& ws, bool have, ui
         uint64_t pos = rank < rem ? next + rank : base2 + (rank - rem);
         if (pos < a.stage_cap) {
             uint64_t* dst = a.stage + pos * a.stage_words;
-            for (uint32_t w = 0; w < nwords; ++w) dst[w] = words_src[w];
+            // fixed trip count: the caller's word array stays in registers
+#pragma unroll
+            for (int w = 0; w < kMaxWords; ++w)
+                if (uint32_t(w) < nwords) dst[w] = words_src[w];
         }
     }
     if (cnt > rem) {
@@ -977,28 +980,62 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
             for (int k = 0; k < NA; ++k) s[k] = kNull;
             bool alive = active;
             if (a.driver == DRIVER_PAGES) {
+                // Bloom cascade on keys that come straight from the scan, each
+                // key column decoded just before its filter and only while a
+                // lane of the block is alive; the other columns are decoded
+                // once the block still has a row after the cascade.
+                // Filters commute, so the cascade runs in column order; every
+                // column index is a compile-time constant (no runtime-indexed
+                // register array).
+                uint32_t bmask = 0; // scanned columns keying an early bloom filter
+                for (uint32_t p = 0; p < a.nprobes; ++p)
+                    if (a.probes[p].bloom && a.probes[p].early) bmask |= 1u << a.probes[p].key_slot;
+                uint32_t decoded = 0; // wave-uniform bitmask of decoded columns
 #pragma unroll
-                for (int c = 0; c < kMaxScan; ++c)
-                    if (uint32_t(c) < a.nscan && c < NA) s[c] = decode_block(a.scan[c], rb / 64, active);
+                for (int c = 0; c < kMaxScan; ++c) {
+                    if (c >= NA || !((bmask >> c) & 1u) || !__ballot(alive)) continue;
+                    s[c] = decode_block(a.scan[c], rb / 64, active);
+                    decoded |= 1u << c;
+                    if (s[c] == kNull) alive = false; // NULL keys never match
+                    for (uint32_t p = 0; p < a.nprobes; ++p) {
+                        const ProbeSpec& P = a.probes[p];
+                        if (P.bloom && P.early && P.key_slot == uint32_t(c) && alive) {
+                            const uint64_t h2 = hash_bloom(hash_key(s[c]));
+                            const uint64_t bb = bloom_bits(h2);
+                            alive             = (P.bloom[bloom_word(h2, P.bloom_log)] & bb) == bb;
+                        }
+                    }
+                }
+                if (__ballot(alive)) {
+#pragma unroll
+                    for (int c = 0; c < kMaxScan; ++c)
+                        if (uint32_t(c) < a.nscan && c < NA && !((decoded >> c) & 1u))
+                            s[c] = decode_block(a.scan[c], rb / 64, active);
+                }
             } else if (active) {
                 // materialised tuples: word 0 holds a join class, never NULL except in unused slots
                 alive = a.tuples[row * a.tuple_words] != kEmpty;
 #pragma unroll
                 for (int c = 0; c < kMaxScan; ++c)
                     if (uint32_t(c) < a.nscan && c < NA) s[c] = a.tuples[row * a.tuple_words + a.tuple_word[c]];
+                // bloom cascade on keys that come straight from the tuples
+                for (uint32_t p = 0; p < a.nprobes; ++p) {
+                    const ProbeSpec& P = a.probes[p];
+                    if (P.bloom && P.early && alive) {
+                        const uint64_t key = sget(s, P.key_slot);
+                        if (key == kNull) {
+                            alive = false;
+                        } else {
+                            uint64_t h2 = hash_bloom(hash_key(key));
+                            uint64_t bb = bloom_bits(h2);
+                            alive       = (P.bloom[bloom_word(h2, P.bloom_log)] & bb) == bb;
+                        }
+                    }
+                }
             }
 #pragma unroll
             for (int k = 0; k < NA; ++k)
                 if ((a.not_null >> k) & 1u) alive = alive && s[k] != kNull;
-            // bloom cascade on keys that come straight from the scan
-            for (uint32_t p = 0; p < a.nprobes; ++p) {
-                const ProbeSpec& P = a.probes[p];
-                if (P.bloom && P.early && alive) {
-                    uint64_t h2 = hash_bloom(hash_key(sget(s, P.key_slot)));
-                    uint64_t bb = bloom_bits(h2);
-                    alive       = (P.bloom[bloom_word(h2, P.bloom_log)] & bb) == bb;
-                }
-            }
 
             // depth-first enumeration of probe matches, one tuple per lane per
             // round. Per level, pos = ~0 (no more matches), the next slot to

@@ -65,6 +65,32 @@ def test_connected_sets_enumeration(ctx, small_pool):
         assert np.all(np.isfinite(feats))
 
 
+def test_independence_and_max_features(ctx, small_pool):
+    """Feature 11 x 8 is log10 of the largest input of a set; feature 10 x 8
+    the independence estimate: sum of log10 sizes plus log10 of each edge's
+    pairwise selectivity, here recomputed from exact counts of the two-member
+    sets (the plans are trees of binary foreign-key joins)."""
+    ps, tabs = small_pool
+    for pid in (0, 5, 11):
+        pp = ps.plans[pid]
+        ins = est.plan_inputs(tabs, pp)
+        rows = [t.num_rows for t in ins]
+        sel = {}
+        for a, _, b in pp.edges:
+            c = ctx.count_subset(pp.plan, ins, (1 << a) | (1 << b))
+            sel[(min(a, b), max(a, b))] = c / (rows[a] * rows[b])
+        sets, feats = ctx.plan_features(pp.plan, ins)
+        for s_, f in zip(sets, feats):
+            members = [r for r in range(len(rows)) if (int(s_) >> r) & 1]
+            lmax = max(np.log10(max(rows[r], 1)) for r in members)
+            assert abs(f[11] * 8 - lmax) < 1e-5
+            ind = sum(np.log10(max(rows[r], 1)) for r in members)
+            for (a, b), v in sel.items():
+                if a in members and b in members:
+                    ind += np.log10(max(v, 1e-30))
+            assert abs(f[10] * 8 - max(ind, -8.0)) < 1e-4, (pid, int(s_), f[10] * 8, ind)
+
+
 def test_subset_counts_vs_subplans_and_oracle(ctx, small_pool):
     ps, tabs = small_pool
     rng = random.Random(5)
