@@ -125,6 +125,28 @@ uint64_t cgj_dtable_num_pages(const cgj_dtable* t, uint32_t column) {
     return t->impl->cols[column].npages;
 }
 
+int cgj_execute_many(cgj_context* ctx, const cgj_plan* plans, uint32_t num_plans, cgj_dtable* const* inputs,
+                     const uint32_t* num_inputs, uint32_t workers, cgj_dresult** out) {
+    return guarded(ctx, [&] {
+        for (uint32_t i = 0; i < num_plans; ++i) out[i] = nullptr;
+        std::vector<const cgj_plan*>                  ps;
+        std::vector<std::vector<const cgj::DevTable*>> views(num_plans);
+        uint64_t                                      k = 0;
+        for (uint32_t i = 0; i < num_plans; ++i) {
+            ps.push_back(&plans[i]);
+            for (uint32_t j = 0; j < num_inputs[i]; ++j, ++k) {
+                if (!inputs[k]) throw std::runtime_error("null input table");
+                views[i].push_back(inputs[k]->impl.get());
+            }
+        }
+        auto res = ctx->impl->execute_many(ps, views, workers);
+        for (uint32_t i = 0; i < num_plans; ++i) {
+            out[i]       = new cgj_dresult;
+            out[i]->impl = std::move(res[i]);
+        }
+    });
+}
+
 int cgj_execute_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
                        cgj_dresult** out) {
     return guarded(ctx, [&] {

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <functional>
 #include <numeric>
+#include <thread>
 
 #include "estimator.hpp"
 #include "gen.hpp"
@@ -107,6 +108,7 @@ Context::Context(int device) : device_(device), est_(std::make_unique<Estimator>
 
 Context::~Context() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    workers_.clear();
     est_.reset(); // its network's buffers go back to the heap
     heap_->trim();
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -968,6 +970,38 @@ std::unique_ptr<DevResult> Executor::run(bool count_only) {
 
 std::unique_ptr<DevResult> Context::execute(const cgj_plan& plan, const std::vector<const DevTable*>& inputs) {
     return run_query(plan, inputs, false);
+}
+
+std::vector<std::unique_ptr<DevResult>> Context::execute_many(const std::vector<const cgj_plan*>& plans,
+                                                              const std::vector<std::vector<const DevTable*>>& inputs,
+                                                              uint32_t workers) {
+    if (plans.size() != inputs.size()) throw std::runtime_error("one input list per plan");
+    workers = std::max<uint32_t>(1, std::min<uint32_t>({workers, 16u, uint32_t(std::max<size_t>(plans.size(), 1))}));
+    while (workers_.size() < workers) {
+        workers_.push_back(std::make_unique<Context>(device_));
+        workers_.back()->options = options;
+    }
+    for (uint32_t w = 0; w < workers; ++w) workers_[w]->options = options;
+    CGJ_HIP(hipStreamSynchronize(stream_)); // inputs uploaded on this stream are complete
+    std::vector<std::unique_ptr<DevResult>> out(plans.size());
+    std::vector<std::string>                errors(workers);
+    std::atomic<size_t>                     next{0};
+    auto                                    run = [&](uint32_t w) {
+        try {
+            CGJ_HIP(hipSetDevice(device_));
+            for (size_t i; (i = next.fetch_add(1)) < plans.size();) out[i] = workers_[w]->execute(*plans[i], inputs[i]);
+        } catch (const std::exception& e) {
+            errors[w] = e.what();
+            next      = plans.size(); // stop the other workers early
+        }
+    };
+    std::vector<std::thread> threads;
+    for (uint32_t w = 1; w < workers; ++w) threads.emplace_back(run, w);
+    run(0);
+    for (auto& t : threads) t.join();
+    for (const auto& e : errors)
+        if (!e.empty()) throw std::runtime_error("execute_many: " + e);
+    return out;
 }
 
 uint64_t Context::count(const cgj_plan& plan, const std::vector<const DevTable*>& inputs) {

@@ -207,6 +207,14 @@ public:
     void prepare_column(DevColumn& c);
 
     std::unique_ptr<DevResult> execute(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
+    // Runs independent plans concurrently: `workers` host threads, each with
+    // its own worker context (HIP stream, device heap), take plans from a
+    // shared counter, so one plan's host round trips overlap another's
+    // kernels. Worker contexts persist across calls (warm heaps). Workers
+    // order joins with the max-of-inputs rule.
+    std::vector<std::unique_ptr<DevResult>> execute_many(const std::vector<const cgj_plan*>& plans,
+                                                         const std::vector<std::vector<const DevTable*>>& inputs,
+                                                         uint32_t workers);
     uint64_t count(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
     void     download(const DevResult& r, cgj_result& out);
     void     download_pool(const PoolView& v, cgj_result& out);
@@ -248,6 +256,7 @@ private:
     hipStream_t stream_ = nullptr;
     std::shared_ptr<DeviceHeap> heap_ = std::make_shared<DeviceHeap>();
     std::unique_ptr<Estimator>  est_;
+    std::vector<std::unique_ptr<Context>> workers_;
 };
 
 } // namespace cgj

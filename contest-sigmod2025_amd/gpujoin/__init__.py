@@ -182,6 +182,8 @@ SIGNATURES = {
     "cgj_mlp_predict": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, C.c_uint32, _F32P]),
     "cgj_mlp_predict_f32": (C.c_int, [C.c_void_p, C.c_void_p, _F32P, C.c_uint32, _F32P]),
     "cgj_mlp_free": (None, [C.c_void_p, C.c_void_p]),
+    "cgj_execute_many": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.c_uint32, C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_void_p)]),
     "cgj_num_features": (C.c_uint32, []),
     "cgj_set_estimator": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_float]),
     "cgj_plan_features": (C.c_int64, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
@@ -608,6 +610,19 @@ class Context:
         if lib().cgj_execute_device(self.handle, C.byref(m.plan), arr, len(tables), C.byref(out)) != 0:
             raise RuntimeError(self.error())
         return DeviceResult(self, out.value)
+
+    def execute_many(self, plans: Sequence[Plan], tables: Sequence[Sequence[DeviceTable]],
+                     workers: int = 4) -> List[DeviceResult]:
+        """Runs independent plans concurrently on `workers` streams; one result per plan."""
+        marshals = [_Marshal(p, []) for p in plans]
+        cplans = (CPlan * max(len(plans), 1))(*[m.plan for m in marshals])
+        flat = [t.handle for ts in tables for t in ts]
+        arr = (C.c_void_p * max(len(flat), 1))(*flat)
+        counts = (C.c_uint32 * max(len(plans), 1))(*[len(ts) for ts in tables])
+        out = (C.c_void_p * max(len(plans), 1))()
+        if lib().cgj_execute_many(self.handle, cplans, len(plans), arr, counts, workers, out) != 0:
+            raise RuntimeError(self.error())
+        return [DeviceResult(self, out[i]) for i in range(len(plans))]
 
     def count_device(self, plan: Plan, tables: Sequence[DeviceTable]) -> int:
         """Result cardinality of the plan over device tables (no pages written)."""

@@ -1,0 +1,73 @@
+"""GPU tests of cgj_execute_many: independent plans run concurrently on worker
+streams give the rows of the oracle and of one execute_device per plan."""
+import random
+import sys
+
+import pytest
+
+import gpujoin as gj
+import oracle
+from gpujoin.pages import INT32, INT64
+from gpujoin import workloads as wl
+from helpers import check_table, random_plan
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _plans(n, seed):
+    rng = random.Random(seed)
+    out = []
+    for i in range(n):
+        ntables = rng.choice([2, 2, 3, 4])
+        max_rows = rng.choice([50, 400, 3000, 20000])
+        domain = max(rng.choice([30, 1000]), max_rows // 4 if ntables >= 3 else 4)
+        out.append(random_plan(rng, ntables=ntables, max_rows=max_rows, key_type=rng.choice([INT32, INT64]),
+                               domain=domain))
+    return out
+
+
+@pytest.mark.parametrize("workers", [1, 4, 7])
+def test_execute_many_matches_oracle_and_single_runs(ctx, workers):
+    plans = _plans(30, 77 + workers)
+    tables = [[ctx.upload(t) for t in p.inputs] for p in plans]
+    say(f"execute_many: {len(plans)} plans on {workers} workers")
+    res = ctx.execute_many(plans, tables, workers=workers)
+    assert len(res) == len(plans)
+    for i, (p, r) in enumerate(zip(plans, res)):
+        types = [t for _, t in p.nodes[p.root].output_attrs]
+        got = check_table(r.download(), types)
+        assert got == check_table(ctx.execute_device(p, tables[i]).download(), types), i
+        assert got == check_table(oracle.execute(p), types), i
+
+
+def test_execute_many_on_generated_joins(ctx):
+    """Sixteen c4-shaped joins (generated in HBM) at 1/100 scale: row counts
+    equal those of one execute_device per join."""
+    bw = wl.c4(scale=0.01, njobs=16)
+    plans = [w.make_plan() for w in bw.jobs]
+    tables = [[ctx.generate(w.seed, t.rows, t.columns) for t in w.tables] for w in bw.jobs]
+    say("execute_many: 16 generated joins")
+    res = ctx.execute_many(plans, tables, workers=4)
+    for p, ts, r in zip(plans, tables, res):
+        assert r.num_rows == ctx.execute_device(p, ts).num_rows
+
+
+def test_execute_many_reports_errors(ctx):
+    """A plan whose scan type does not match its input fails the call with the
+    engine's message; the context stays usable."""
+    plans = _plans(6, 5)
+    tables = [[ctx.upload(t) for t in p.inputs] for p in plans]
+    bad = plans[2]
+    # declare a wrong type for a column of a non-empty scanned table
+    scan = next(n for n in bad.nodes if n.kind == gj.NODE_SCAN and bad.inputs[n.base_table_id].num_rows > 0)
+    col, ty = scan.output_attrs[0]
+    scan.output_attrs[0] = (col, INT64 if ty != INT64 else INT32)
+    with pytest.raises(RuntimeError):
+        ctx.execute_many(plans, tables, workers=3)
+    ok = ctx.execute_many(plans[:2], tables[:2], workers=2)
+    assert len(ok) == 2
