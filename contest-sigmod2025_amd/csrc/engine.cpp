@@ -418,6 +418,7 @@ private:
         return uint32_t(std::min<uint64_t>(std::max<uint64_t>(m, 64), kMorselRows));
     }
     uint32_t per_cu_for(const PipeArgs& a, uint32_t nslots) const {
+        if (a.grid_per_cu) return a.grid_per_cu;
         return use_lean(a, nslots) ? ctx_.options.lean_waves_per_cu : 16;
     }
     // Whether the pipeline runs on the lean one-probe kernel, and its grid.
@@ -887,12 +888,21 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         a.out[i].vc   = a.out[i].type == CGJ_VARCHAR ? a.nvarchar++ : 0;
         bytes_per_row += a.out[i].type == CGJ_INT32 ? 4.2 : (a.out[i].type == CGJ_VARCHAR ? 48.0 : 8.2);
     }
+    // Expected data pages; every wave also ends with a partly filled page per
+    // column and may hold a partly used reservation. So that those stay small
+    // beside the data, the grid gets about as many waves per CU as there are
+    // pages per column per CU (4..the default), and a reservation holds about
+    // one wave's share of the pages (1..kPageChunk).
+    const double   dp      = std::min(t.est, 4e9) * bytes_per_row / 7800.0;
+    const uint32_t def_pcu = per_cu_for(a, nslots);
+    a.grid_per_cu          = uint32_t(std::min<double>(
+        def_pcu, std::max<double>(4.0, dp / (std::max<uint32_t>(a.nout, 1) * double(ctx_.cu_count())))));
     uint64_t waves   = waves_for(a, nslots, nrows);
+    a.page_chunk     = uint32_t(std::min<double>(kPageChunk, std::max<double>(1.0, std::ceil(dp / double(waves)))));
     DevBuf   stage_s = DevBuf(&ctx_.heap(), std::max<uint64_t>(waves * a.nvarchar * kPageSize, 64));
     a.str_stage      = stage_s.get<uint8_t>();
-    // every wave may hold one partly used reservation of pool pages
-    uint64_t slack   = waves * 32 + 64;
-    uint64_t pages   = uint64_t(std::min(t.est, 4e9) * bytes_per_row / 7800.0) + waves * a.nout + slack;
+    uint64_t slack   = waves * a.page_chunk + 64;
+    uint64_t pages   = uint64_t(dp) + waves * a.nout + slack;
     for (int attempt = 0;; ++attempt) {
         res->pool    = DevBuf(&ctx_.heap(), pages * kPageSize + pages * 16 + 64);
         a.pool       = res->pool.get<uint8_t>();
@@ -982,6 +992,14 @@ std::vector<std::unique_ptr<DevResult>> Context::execute_many(const std::vector<
         workers_.back()->options = options;
     }
     for (uint32_t w = 0; w < workers; ++w) workers_[w]->options = options;
+    // With a model other than the max rule, every query is planned here with
+    // this context's estimator (statistics catalog, one batched inference per
+    // query); the workers only execute the planned join trees.
+    std::vector<PlannedQuery> planned;
+    if (est_->mode != EST_MAX) {
+        planned.reserve(plans.size());
+        for (size_t i = 0; i < plans.size(); ++i) planned.push_back(plan_query(*plans[i], inputs[i]));
+    }
     CGJ_HIP(hipStreamSynchronize(stream_)); // inputs uploaded on this stream are complete
     std::vector<std::unique_ptr<DevResult>> out(plans.size());
     std::vector<std::string>                errors(workers);
@@ -989,7 +1007,9 @@ std::vector<std::unique_ptr<DevResult>> Context::execute_many(const std::vector<
     auto                                    run = [&](uint32_t w) {
         try {
             CGJ_HIP(hipSetDevice(device_));
-            for (size_t i; (i = next.fetch_add(1)) < plans.size();) out[i] = workers_[w]->execute(*plans[i], inputs[i]);
+            for (size_t i; (i = next.fetch_add(1)) < plans.size();)
+                out[i] = planned.empty() ? workers_[w]->execute(*plans[i], inputs[i])
+                                         : workers_[w]->run_planned(planned[i], inputs[i], false);
         } catch (const std::exception& e) {
             errors[w] = e.what();
             next      = plans.size(); // stop the other workers early
@@ -1075,19 +1095,31 @@ std::vector<Context::PlannedJoin> Context::plan_tree(const cgj_plan& plan, const
     return out;
 }
 
+PlannedQuery Context::plan_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs) {
+    PlannedQuery pq;
+    pq.q = import_inputs(plan, inputs);
+    check_types(pq.q, inputs);
+    const auto t0 = std::chrono::steady_clock::now();
+    pq.tree       = plan_joins(pq.q, est_->model(*this, pq.q, inputs, pq.rep));
+    pq.ms_plan =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() - pq.rep.ms_stats;
+    return pq;
+}
+
 std::unique_ptr<DevResult> Context::run_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs,
                                               bool count_only) {
-    Query q = import_inputs(plan, inputs);
-    check_types(q, inputs);
-    const auto     t0 = std::chrono::steady_clock::now();
-    EstimateReport rep;
-    PlanTree       tree = plan_joins(q, est_->model(*this, q, inputs, rep));
-    const double   ms_plan =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() - rep.ms_stats;
-    Executor ex(*this, q, tree, inputs);
-    auto     res = ex.run(count_only);
+    PlannedQuery pq = plan_query(plan, inputs);
+    return run_planned(pq, inputs, count_only);
+}
+
+std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const std::vector<const DevTable*>& inputs,
+                                                bool count_only) {
+    const Query&          q   = pq.q;
+    const EstimateReport& rep = pq.rep;
+    Executor              ex(*this, q, pq.tree, inputs);
+    auto                  res = ex.run(count_only);
     stats                   = cgj_stats{};
-    stats.ms_plan           = ms_plan;
+    stats.ms_plan           = pq.ms_plan;
     stats.ms_stats          = rep.ms_stats;
     stats.est_sets          = rep.sets;
     stats.est_fallbacks     = rep.fallbacks;

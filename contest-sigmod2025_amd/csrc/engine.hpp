@@ -16,6 +16,7 @@
 
 #include "../../include/gpujoin.h"
 #include "kernels.hpp"
+#include "planner.hpp"
 
 namespace cgj {
 
@@ -129,7 +130,6 @@ struct PoolView {
 
 class Context;
 class Estimator;
-struct Query;
 
 // Tunables of one hash join, chosen per (build rows, probe rows, key width).
 struct JoinTuning {
@@ -210,8 +210,9 @@ public:
     // Runs independent plans concurrently: `workers` host threads, each with
     // its own worker context (HIP stream, device heap), take plans from a
     // shared counter, so one plan's host round trips overlap another's
-    // kernels. Worker contexts persist across calls (warm heaps). Workers
-    // order joins with the max-of-inputs rule.
+    // kernels. Worker contexts persist across calls (warm heaps). Queries are
+    // planned by this context's estimator (by the workers, with the
+    // max-of-inputs rule, when that is the model).
     std::vector<std::unique_ptr<DevResult>> execute_many(const std::vector<const cgj_plan*>& plans,
                                                          const std::vector<std::vector<const DevTable*>>& inputs,
                                                          uint32_t workers);
@@ -244,6 +245,12 @@ public:
     std::vector<PlannedJoin> plan_tree(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
     uint32_t   cu_count() const { return cus_; }
     uint64_t   lds_bytes() const { return lds_; } // LDS per workgroup
+
+    // A query imported and ordered by this context's estimator, and its
+    // execution (on any context of the device) as planned.
+    PlannedQuery               plan_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
+    std::unique_ptr<DevResult> run_planned(const PlannedQuery& pq, const std::vector<const DevTable*>& inputs,
+                                           bool count_only);
 
 private:
     void check_types(const Query& q, const std::vector<const DevTable*>& inputs) const;

@@ -122,14 +122,6 @@ bool connected_subsets(const Query& q, uint32_t limit, std::vector<uint64_t>& ou
 
 enum : uint32_t { EST_MAX = 0, EST_INDEPENDENCE = 1, EST_LEARNED = 2 };
 
-struct EstimateReport {
-    uint32_t sets       = 0; // connected sets estimated (two or more relations)
-    uint32_t fallbacks  = 0; // of them, estimated by the independence rule (low confidence)
-    uint32_t stat_joins = 0; // 2-way counts run for the statistics catalog
-    double   ms_stats   = 0; // time in those counts
-    double   ms_model   = 0; // featurisation + inference
-};
-
 // Cardinality model of the join ordering.
 //   EST_MAX          the reference's rule: a join is as large as its largest input
 //   EST_INDEPENDENCE relation sizes x exact pairwise selectivities

@@ -690,10 +690,10 @@ __device__ inline void page_open(const PipeArgs& a, WaveState& ws, OutState& st,
     uint64_t next = ws.next, end = ws.end;
     if (next == end) {
         unsigned long long base = 0;
-        if (lane_id() == 0) base = atomicAdd(&a.counters->pages, (unsigned long long)kPageChunk);
+        if (lane_id() == 0) base = atomicAdd(&a.counters->pages, (unsigned long long)a.page_chunk);
         base   = __shfl(base, 0, 64);
         next   = base;
-        ws.end = base + kPageChunk;
+        ws.end = base + a.page_chunk;
     }
     uint64_t idx = next;
     ws.next      = next + 1;
@@ -1801,7 +1801,7 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
     l.pool_pages = a.pool_pages;
     l.dir        = a.dir;
     l.counters   = a.counters;
-    l.page_chunk = kPageChunk;
+    l.page_chunk = a.page_chunk ? a.page_chunk : kPageChunk;
     l.aligned_pairs = aligned_pairs && P.words <= 2 ? 1u : 0u;
     // With the dynamic morsel schedule a grid larger than one resident wave
     // set only adds a tail: cap it at what the kernel's occupancy fits.

@@ -92,6 +92,8 @@ struct PipeArgs {
     const uint8_t* const* col_base;  // pages of column id (string handles)
     const uint32_t* col_npages;
     PipeCounters*   counters;
+    uint32_t        page_chunk;  // pool pages reserved per atomic (1..kPageChunk)
+    uint32_t        grid_per_cu; // host only: waves per CU of the launch (0 = default)
 };
 
 // ---- lean one-probe pipeline ----

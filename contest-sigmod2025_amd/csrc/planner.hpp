@@ -86,6 +86,23 @@ CardFn max_card_model(const Query& q, const std::vector<double>& base_card);
 // more than a streamed one.
 PlanTree plan_joins(const Query& q, const CardFn& card);
 
+// What the cardinality model did while a query was planned.
+struct EstimateReport {
+    uint32_t sets       = 0; // connected sets estimated (two or more relations)
+    uint32_t fallbacks  = 0; // of them, estimated by the independence rule (low confidence)
+    uint32_t stat_joins = 0; // 2-way counts run for the statistics catalog
+    double   ms_stats   = 0; // time in those counts
+    double   ms_model   = 0; // featurisation + inference
+};
+
+// A query and the join tree chosen for it.
+struct PlannedQuery {
+    Query          q;
+    PlanTree       tree;
+    EstimateReport rep;
+    double         ms_plan = 0; // estimation + join ordering, statistics counts excluded
+};
+
 // Per-join tuning chosen from (build rows, probe rows, key width).
 JoinTuning autotune_join(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes);
 
