@@ -992,32 +992,52 @@ std::vector<std::unique_ptr<DevResult>> Context::execute_many(const std::vector<
         workers_.back()->options = options;
     }
     for (uint32_t w = 0; w < workers; ++w) workers_[w]->options = options;
-    // With a model other than the max rule, every query is planned here with
+    // With a model other than the max rule, this thread plans every query with
     // this context's estimator (statistics catalog, one batched inference per
-    // query); the workers only execute the planned join trees.
-    std::vector<PlannedQuery> planned;
-    if (est_->mode != EST_MAX) {
-        planned.reserve(plans.size());
-        for (size_t i = 0; i < plans.size(); ++i) planned.push_back(plan_query(*plans[i], inputs[i]));
-    }
+    // query) and publishes each plan as it is done; the workers execute
+    // planned join trees as soon as theirs is ready, so planning overlaps
+    // execution. Under the max rule the workers plan for themselves.
+    const bool                         plan_here = est_->mode != EST_MAX;
+    std::vector<PlannedQuery>          planned(plan_here ? plans.size() : 0);
+    std::atomic<size_t>                ready{plan_here ? 0 : plans.size()};
+    std::atomic<bool>                  failed{false};
     CGJ_HIP(hipStreamSynchronize(stream_)); // inputs uploaded on this stream are complete
     std::vector<std::unique_ptr<DevResult>> out(plans.size());
-    std::vector<std::string>                errors(workers);
+    std::vector<std::string>                errors(workers + 1);
     std::atomic<size_t>                     next{0};
     auto                                    run = [&](uint32_t w) {
         try {
             CGJ_HIP(hipSetDevice(device_));
-            for (size_t i; (i = next.fetch_add(1)) < plans.size();)
-                out[i] = planned.empty() ? workers_[w]->execute(*plans[i], inputs[i])
-                                         : workers_[w]->run_planned(planned[i], inputs[i], false);
+            for (size_t i; (i = next.fetch_add(1)) < plans.size();) {
+                if (!plan_here) {
+                    out[i] = workers_[w]->execute(*plans[i], inputs[i]);
+                    continue;
+                }
+                while (ready.load(std::memory_order_acquire) <= i) {
+                    if (failed.load(std::memory_order_relaxed)) return;
+                    std::this_thread::yield();
+                }
+                out[i] = workers_[w]->run_planned(planned[i], inputs[i], false);
+            }
         } catch (const std::exception& e) {
             errors[w] = e.what();
+            failed    = true;
             next      = plans.size(); // stop the other workers early
         }
     };
     std::vector<std::thread> threads;
-    for (uint32_t w = 1; w < workers; ++w) threads.emplace_back(run, w);
-    run(0);
+    for (uint32_t w = 0; w < workers; ++w) threads.emplace_back(run, w);
+    if (plan_here) {
+        try {
+            for (size_t i = 0; i < plans.size() && !failed.load(std::memory_order_relaxed); ++i) {
+                planned[i] = plan_query(*plans[i], inputs[i]);
+                ready.store(i + 1, std::memory_order_release);
+            }
+        } catch (const std::exception& e) {
+            errors[workers] = e.what();
+            failed          = true;
+        }
+    }
     for (auto& t : threads) t.join();
     for (const auto& e : errors)
         if (!e.empty()) throw std::runtime_error("execute_many: " + e);

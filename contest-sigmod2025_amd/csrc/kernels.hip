@@ -321,6 +321,29 @@ __global__ __launch_bounds__(1024) void k_part_scatter(const uint64_t* stage, ui
     __syncthreads();
     uint64_t begin = uint64_t(blockIdx.x) * chunk;
     uint64_t end   = begin + chunk < n ? begin + chunk : n;
+    if (words == 2) {
+        // two-word tuples: one 16-byte load and store each, four per thread in
+        // flight before their position atomics
+        constexpr int   U  = 4;
+        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(stage);
+        ulonglong2*       dst = reinterpret_cast<ulonglong2*>(out);
+        for (uint64_t i0 = begin + threadIdx.x; i0 < end; i0 += U * uint64_t(blockDim.x)) {
+            ulonglong2 t[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t i = i0 + uint64_t(u) * blockDim.x;
+                t[u]             = i < end ? src[i] : make_ulonglong2(kEmpty, kEmpty);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (t[u].x == kEmpty) continue; // unused staging slot or past the chunk
+                const uint32_t r   = rshift >= 64 ? 0u : uint32_t(hash_key(t[u].x) >> rshift);
+                const uint32_t pos = atomicAdd(&cur[r], 1u);
+                dst[pos]           = t[u];
+            }
+        }
+        return;
+    }
     for (uint64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
         const uint64_t* t = stage + i * words;
         if (t[0] == kEmpty) continue; // unused staging slot

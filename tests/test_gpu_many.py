@@ -71,3 +71,29 @@ def test_execute_many_reports_errors(ctx):
         ctx.execute_many(plans, tables, workers=3)
     ok = ctx.execute_many(plans[:2], tables[:2], workers=2)
     assert len(ok) == 2
+
+
+@pytest.mark.parametrize("mode", [gj.EST_INDEPENDENCE, gj.EST_LEARNED])
+def test_execute_many_planned_here_matches_single_runs(ctx, mode):
+    """Under a model other than the max rule, the calling thread plans each
+    query while the workers execute the planned ones: the rows equal those
+    of one execute_device per plan under the max rule."""
+    from helpers import check_table as ct
+    ps = wl.c5(scale=0.002, nplans=24)
+    pool = [ctx.generate(ps.seed, t.rows, t.columns) for t in ps.pool]
+    plans = [pp.plan for pp in ps.plans]
+    tables = [[pool[t] for t in pp.tables] for pp in ps.plans]
+    expect = []
+    for p, ts in zip(plans, tables):
+        types = [t for _, t in p.nodes[p.root].output_attrs]
+        expect.append(ct(ctx.execute_device(p, ts).download(), types))
+    mlp = ctx.mlp(ctx.num_features(), seed=4) if mode == gj.EST_LEARNED else None
+    ctx.set_estimator(mode, mlp, max_sigma=100.0)
+    try:
+        say(f"execute_many planned here, estimator {mode}")
+        res = ctx.execute_many(plans, tables, workers=4)
+    finally:
+        ctx.set_estimator(gj.EST_MAX)
+    for p, r, e in zip(plans, res, expect):
+        types = [t for _, t in p.nodes[p.root].output_attrs]
+        assert ct(r.download(), types) == e
