@@ -272,6 +272,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "build_threads") options.build_threads = (v == 256 || v == 512) ? uint32_t(v) : 1024u;
     else if (name == "group_walk") options.group_walk = v >= 0 && v < 1e6 ? uint32_t(v) : 32u;
     else if (name == "force_grouped") options.force_grouped = v != 0;
+    else if (name == "scan_build") options.scan_build = v != 0;
     else return false;
     return true;
 }
@@ -405,6 +406,10 @@ private:
         return uint64_t(std::min(size_bound(p.set, p.card), 1.8e19));
     }
     const HashTable* make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est);
+    // Hash table on a plain scan of relation `rel`, partitioned straight from
+    // its pages (no staged tuples). False when that path does not apply or the
+    // table needs more than the plain partitioned build (the caller stages).
+    bool scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64_t probe_est, const HashTable*& out);
     void pipeline(Driver d, std::vector<ProbeStep> probes, Target& t);
     uint64_t launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms);
     uint64_t grid_waves(uint64_t nrows, bool xcd_order = false, uint32_t per_cu = 16) const;
@@ -712,12 +717,99 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
         probes.push_back(ProbeStep{ht, sh.front(), bn.set});
         cur |= bn.set;
     }
+    if (probes.empty() && ctx_.options.scan_build) {
+        const HashTable* ht = nullptr;
+        if (scan_build(uint32_t(leaf), eqs, probe_rows, ht)) return ht; // nullptr: empty build side
+    }
     Driver d;
     d.rel = leaf;
     d.set = 1ull << leaf;
     pipeline(d, probes, t);
     if (t.count == 0) return nullptr; // empty build side: the inner join is empty
     return make_table(t, eqs, probe_rows);
+}
+
+bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64_t probe_est, const HashTable*& out) {
+    out               = nullptr;
+    const uint64_t n0 = q_.rels[rel].rows; // upper bound on the tuples (NULL keys are dropped)
+    if (eqs.size() > size_t(kMaxWords) || n0 < 65536) return false;
+    const uint32_t words = uint32_t(eqs.size());
+    JoinTuning     tn    = ctx_.tune(n0, probe_est, 8);
+    uint32_t       lg    = log2_ceil(uint64_t(std::ceil(double(n0) / tn.max_load)));
+    if (lg < 6) lg = 6;
+    uint32_t rs_log = 12;
+    while (rs_log > 4 && (uint64_t(1) << rs_log) * words * 8 > 65536) --rs_log;
+    if (lg > rs_log + ctx_.options.build_regions_log) rs_log = lg - ctx_.options.build_regions_log;
+    if (!tn.partitioned || lg <= rs_log || (uint64_t(1) << rs_log) * words * 8 > ctx_.lds_bytes() ||
+        ctx_.options.force_grouped)
+        return false;
+    ScanBuildArgs sa{};
+    sa.words = words;
+    for (uint32_t w = 0; w < words; ++w) {
+        const uint32_t a = q_.attr_of(rel, eqs[w]);
+        const uint64_t c = q_.attrs[a].col;
+        sa.col[w]        = column(rel, c).view(col_id(rel, c));
+        if (q_.is_join_eq(eqs[w])) sa.notnull |= 1u << w;
+    }
+    sa.notnull |= 1u; // the key
+    sa.nrows                    = n0;
+    const uint32_t log_regions  = lg - rs_log;
+    sa.nregions                 = 1u << log_regions;
+    sa.rshift                   = 64 - log_regions;
+    const uint64_t nb           = (n0 + 63) / 64;
+    const uint32_t blocks       = uint32_t(std::min<uint64_t>(512, std::max<uint64_t>(1, nb / 64)));
+    sa.blocks_per_wg            = (nb + blocks - 1) / blocks;
+    auto ht    = std::make_unique<HashTable>();
+    ht->eqs    = eqs;
+    ht->words  = words;
+    ht->cap    = 1ull << lg;
+    ht->shift  = 64 - lg;
+    ht->rmask  = (1ull << rs_log) - 1;
+    ht->slots  = DevBuf(&ctx_.heap(), ht->cap * words * sizeof(uint64_t));
+    DevBuf flags(&ctx_.heap(), 4 * sizeof(uint32_t));
+    DevBuf bh(&ctx_.heap(), uint64_t(sa.nregions) * blocks * sizeof(uint32_t));
+    DevBuf scratch(&ctx_.heap(), scan_scratch_words(uint64_t(sa.nregions) * blocks) * sizeof(uint32_t));
+    DevBuf total(&ctx_.heap(), sizeof(uint64_t));
+    DevBuf parts(&ctx_.heap(), n0 * words * sizeof(uint64_t));
+    if (tn.bloom_bits) {
+        uint64_t bw   = std::max<uint64_t>(1, n0 * tn.bloom_bits / 64);
+        ht->bloom_log = log2_ceil(bw);
+        ht->bloom     = DevBuf(&ctx_.heap(), (1ull << ht->bloom_log) * sizeof(uint64_t));
+        ht->has_bloom = true;
+    }
+    uint64_t*   bloom = ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr;
+    hipStream_t s     = ctx_.stream();
+    CGJ_HIP(hipEventRecord(ev0_, s));
+    CGJ_HIP(hipMemsetAsync(flags.get(), 0, 4 * sizeof(uint32_t), s));
+    if (bloom) CGJ_HIP(launch_fill_u64(bloom, 1ull << ht->bloom_log, 0, s));
+    CGJ_HIP(launch_scan_partition(sa, blocks, bh.get<uint32_t>(), scratch.get<uint32_t>(), total.get<uint64_t>(),
+                                  parts.get<uint64_t>(), ctx_.options.build_threads, s));
+    uint64_t n = 0;
+    CGJ_HIP(hipMemcpyAsync(&n, total.get(), sizeof(n), hipMemcpyDeviceToHost, s));
+    CGJ_HIP(hipStreamSynchronize(s));
+    if (n == 0) return true; // empty build side
+    const uint32_t walk_cap = ctx_.options.group_walk;
+    CGJ_HIP(launch_region_build(parts.get<uint64_t>(), bh.get<uint32_t>(), blocks, n, words, ht->slots.get<uint64_t>(),
+                                lg, log_regions, bloom, ht->bloom_log, flags.get<uint32_t>(),
+                                ctx_.options.build_threads, walk_cap, s));
+    uint32_t f[4];
+    CGJ_HIP(hipMemcpyAsync(f, flags.get(), sizeof(f), hipMemcpyDeviceToHost, s));
+    CGJ_HIP(hipEventRecord(ev1_, s));
+    CGJ_HIP(hipEventSynchronize(ev1_));
+    float fms = 0;
+    CGJ_HIP(hipEventElapsedTime(&fms, ev0_, ev1_));
+    ms_build += fms;
+    if (f[1] || f[2]) { // a region overflowed or a key repeats heavily: the staged path handles it
+        ++retries;
+        return false;
+    }
+    ht->tuples = n;
+    ht->unique = f[0] == 0;
+    build_tuples += n;
+    build_bytes += n * words * sizeof(uint64_t);
+    tables_.push_back(std::move(ht));
+    out = tables_.back().get();
+    return true;
 }
 
 void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {

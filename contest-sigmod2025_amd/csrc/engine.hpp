@@ -161,6 +161,9 @@ struct Options {
     // `force_grouped` groups every table (tests).
     uint32_t group_walk            = 32;
     bool     force_grouped         = false;
+    // Hash tables on a plain scan are partitioned straight from the pages
+    // (no staged tuples).
+    bool     scan_build            = true;
 };
 
 // Tunables of one batched join.

@@ -447,6 +447,106 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
 }
 
 // ---------------------------------------------------------------------------
+// partitioning straight from pages (hash table on a plain scan)
+// ---------------------------------------------------------------------------
+// Decodes the tuple words of a 64-row block; returns active and no NULL in a
+// `notnull` word (the key, word 0, is always one).
+__device__ inline bool scan_build_row(const ScanBuildArgs& a, uint64_t blk, bool active, uint64_t (&v)[kMaxWords]) {
+    bool valid = active;
+#pragma unroll
+    for (int w = 0; w < kMaxWords; ++w) {
+        v[w] = kNull;
+        if (uint32_t(w) >= a.words) continue;
+        v[w] = decode_block(a.col[w], blk, active);
+        if ((a.notnull >> w) & 1u) valid = valid && v[w] != kNull;
+    }
+    return valid;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_hist(ScanBuildArgs a, uint32_t* bh) {
+    extern __shared__ uint32_t hist[];
+    for (uint32_t i = threadIdx.x; i < a.nregions; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    const uint64_t nb     = (a.nrows + 63) / 64;
+    const uint64_t b0     = uint64_t(blockIdx.x) * a.blocks_per_wg;
+    const uint64_t b1     = b0 + a.blocks_per_wg < nb ? b0 + a.blocks_per_wg : nb;
+    const uint32_t nwaves = blockDim.x / 64;
+    for (uint64_t b = b0 + wave_in_block(); b < b1; b += nwaves) {
+        const bool     active = b * 64 + lane_id() < a.nrows;
+        const uint64_t key    = decode_block(a.col[0], b, active);
+        bool           valid  = active && key != kNull;
+        for (uint32_t w = 1; w < a.words; ++w) // NULL checks of the other join columns
+            if ((a.notnull >> w) & 1u) valid = valid && decode_block(a.col[w], b, active) != kNull;
+        if (valid) atomicAdd(&hist[a.rshift >= 64 ? 0u : uint32_t(hash_key(key) >> a.rshift)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.nregions; i += blockDim.x) bh[uint64_t(i) * gridDim.x + blockIdx.x] = hist[i];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_scatter(ScanBuildArgs a, const uint32_t* offs, uint64_t* out) {
+    extern __shared__ uint32_t cur[];
+    for (uint32_t i = threadIdx.x; i < a.nregions; i += blockDim.x) cur[i] = offs[uint64_t(i) * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const uint64_t nb     = (a.nrows + 63) / 64;
+    const uint64_t b0     = uint64_t(blockIdx.x) * a.blocks_per_wg;
+    const uint64_t b1     = b0 + a.blocks_per_wg < nb ? b0 + a.blocks_per_wg : nb;
+    const uint32_t nwaves = blockDim.x / 64;
+    for (uint64_t b = b0 + wave_in_block(); b < b1; b += nwaves) {
+        uint64_t   v[kMaxWords];
+        const bool valid = scan_build_row(a, b, b * 64 + lane_id() < a.nrows, v);
+        if (!valid) continue;
+        const uint32_t r   = a.rshift >= 64 ? 0u : uint32_t(hash_key(v[0]) >> a.rshift);
+        const uint32_t pos = atomicAdd(&cur[r], 1u);
+        if (a.words == 2) {
+            reinterpret_cast<ulonglong2*>(out)[pos] = make_ulonglong2(v[0], v[1]);
+        } else {
+            uint64_t* d = out + uint64_t(pos) * a.words;
+#pragma unroll
+            for (int w = 0; w < kMaxWords; ++w)
+                if (uint32_t(w) < a.words) d[w] = v[w];
+        }
+    }
+}
+
+hipError_t launch_scan_partition(const ScanBuildArgs& a, uint32_t blocks, uint32_t* bh, uint32_t* scan_scratch,
+                                 uint64_t* total, uint64_t* parts, uint32_t threads, hipStream_t s) {
+    if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;
+    if (a.words == 0 || a.words > uint32_t(kMaxWords)) return hipErrorInvalidValue;
+    if (uint64_t(blocks) * a.blocks_per_wg * 64 < a.nrows) return hipErrorInvalidValue; // every row covered
+    const size_t hist_lds = size_t(a.nregions) * sizeof(uint32_t);
+    k_scan_hist<<<dim3(blocks), dim3(threads), hist_lds, s>>>(a, bh);
+    hipError_t e = launch_exclusive_scan(bh, uint64_t(a.nregions) * blocks, scan_scratch, total, s);
+    if (e != hipSuccess) return e;
+    k_scan_scatter<<<dim3(blocks), dim3(threads), hist_lds, s>>>(a, bh, parts);
+    return hipGetLastError();
+}
+
+hipError_t launch_region_build(const uint64_t* parts, const uint32_t* bh, uint32_t blocks, uint64_t n,
+                               uint32_t words, uint64_t* ht, uint32_t log_cap, uint32_t log_regions,
+                               uint64_t* bloom, uint32_t bloom_log, uint32_t* flags, uint32_t threads,
+                               uint32_t walk_cap, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t nregions = 1u << log_regions;
+    const uint32_t rslots   = 1u << (log_cap - log_regions);
+    const size_t   tab_lds  = size_t(rslots) * words * sizeof(uint64_t);
+    if (tab_lds > 65536) {
+        constexpr int              kMaxDev = 16;
+        static std::atomic<size_t> lds_set[kMaxDev];
+        int                        dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev ||
+            lds_set[dev].load(std::memory_order_relaxed) < tab_lds) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part_build),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(tab_lds));
+            if (e != hipSuccess) return e;
+            if (dev >= 0 && dev < kMaxDev) lds_set[dev].store(tab_lds, std::memory_order_relaxed);
+        }
+    }
+    k_part_build<<<dim3(nregions), dim3(threads), tab_lds, s>>>(parts, bh, blocks, n, nregions, words, rslots,
+                                                                64 - log_cap, ht, bloom, bloom_log, flags, walk_cap);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // grouped build: a table whose keys repeat many times keeps each distinct key
 // once, as [key, start << 24 | count], and the key's tuples (words 1.. of the
 // staged tuple) contiguously in a payload array. Probe sequences then stay

@@ -249,6 +249,27 @@ hipError_t launch_group_build(const uint64_t* stage, uint64_t n, uint64_t n_real
                               uint32_t log_regions, uint64_t* bloom, uint32_t bloom_log, uint32_t* flags,
                               uint32_t threads, hipStream_t s);
 size_t     group_build_lds(uint32_t rslots, uint32_t threads);
+// Partitioning straight from a base table's pages (a hash table built on a
+// plain scan): tuple word w is column col[w]; rows with NULL in a `notnull`
+// word are dropped. Same histogram layout (bh[region * blocks + block]) and
+// region-ordered `parts` as the staged path; workgroup b takes 64-row blocks
+// [b * blocks_per_wg, ...).
+struct ScanBuildArgs {
+    ScanCol  col[kMaxWords];
+    uint32_t words;
+    uint32_t notnull;
+    uint64_t nrows;
+    uint64_t blocks_per_wg;
+    uint32_t rshift;   // region = hash >> rshift (64: one region)
+    uint32_t nregions;
+};
+hipError_t launch_scan_partition(const ScanBuildArgs& a, uint32_t blocks, uint32_t* bh, uint32_t* scan_scratch,
+                                 uint64_t* total, uint64_t* parts, uint32_t threads, hipStream_t s);
+// The LDS region build alone, over an already partitioned `parts` (n tuples).
+hipError_t launch_region_build(const uint64_t* parts, const uint32_t* bh, uint32_t blocks, uint64_t n,
+                               uint32_t words, uint64_t* ht, uint32_t log_cap, uint32_t log_regions,
+                               uint64_t* bloom, uint32_t bloom_log, uint32_t* flags, uint32_t threads,
+                               uint32_t walk_cap, hipStream_t s);
 // Launches the fused pipeline. `slots` = tuple width (selects the kernel variant).
 hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s);
 // Lean variant for one probe without residual predicates and a count or

@@ -139,7 +139,9 @@ typedef struct {
  *       1024 = default);
  *   "group_walk": a hash-table insert that passes this many slots holding its own key
  *       (a heavily repeated key) rebuilds the table grouped by key (default 32; 0 = never);
- *   "force_grouped": build every hash table grouped by key (1) or only when needed (0). */
+ *   "force_grouped": build every hash table grouped by key (1) or only when needed (0);
+ *   "scan_build": hash tables on a plain scan are partitioned straight from its pages (1,
+ *       default) or from staged tuples (0). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */

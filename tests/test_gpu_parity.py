@@ -131,6 +131,41 @@ def test_grouped_and_plain_tables_agree_on_a_star_plan(ctx):
     assert plain == grouped
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_scan_build_matches_staged_build(ctx, seed):
+    """Hash tables on a plain scan partitioned straight from the pages (the
+    default for builds of 64K rows or more) against the staged path and the
+    oracle: NULL keys, a second join column with NULLs in the build tuple, a
+    VARCHAR payload, and a three-table chain."""
+    rng = np.random.default_rng(seed)
+    nb, npr, nd = 120_000, 300_000, 90_000
+    bkey = rng.permutation(np.arange(nb, dtype=np.int64) * 3 + 1)
+    bv = np.array([f"s{v % 977}" for v in range(nb)], dtype=object)
+    bfk = rng.integers(0, nd, nb).astype(np.int32)
+    pkey = np.where(rng.random(npr) < 0.8, bkey[rng.integers(0, nb, npr)], -7)
+    dkey = rng.permutation(np.arange(nd, dtype=np.int32))
+    plan = gj.Plan()
+    plan.new_input(gj.make_table([(INT64, bkey, rng.random(nb) > 0.05), (VARCHAR, bv, rng.random(nb) > 0.1),
+                                  (INT32, bfk, rng.random(nb) > 0.05)], nb))
+    plan.new_input(gj.make_table([(INT64, pkey, rng.random(npr) > 0.05), (FP64, rng.random(npr), None)], npr))
+    plan.new_input(gj.make_table([(INT32, dkey, rng.random(nd) > 0.02), (INT32, rng.integers(0, 9, nd), None)], nd))
+    b = plan.new_scan_node(0, [(0, INT64), (1, VARCHAR), (2, INT32)])
+    q = plan.new_scan_node(1, [(0, INT64), (1, FP64)])
+    d = plan.new_scan_node(2, [(0, INT32), (1, INT32)])
+    j = plan.new_join_node(True, b, q, 0, 0, [(1, VARCHAR), (2, INT32), (4, FP64)])
+    plan.root = plan.new_join_node(True, j, d, 1, 0, [(0, VARCHAR), (2, FP64), (4, INT32)])
+    types = [VARCHAR, FP64, INT32]
+    say(f"scan build vs staged build, seed {seed}")
+    got = check_table(ctx.execute(plan), types)
+    ctx.set_option("scan_build", 0)
+    try:
+        staged = check_table(ctx.execute(plan), types)
+    finally:
+        ctx.set_option("scan_build", 1)
+    exp = check_table(oracle.execute(plan), types)
+    assert got == staged == exp
+
+
 def slots_of(table):
     """Fixed-width result columns as uint64 slot arrays (NULL = all ones)."""
     out = []
