@@ -102,7 +102,8 @@ def run_c4(args):
     on device-resident tables, all in one cgj_execute_many call on concurrent
     worker streams; the same joins as one execute_device call each are timed
     beside it. (The persistent-kernel batch over a device work queue is
-    disabled in this build.) A step is one pass over all joins."""
+    disabled: cgj_batch_run refuses to launch it.) A step is one pass over
+    all joins."""
     import gpujoin as gj
     from gpujoin import workloads
 
@@ -366,8 +367,9 @@ def main():
                     help="also time the host-pages-in/host-pages-out path at this table scale")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs the join pipeline on a single MI355X "
-              f"(--gpus {args.gpus} is not supported; sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: the join pipeline runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported: sharding across GPUs is a design, not built)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.scale is None:

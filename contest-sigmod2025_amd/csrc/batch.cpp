@@ -305,7 +305,7 @@ void Batch::run() {
     // is not found yet; until it is, running a batch is an error rather than
     // a launch that may not return.
     throw std::runtime_error("batched execution is disabled: its kernels did not finish on the GPU "
-                             "(cause under investigation); run the plans one by one");
+                             "(cause not found); run the plans with cgj_execute_many instead");
     for (int attempt = 0;; ++attempt) {
         launch_once();
         bool grow = false;

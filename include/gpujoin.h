@@ -201,7 +201,7 @@ int      cgj_last_stats(const cgj_context* ctx, cgj_stats* out);
 
 /* Counts the result rows of the plan without materialising pages: the same
  * builds and probe chain, with a counting target. Used for exact sub-plan
- * cardinalities (the estimator's training labels and q-error judge). */
+ * cardinalities (the estimator's training labels and its q-error reference). */
 int      cgj_count_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
                           uint64_t* rows);
 

@@ -68,7 +68,7 @@ def test_page_writer_round_trip(t):
 
 def test_page_writer_capacities():
     """ColumnInserter packs 1984 INT32 / 1007 INT64 values per page without NULLs
-    (measured in the reference survey: 5041 and 9935 pages for 10M rows)."""
+    (the reference's page writer gives 5041 and 9935 pages for 10M rows)."""
     n = 10_000_000 // 100
     c32 = gj.encode_column(INT32, np.arange(n, dtype=np.int32))
     c64 = gj.encode_column(INT64, np.arange(n, dtype=np.int64))

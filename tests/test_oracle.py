@@ -1,7 +1,7 @@
 """The CPU oracle pinned against the reference's golden vectors and brute force.
 
 Oracle = test infrastructure (oracle/); these tests check it before it is
-used to judge the GPU path.
+used to check the GPU path.
 """
 import random
 
