@@ -784,21 +784,22 @@ bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64
     if (bloom) CGJ_HIP(launch_fill_u64(bloom, 1ull << ht->bloom_log, 0, s));
     CGJ_HIP(launch_scan_partition(sa, blocks, bh.get<uint32_t>(), scratch.get<uint32_t>(), total.get<uint64_t>(),
                                   parts.get<uint64_t>(), ctx_.options.build_threads, s));
-    uint64_t n = 0;
-    CGJ_HIP(hipMemcpyAsync(&n, total.get(), sizeof(n), hipMemcpyDeviceToHost, s));
-    CGJ_HIP(hipStreamSynchronize(s));
-    if (n == 0) return true; // empty build side
+    // the region build reads the tuple count on the device; one round trip
+    // brings back the count and the flags
     const uint32_t walk_cap = ctx_.options.group_walk;
-    CGJ_HIP(launch_region_build(parts.get<uint64_t>(), bh.get<uint32_t>(), blocks, n, words, ht->slots.get<uint64_t>(),
-                                lg, log_regions, bloom, ht->bloom_log, flags.get<uint32_t>(),
+    CGJ_HIP(launch_region_build(parts.get<uint64_t>(), bh.get<uint32_t>(), blocks, total.get<uint64_t>(), words,
+                                ht->slots.get<uint64_t>(), lg, log_regions, bloom, ht->bloom_log, flags.get<uint32_t>(),
                                 ctx_.options.build_threads, walk_cap, s));
+    uint64_t n = 0;
     uint32_t f[4];
+    CGJ_HIP(hipMemcpyAsync(&n, total.get(), sizeof(n), hipMemcpyDeviceToHost, s));
     CGJ_HIP(hipMemcpyAsync(f, flags.get(), sizeof(f), hipMemcpyDeviceToHost, s));
     CGJ_HIP(hipEventRecord(ev1_, s));
     CGJ_HIP(hipEventSynchronize(ev1_));
     float fms = 0;
     CGJ_HIP(hipEventElapsedTime(&fms, ev0_, ev1_));
     ms_build += fms;
+    if (n == 0) return true; // empty build side
     if (f[1] || f[2]) { // a region overflowed or a key repeats heavily: the staged path handles it
         ++retries;
         return false;

@@ -363,11 +363,11 @@ __global__ __launch_bounds__(1024) void k_part_build(const uint64_t* parts, cons
                                                     uint64_t n, uint32_t nregions, uint32_t words,
                                                     uint32_t rslots, uint32_t shift, uint64_t* ht,
                                                     uint64_t* bloom, uint32_t bloom_log, uint32_t* flags,
-                                                    uint32_t walk_cap) {
+                                                    uint32_t walk_cap, const uint64_t* n_dev) {
     extern __shared__ uint64_t tab[];
     uint32_t r     = blockIdx.x;
     uint64_t start = offs[uint64_t(r) * blocks];
-    uint64_t end   = r + 1 < nregions ? offs[uint64_t(r + 1) * blocks] : n;
+    uint64_t end   = r + 1 < nregions ? offs[uint64_t(r + 1) * blocks] : (n_dev ? *n_dev : n);
     uint64_t cnt   = end - start;
     uint64_t* dst  = ht + uint64_t(r) * rslots * words;
     if (cnt >= rslots) { // a full region would leave probes without an empty slot
@@ -442,7 +442,8 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
         }
     }
     k_part_build<<<dim3(nregions), dim3(threads), tab_lds, s>>>(parts, bh, blocks, n_real, nregions, words, rslots,
-                                                                64 - log_cap, ht, bloom, bloom_log, flags, walk_cap);
+                                                                64 - log_cap, ht, bloom, bloom_log, flags, walk_cap,
+                                                                nullptr);
     return hipGetLastError();
 }
 
@@ -521,11 +522,10 @@ hipError_t launch_scan_partition(const ScanBuildArgs& a, uint32_t blocks, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_region_build(const uint64_t* parts, const uint32_t* bh, uint32_t blocks, uint64_t n,
+hipError_t launch_region_build(const uint64_t* parts, const uint32_t* bh, uint32_t blocks, const uint64_t* n_dev,
                                uint32_t words, uint64_t* ht, uint32_t log_cap, uint32_t log_regions,
                                uint64_t* bloom, uint32_t bloom_log, uint32_t* flags, uint32_t threads,
                                uint32_t walk_cap, hipStream_t s) {
-    if (n == 0) return hipSuccess;
     const uint32_t nregions = 1u << log_regions;
     const uint32_t rslots   = 1u << (log_cap - log_regions);
     const size_t   tab_lds  = size_t(rslots) * words * sizeof(uint64_t);
@@ -541,8 +541,9 @@ hipError_t launch_region_build(const uint64_t* parts, const uint32_t* bh, uint32
             if (dev >= 0 && dev < kMaxDev) lds_set[dev].store(tab_lds, std::memory_order_relaxed);
         }
     }
-    k_part_build<<<dim3(nregions), dim3(threads), tab_lds, s>>>(parts, bh, blocks, n, nregions, words, rslots,
-                                                                64 - log_cap, ht, bloom, bloom_log, flags, walk_cap);
+    k_part_build<<<dim3(nregions), dim3(threads), tab_lds, s>>>(parts, bh, blocks, 0, nregions, words, rslots,
+                                                                64 - log_cap, ht, bloom, bloom_log, flags, walk_cap,
+                                                                n_dev);
     return hipGetLastError();
 }
 

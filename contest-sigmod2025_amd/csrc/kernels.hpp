@@ -265,8 +265,9 @@ struct ScanBuildArgs {
 };
 hipError_t launch_scan_partition(const ScanBuildArgs& a, uint32_t blocks, uint32_t* bh, uint32_t* scan_scratch,
                                  uint64_t* total, uint64_t* parts, uint32_t threads, hipStream_t s);
-// The LDS region build alone, over an already partitioned `parts` (n tuples).
-hipError_t launch_region_build(const uint64_t* parts, const uint32_t* bh, uint32_t blocks, uint64_t n,
+// The LDS region build alone, over an already partitioned `parts` holding
+// *n_dev tuples (read on the device: no host round trip in between).
+hipError_t launch_region_build(const uint64_t* parts, const uint32_t* bh, uint32_t blocks, const uint64_t* n_dev,
                                uint32_t words, uint64_t* ht, uint32_t log_cap, uint32_t log_regions,
                                uint64_t* bloom, uint32_t bloom_log, uint32_t* flags, uint32_t threads,
                                uint32_t walk_cap, hipStream_t s);
