@@ -364,7 +364,7 @@ __global__ __launch_bounds__(1024) void k_part_build(const uint64_t* parts, cons
                                                     uint32_t rslots, uint32_t shift, uint64_t* ht,
                                                     uint64_t* bloom, uint32_t bloom_log, uint32_t* flags,
                                                     uint32_t walk_cap, const uint64_t* n_dev) {
-    extern __shared__ uint64_t tab[];
+    extern __shared__ __attribute__((aligned(16))) uint64_t tab[];
     uint32_t r     = blockIdx.x;
     uint64_t start = offs[uint64_t(r) * blocks];
     uint64_t end   = r + 1 < nregions ? offs[uint64_t(r + 1) * blocks] : (n_dev ? *n_dev : n);
@@ -374,7 +374,10 @@ __global__ __launch_bounds__(1024) void k_part_build(const uint64_t* parts, cons
         if (threadIdx.x == 0) flags[1] = 1;
         return;
     }
-    for (uint32_t i = threadIdx.x; i < rslots * words; i += blockDim.x) tab[i] = kEmpty;
+    // 16-byte LDS and global accesses: rslots * words is even (rslots >= 16)
+    const uint32_t n2 = rslots * words / 2;
+    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x)
+        reinterpret_cast<ulonglong2*>(tab)[i] = make_ulonglong2(kEmpty, kEmpty);
     __syncthreads();
     bool     dup   = false, stop = false;
     uint32_t rmask = rslots - 1;
@@ -406,7 +409,8 @@ __global__ __launch_bounds__(1024) void k_part_build(const uint64_t* parts, cons
     if (dup) flags[0] = 1;
     if (stop) flags[2] = 1;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < rslots * words; i += blockDim.x) dst[i] = tab[i];
+    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x)
+        reinterpret_cast<ulonglong2*>(dst)[i] = reinterpret_cast<const ulonglong2*>(tab)[i];
 }
 
 hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real, uint32_t words, uint64_t* parts,
