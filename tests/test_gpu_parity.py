@@ -468,6 +468,48 @@ def test_c2_recipe_scaled_vs_oracle(ctx):
     assert s["output_rows"] > 3_000_000
 
 
+def test_c2_full_build_sampled_probe_vs_oracle(ctx):
+    """The c2 build side at full size (50M INT64 keys, 5% NULLs: the 2^27-slot
+    table in 16K LDS regions that the benchmark builds) probed by a 2M-row
+    slice from the middle of the 500M-row probe recipe; bit-exact against the
+    oracle on the same rows."""
+    from gpujoin import workloads
+
+    wl = workloads.c2(1.0)
+    bt, pt = wl.tables
+    first, n = 123_456_789, 2_000_000
+    say("c2 full build (50M) x 2M-row probe slice")
+    build_dev = ctx.generate(wl.seed, bt.rows, bt.columns)
+    probe = gj.ColumnarTable(n, [gj.host_generate(wl.seed, first, n, c) for c in pt.columns])
+    plan = wl.make_plan()
+    got = ctx.execute_device(plan, [build_dev, ctx.upload(probe)]).download()
+    build = gj.ColumnarTable(bt.rows, [gj.host_generate(wl.seed, 0, bt.rows, c) for c in bt.columns])
+    plan.new_input(build)
+    plan.new_input(probe)
+    exp = oracle.execute(plan)
+    assert got.num_rows == exp.num_rows > 1_000_000
+    for c in got.columns:
+        assert gj.pages.validate_column(c.pages, c.type) == got.num_rows
+    for a, b_ in zip(sorted_rows(slots_of(got)), sorted_rows(slots_of(exp))):
+        assert np.array_equal(a, b_)
+
+
+def test_c2_full_size_count_matches_pages(ctx):
+    """At full c2 size (50M x 500M) the counting target and the page target
+    agree on the result size, and the pages hold that many rows."""
+    from gpujoin import workloads
+
+    wl = workloads.c2(1.0)
+    say("c2 full size: count vs pages")
+    tables = [ctx.generate(wl.seed, t.rows, t.columns) for t in wl.tables]
+    plan = wl.make_plan()
+    count = ctx.count_device(plan, tables)
+    res = ctx.execute_device(plan, tables)
+    assert res.num_rows == count > 400_000_000
+    st = ctx.stats()
+    assert st["output_rows"] == count and st["retries"] == 0, st
+
+
 def test_c3_recipe_scaled_vs_oracle(ctx):
     """Snowflake: fact x 4 dimensions in one fused probe chain with bloom
     filters; the output carries a VARCHAR column with long strings."""
