@@ -525,6 +525,29 @@ def test_c3_recipe_scaled_vs_oracle(ctx):
     assert s["pipelines"] == 5, s  # four dimension builds + one fused probe chain
 
 
+def test_c1_full_size_vs_oracle(ctx):
+    """c1 at full size: 1M-row INT32 build x 10M-row probe, bit-exact."""
+    from gpujoin import workloads
+
+    plan = _workload_plan(workloads.c1(1.0))
+    say("c1 at full size")
+    s = _compare_large(ctx, plan)
+    assert s["output_rows"] == 10_000_000
+
+
+def test_c3_full_size_vs_oracle(ctx):
+    """c3 at full size: the 36M-row fact table through the fused 4-probe chain
+    (bloom filters, lazily decoded columns) with the VARCHAR output, bit-exact."""
+    from gpujoin import workloads
+
+    plan = _workload_plan(workloads.c3(1.0))
+    say("c3 at full size")
+    types = [t for _, t in plan.nodes[plan.root].output_attrs]
+    got = check_table(ctx.execute(plan), types)
+    exp = check_table(oracle.execute(plan), types)
+    assert got == exp and len(got) > 1000
+
+
 def test_tables_outlive_their_context():
     """Device tables and results freed after their context is destroyed."""
     ctx2 = gj.Context(0)
