@@ -273,6 +273,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "group_walk") options.group_walk = v >= 0 && v < 1e6 ? uint32_t(v) : 32u;
     else if (name == "force_grouped") options.force_grouped = v != 0;
     else if (name == "scan_build") options.scan_build = v != 0;
+    else if (name == "scan_workgroups") options.scan_workgroups = v >= 16 && v <= 4096 ? uint32_t(v) : 512u;
     else return false;
     return true;
 }
@@ -757,7 +758,7 @@ bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64
     sa.nregions                 = 1u << log_regions;
     sa.rshift                   = 64 - log_regions;
     const uint64_t nb           = (n0 + 63) / 64;
-    const uint32_t blocks       = uint32_t(std::min<uint64_t>(512, std::max<uint64_t>(1, nb / 64)));
+    const uint32_t blocks       = uint32_t(std::min<uint64_t>(ctx_.options.scan_workgroups, std::max<uint64_t>(1, nb / 64)));
     sa.blocks_per_wg            = (nb + blocks - 1) / blocks;
     auto ht    = std::make_unique<HashTable>();
     ht->eqs    = eqs;

@@ -164,6 +164,7 @@ struct Options {
     // Hash tables on a plain scan are partitioned straight from the pages
     // (no staged tuples).
     bool     scan_build            = true;
+    uint32_t scan_workgroups       = 512; // workgroups of the scan-driven partition (16..4096)
 };
 
 // Tunables of one batched join.

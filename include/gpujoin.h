@@ -141,7 +141,8 @@ typedef struct {
  *       (a heavily repeated key) rebuilds the table grouped by key (default 32; 0 = never);
  *   "force_grouped": build every hash table grouped by key (1) or only when needed (0);
  *   "scan_build": hash tables on a plain scan are partitioned straight from its pages (1,
- *       default) or from staged tuples (0). */
+ *       default) or from staged tuples (0);
+ *   "scan_workgroups": workgroups of that partition (16..4096, default 512). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
