@@ -296,8 +296,9 @@ def run_c5(args):
                          "ms_device_build_per_step": round(ms_build / args.steps, 1),
                          "ms_device_probe_per_step": round(ms_probe / args.steps, 1)}
         log(f"{name}: {results[name]}")
-    # batched + learned: every evaluated plan in one cgj_execute_many call,
-    # planned here by the learned estimator, executed on worker streams
+    # batched + learned: every evaluated plan in one cgj_execute_many call;
+    # each worker plans its queries with the learned estimator and runs them
+    # on its own stream
     ctx.set_estimator(gj.EST_LEARNED, mlp)
     all_plans = [ps.plans[pid].plan for pid in eval_ids]
     all_tabs = [est.plan_inputs(tabs, ps.plans[pid]) for pid in eval_ids]
@@ -358,7 +359,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
                     help="execution option (cgj_set_option), repeatable")
-    ap.add_argument("--workers", type=int, default=4, help="c4: concurrent worker streams")
+    ap.add_argument("--workers", type=int, default=None,
+                    help="c4/c5: concurrent worker streams of cgj_execute_many (default: c4 6, c5 8)")
     ap.add_argument("--plans", type=int, default=2000, help="c5: evaluated plans")
     ap.add_argument("--train-plans", type=int, default=150, help="c5: training plans")
     ap.add_argument("--epochs", type=int, default=150, help="c5: training epochs")
@@ -374,6 +376,8 @@ def main():
 
     if args.scale is None:
         args.scale = 0.1 if args.workload == "c5" else 1.0
+    if args.workers is None:  # the fastest counts measured on one MI355X
+        args.workers = 8 if args.workload == "c5" else 6
     if args.workload == "c5":
         return run_c5(args)
     if args.workload == "c4":
@@ -418,7 +422,8 @@ def main():
     elapsed = time.perf_counter() - t_start
     ms_step = elapsed * 1000.0 / args.steps
     for i, s in enumerate(stats):
-        log(f"step {i}: build {s['ms_build']:.2f} ms, probe kernel {s['ms_probe_kernel']:.2f} ms, rows {s['rows']}")
+        part = f", probe partition {s['ms_probe_partition']:.2f} ms" if s["ms_probe_partition"] else ""
+        log(f"step {i}: build {s['ms_build']:.2f} ms, probe kernel {s['ms_probe_kernel']:.2f} ms{part}, rows {s['rows']}")
 
     # achievable streaming bandwidth on this GPU: a plain 2 GiB copy (reads + writes)
     copy_bytes = 2 << 30

@@ -56,6 +56,17 @@ void free_column(cgj_result_column& c) {
 
 } // namespace
 
+namespace {
+std::vector<const cgj::DevTable*> views_of(cgj_dtable* const* inputs, uint32_t n) {
+    std::vector<const cgj::DevTable*> v;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!inputs[i]) throw std::runtime_error("null input table");
+        v.push_back(inputs[i]->impl.get());
+    }
+    return v;
+}
+} // namespace
+
 extern "C" {
 
 cgj_context* cgj_context_create(int device) {
@@ -272,16 +283,6 @@ int cgj_count_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* 
     });
 }
 
-namespace {
-std::vector<const cgj::DevTable*> views_of(cgj_dtable* const* inputs, uint32_t n) {
-    std::vector<const cgj::DevTable*> v;
-    for (uint32_t i = 0; i < n; ++i) {
-        if (!inputs[i]) throw std::runtime_error("null input table");
-        v.push_back(inputs[i]->impl.get());
-    }
-    return v;
-}
-} // namespace
 
 uint32_t cgj_num_features(void) { return cgj::kFeatures; }
 

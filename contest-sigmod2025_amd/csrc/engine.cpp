@@ -1086,52 +1086,28 @@ std::vector<std::unique_ptr<DevResult>> Context::execute_many(const std::vector<
         workers_.back()->options = options;
     }
     for (uint32_t w = 0; w < workers; ++w) workers_[w]->options = options;
-    // With a model other than the max rule, this thread plans every query with
-    // this context's estimator (statistics catalog, one batched inference per
-    // query) and publishes each plan as it is done; the workers execute
-    // planned join trees as soon as theirs is ready, so planning overlaps
-    // execution. Under the max rule the workers plan for themselves.
-    const bool                         plan_here = est_->mode != EST_MAX;
-    std::vector<PlannedQuery>          planned(plan_here ? plans.size() : 0);
-    std::atomic<size_t>                ready{plan_here ? 0 : plans.size()};
-    std::atomic<bool>                  failed{false};
+    // Each worker plans its query with this context's estimator (statistics
+    // catalog shared under a lock, the network's inference on the worker's own
+    // stream) and executes it, so planning runs on as many threads as
+    // execution does.
+    if (est_->mode == EST_LEARNED && est_->mlp) est_->mlp->refresh_bf16(); // before any thread reads the weights
     CGJ_HIP(hipStreamSynchronize(stream_)); // inputs uploaded on this stream are complete
     std::vector<std::unique_ptr<DevResult>> out(plans.size());
-    std::vector<std::string>                errors(workers + 1);
+    std::vector<std::string>                errors(workers);
     std::atomic<size_t>                     next{0};
     auto                                    run = [&](uint32_t w) {
         try {
             CGJ_HIP(hipSetDevice(device_));
-            for (size_t i; (i = next.fetch_add(1)) < plans.size();) {
-                if (!plan_here) {
-                    out[i] = workers_[w]->execute(*plans[i], inputs[i]);
-                    continue;
-                }
-                while (ready.load(std::memory_order_acquire) <= i) {
-                    if (failed.load(std::memory_order_relaxed)) return;
-                    std::this_thread::yield();
-                }
-                out[i] = workers_[w]->run_planned(planned[i], inputs[i], false);
-            }
+            Context& wc = *workers_[w];
+            for (size_t i; (i = next.fetch_add(1)) < plans.size();)
+                out[i] = wc.run_planned(wc.plan_query(*plans[i], inputs[i], *est_), inputs[i], false);
         } catch (const std::exception& e) {
             errors[w] = e.what();
-            failed    = true;
             next      = plans.size(); // stop the other workers early
         }
     };
     std::vector<std::thread> threads;
     for (uint32_t w = 0; w < workers; ++w) threads.emplace_back(run, w);
-    if (plan_here) {
-        try {
-            for (size_t i = 0; i < plans.size() && !failed.load(std::memory_order_relaxed); ++i) {
-                planned[i] = plan_query(*plans[i], inputs[i]);
-                ready.store(i + 1, std::memory_order_release);
-            }
-        } catch (const std::exception& e) {
-            errors[workers] = e.what();
-            failed          = true;
-        }
-    }
     for (auto& t : threads) t.join();
     for (const auto& e : errors)
         if (!e.empty()) throw std::runtime_error("execute_many: " + e);
@@ -1210,11 +1186,15 @@ std::vector<Context::PlannedJoin> Context::plan_tree(const cgj_plan& plan, const
 }
 
 PlannedQuery Context::plan_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs) {
+    return plan_query(plan, inputs, *est_);
+}
+
+PlannedQuery Context::plan_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs, Estimator& est) {
     PlannedQuery pq;
     pq.q = import_inputs(plan, inputs);
     check_types(pq.q, inputs);
     const auto t0 = std::chrono::steady_clock::now();
-    pq.tree       = plan_joins(pq.q, est_->model(*this, pq.q, inputs, pq.rep));
+    pq.tree       = plan_joins(pq.q, est.model(*this, pq.q, inputs, pq.rep));
     pq.ms_plan =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() - pq.rep.ms_stats;
     return pq;

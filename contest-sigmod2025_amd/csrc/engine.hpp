@@ -253,6 +253,8 @@ public:
     // A query imported and ordered by this context's estimator, and its
     // execution (on any context of the device) as planned.
     PlannedQuery               plan_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
+    // Planned with another context's estimator (its counts and inference run here).
+    PlannedQuery plan_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs, Estimator& est);
     std::unique_ptr<DevResult> run_planned(const PlannedQuery& pq, const std::vector<const DevTable*>& inputs,
                                            bool count_only);
 

@@ -165,14 +165,20 @@ void Mlp::refresh_bf16() {
     CGJ_HIP(hipMemsetAsync(bf1_.get(), 0, uint64_t(in_pad) * kMlpHidden * sizeof(uint16_t), s));
     CGJ_HIP(launch_to_bf16(params_.get<float>() + off_w1(), bf1_.get<uint16_t>(), uint64_t(in_) * kMlpHidden, s));
     CGJ_HIP(launch_to_bf16(params_.get<float>() + off_w2(), bf2_.get<uint16_t>(), uint64_t(kMlpHidden) * kMlpHidden, s));
+    CGJ_HIP(hipStreamSynchronize(s)); // other contexts' streams read the BF16 weights next
     dirty_ = false;
 }
 
 void Mlp::predict(const float* x, uint32_t n, float* out) {
-    if (n == 0) return;
-    hipStream_t s = ctx_.stream();
     refresh_bf16();
-    DevBuf X(&ctx_.heap(), uint64_t(n) * in_ * sizeof(float)), Y(&ctx_.heap(), uint64_t(n) * kMlpOut * sizeof(float));
+    predict_on(ctx_, x, n, out);
+}
+
+void Mlp::predict_on(Context& on, const float* x, uint32_t n, float* out) const {
+    if (n == 0) return;
+    if (dirty_) throw std::runtime_error("network weights changed since their BF16 conversion");
+    hipStream_t s = on.stream();
+    DevBuf X(&on.heap(), uint64_t(n) * in_ * sizeof(float)), Y(&on.heap(), uint64_t(n) * kMlpOut * sizeof(float));
     CGJ_HIP(hipMemcpyAsync(X.get(), x, uint64_t(n) * in_ * sizeof(float), hipMemcpyHostToDevice, s));
     const float* P = params_.get<float>();
     CGJ_HIP(launch_mlp_infer(X.get<float>(), n, in_, bf1_.get<uint16_t>(), P + off_b1(), bf2_.get<uint16_t>(),
@@ -196,7 +202,7 @@ void Mlp::predict_f32(const float* x, uint32_t n, float* out) {
 // ---------------------------------------------------------------------------
 // Per-query structures for featurising many sets: the join classes with their
 // member relations (sorted) and their measured edges, the log sizes.
-SetFeaturizer::SetFeaturizer(const Query& q, const QueryStats& st) : st_(st) {
+SetFeaturizer::SetFeaturizer(const Query& q, const QueryStats& st) {
     nrels_ = uint32_t(q.rels.size());
     for (uint32_t r = 0; r < nrels_; ++r) lrows_.push_back(std::log10(std::max(st.rows[r], 1.0)));
     for (uint32_t r = 0; r < nrels_; ++r) empty_ |= st.rows[r] <= 0 ? (1ull << r) : 0;
@@ -399,9 +405,17 @@ QueryStats Estimator::stats(Context& ctx, const Query& q, const std::vector<cons
                 PairKey         k  = std::make_tuple(tx->uid, x.col, ty->uid, y.col);
                 if (std::make_pair(tx->uid, x.col) > std::make_pair(ty->uid, y.col))
                     k = std::make_tuple(ty->uid, y.col, tx->uid, x.col);
-                auto it = pair_rows_.find(k);
-                if (it == pair_rows_.end()) {
-                    uint64_t rows = 0;
+                uint64_t rows   = 0;
+                bool     cached = false;
+                {
+                    std::lock_guard<std::mutex> lock(pairs_mu_);
+                    auto                        it = pair_rows_.find(k);
+                    if (it != pair_rows_.end()) {
+                        rows   = it->second;
+                        cached = true;
+                    }
+                }
+                if (!cached) {
                     if (tx->nrows && ty->nrows) {
                         // two scans joined on the pair's columns
                         Query pq;
@@ -422,10 +436,12 @@ QueryStats Estimator::stats(Context& ctx, const Query& q, const std::vector<cons
                             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                         ++rep.stat_joins;
                     }
-                    it = pair_rows_.emplace(k, rows).first;
+                    // counted outside the lock: two threads may count one pair, with equal results
+                    std::lock_guard<std::mutex> lock(pairs_mu_);
+                    pair_rows_.emplace(k, rows);
                 }
                 const double denom = double(q.rels[x.rel].rows) * double(q.rels[y.rel].rows);
-                st.edges.push_back(QueryStats::Edge{x.rel, y.rel, e, denom > 0 ? double(it->second) / denom : 0.0});
+                st.edges.push_back(QueryStats::Edge{x.rel, y.rel, e, denom > 0 ? double(rows) / denom : 0.0});
             }
     }
     return st;
@@ -452,7 +468,8 @@ CardFn Estimator::model(Context& ctx, const Query& q, const std::vector<const De
         std::vector<float> x(multi.size() * kFeatures), y(multi.size() * kMlpOut);
         const SetFeaturizer fz(q, *st);
         for (size_t i = 0; i < multi.size(); ++i) fz.featurize(multi[i], &x[i * kFeatures]);
-        mlp->predict(x.data(), uint32_t(multi.size()), y.data());
+        mlp->refresh_bf16(); // a no-op unless the weights changed (never while threads plan)
+        mlp->predict_on(ctx, x.data(), uint32_t(multi.size()), y.data());
         // The network predicts a correction (mu) to the independence estimate
         // in log10 units, and its spread sigma = exp(s).
         const float max_s = std::log(std::max(max_sigma, 1e-6f));

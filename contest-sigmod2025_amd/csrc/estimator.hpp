@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <tuple>
 #include <utility>
 #include <vector>
@@ -41,11 +42,16 @@ public:
                    float weight_decay, uint64_t seed);
     void     predict(const float* x, uint32_t n, float* out);     // BF16 MFMA, host arrays (n x 2)
     void     predict_f32(const float* x, uint32_t n, float* out); // FP32 forward, host arrays
+    // BF16 inference on the stream and heap of `on` (another context of the
+    // same device); reads the weights only, so threads may call it at once
+    // with contexts of their own after refresh_bf16().
+    void     predict_on(Context& on, const float* x, uint32_t n, float* out) const;
+    // Converts the first two layers' weights to BF16 after they changed.
+    void     refresh_bf16();
 
 private:
     float forward(const float* xb, const float* yb, uint32_t m, bool backward);
     void  ensure_rows(uint32_t m);
-    void  refresh_bf16();
     uint64_t off_w1() const { return 0; }
     uint64_t off_b1() const { return uint64_t(in_) * kMlpHidden; }
     uint64_t off_w2() const { return off_b1() + kMlpHidden; }
@@ -94,7 +100,6 @@ private:
         std::vector<const QueryStats::Edge*> edges;
     };
     const QueryStats::Edge* edge(const Cls& c, uint32_t a, uint32_t b) const;
-    const QueryStats&       st_;
     uint32_t                nrels_ = 0;
     uint64_t                empty_ = 0; // relations without rows
     std::vector<double>     lrows_;
@@ -131,7 +136,8 @@ enum : uint32_t { EST_MAX = 0, EST_INDEPENDENCE = 1, EST_LEARNED = 2 };
 //                    units) exceeds `max_sigma` keep the independence estimate
 // Pairwise selectivities come from exact 2-way counts on the device, cached
 // per (table, column) pair for the context's lifetime: tables of a shared
-// pool are joined by many plans.
+// pool are joined by many plans. model() may run on several threads at once,
+// each with its own context (counts and inference run on that context).
 class Estimator {
 public:
     uint32_t             mode      = EST_MAX;
@@ -141,10 +147,14 @@ public:
 
     CardFn     model(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep);
     QueryStats stats(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep);
-    size_t     cached_pairs() const { return pair_rows_.size(); }
+    size_t     cached_pairs() const {
+        std::lock_guard<std::mutex> lock(pairs_mu_);
+        return pair_rows_.size();
+    }
 
 private:
     using PairKey = std::tuple<uint64_t, uint64_t, uint64_t, uint64_t>; // (uid, col) of each side, ordered
+    mutable std::mutex          pairs_mu_;
     std::map<PairKey, uint64_t> pair_rows_;
 };
 

@@ -75,9 +75,10 @@ def test_execute_many_reports_errors(ctx):
 
 @pytest.mark.parametrize("mode", [gj.EST_INDEPENDENCE, gj.EST_LEARNED])
 def test_execute_many_planned_here_matches_single_runs(ctx, mode):
-    """Under a model other than the max rule, the calling thread plans each
-    query while the workers execute the planned ones: the rows equal those
-    of one execute_device per plan under the max rule."""
+    """Under a model other than the max rule, each worker plans its queries
+    with the context's estimator (shared statistics catalog, inference on the
+    worker's stream) and executes them: the rows equal those of one
+    execute_device per plan under the max rule."""
     from helpers import check_table as ct
     ps = wl.c5(scale=0.002, nplans=24)
     pool = [ctx.generate(ps.seed, t.rows, t.columns) for t in ps.pool]
@@ -90,7 +91,7 @@ def test_execute_many_planned_here_matches_single_runs(ctx, mode):
     mlp = ctx.mlp(ctx.num_features(), seed=4) if mode == gj.EST_LEARNED else None
     ctx.set_estimator(mode, mlp, max_sigma=100.0)
     try:
-        say(f"execute_many planned here, estimator {mode}")
+        say(f"execute_many planned by the workers, estimator {mode}")
         res = ctx.execute_many(plans, tables, workers=4)
     finally:
         ctx.set_estimator(gj.EST_MAX)
