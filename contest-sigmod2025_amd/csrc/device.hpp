@@ -110,11 +110,14 @@ struct BlockIdx {
 
 // One scanned column as the kernels see it.
 struct ScanCol {
-    const uint8_t*  pages;     // npages x 8 KiB
-    const uint32_t* row_start; // npages + 1 prefix of rows per page
-    const BlockIdx* blocks;    // ceil(nrows / 64) block entries
+    const uint8_t*  pages;      // npages x 8 KiB
+    const uint32_t* row_start;  // npages + 1 prefix of rows per page
+    const BlockIdx* blocks;     // ceil(nrows / 64) block entries
     int32_t         type;
-    uint32_t        col_id;    // id used in string handles
+    uint32_t        col_id;     // id used in string handles
+    uint32_t        dense_rows; // > 0: fixed width, no NULLs, every page but the last holds this many
+                                // rows (>= 64) and the column fewer than 2^32: row r is value r % d of
+                                // page r / d, found without the block index
 };
 
 // Value slot of non-NULL value `nn` of page `p`.
@@ -163,8 +166,8 @@ __device__ inline BlockIdx load_block_idx(const BlockIdx* blocks, uint64_t block
     return bi;
 }
 
-__device__ inline uint64_t decode_block(const ScanCol& c, uint64_t block, bool active) {
-    const BlockIdx bi   = load_block_idx(c.blocks, block);
+// The vector part of decode_block, given the block's index entry.
+__device__ inline uint64_t decode_with(const ScanCol& c, const BlockIdx& bi, uint64_t block, bool active) {
     const uint32_t lane = lane_id();
     if (bi.complex) return active ? decode_row_slow(c, bi.page, uint32_t(block * 64 + lane)) : kNull;
     bool second = lane >= bi.split;
@@ -174,6 +177,22 @@ __device__ inline uint64_t decode_block(const ScanCol& c, uint64_t block, bool a
     uint32_t nn     = second ? __popcll(before & ~((bi.split >= 64 ? ~0ull : (1ull << bi.split)) - 1ull))
                              : bi.nn + __popcll(before);
     return load_value(c, bi.page + (second ? 1 : 0), nn, false);
+}
+
+// decode_block of a dense column: a block spans at most two pages (d >= 64).
+__device__ inline uint64_t decode_dense(const ScanCol& c, uint64_t block, bool active) {
+    if (!active) return kNull;
+    const uint32_t d  = c.dense_rows;
+    const uint32_t r0 = uint32_t(block) * 64;
+    const uint32_t p0 = r0 / d;
+    const uint32_t j  = r0 - p0 * d + lane_id();
+    const uint32_t s  = j >= d ? 1u : 0u;
+    return load_value(c, p0 + s, j - s * d, false);
+}
+
+__device__ inline uint64_t decode_block(const ScanCol& c, uint64_t block, bool active) {
+    if (c.dense_rows) return decode_dense(c, block, active);
+    return decode_with(c, load_block_idx(c.blocks, block), block, active);
 }
 
 } // namespace cgj

@@ -83,8 +83,9 @@ ScanCol DevColumn::view(uint32_t col_id) const {
     c.pages     = pages.get<uint8_t>();
     c.row_start = row_start.get<uint32_t>();
     c.blocks    = blocks.get<BlockIdx>();
-    c.type      = type;
-    c.col_id    = col_id;
+    c.type       = type;
+    c.col_id     = col_id;
+    c.dense_rows = dense_rows;
     return c;
 }
 
@@ -123,15 +124,20 @@ void Context::prepare_column(DevColumn& c) {
         return;
     }
     DevBuf scratch(heap_.get(), prepare_scratch_words(c.npages) * sizeof(uint32_t) + 16);
-    DevBuf total(heap_.get(), sizeof(uint64_t));
+    DevBuf total(heap_.get(), 2 * sizeof(uint64_t)); // rows, then the two dense words
+    CGJ_HIP(hipMemsetAsync(total.get(), 0, 2 * sizeof(uint64_t), stream_));
     CGJ_HIP(launch_prepare_column(c.pages.get<uint8_t>(), c.npages, c.type, c.nrows, c.row_start.get<uint32_t>(),
-                                  c.blocks.get<BlockIdx>(), scratch.get<uint32_t>(), total.get<uint64_t>(), stream_));
-    uint64_t rows = 0;
-    CGJ_HIP(hipMemcpyAsync(&rows, total.get(), sizeof(rows), hipMemcpyDeviceToHost, stream_));
+                                  c.blocks.get<BlockIdx>(), scratch.get<uint32_t>(), total.get<uint64_t>(),
+                                  total.get<uint32_t>() + 2, stream_));
+    uint64_t back[2] = {0, 0};
+    CGJ_HIP(hipMemcpyAsync(back, total.get(), sizeof(back), hipMemcpyDeviceToHost, stream_));
     CGJ_HIP(hipStreamSynchronize(stream_));
+    const uint64_t rows = back[0];
     if (rows != c.nrows)
         throw std::runtime_error("column pages hold " + std::to_string(rows) + " rows, table has " +
                                  std::to_string(c.nrows));
+    const uint32_t uneven = uint32_t(back[1]), n0 = uint32_t(back[1] >> 32);
+    c.dense_rows = (!uneven && n0 >= 64 && c.nrows < (uint64_t(1) << 32)) ? n0 : 0;
 }
 
 std::unique_ptr<DevTable> Context::upload(const cgj_host_table& t, const std::vector<bool>* used) {
@@ -273,6 +279,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "group_walk") options.group_walk = v >= 0 && v < 1e6 ? uint32_t(v) : 32u;
     else if (name == "force_grouped") options.force_grouped = v != 0;
     else if (name == "scan_build") options.scan_build = v != 0;
+    else if (name == "block_prefilter") options.block_prefilter = v != 0;
     else if (name == "scan_workgroups") options.scan_workgroups = v >= 16 && v <= 4096 ? uint32_t(v) : 512u;
     else return false;
     return true;
@@ -858,6 +865,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
     PipeArgs a;
     std::memset(&a, 0, sizeof(a));
     a.xcd_order = xcd_order ? 1 : 0;
+    a.prefilter = ctx_.options.block_prefilter ? 1 : 0;
     std::map<uint32_t, uint32_t> slot; // class -> tuple slot
     uint32_t                     nslots = 0;
     uint64_t                     nrows  = 0;

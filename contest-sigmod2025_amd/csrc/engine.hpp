@@ -89,6 +89,7 @@ struct DevColumn {
     DevBuf   pages;     // npages x 8 KiB
     DevBuf   row_start; // u32 [npages + 1]
     DevBuf   blocks;    // BlockIdx [ceil(nrows / 64)]
+    uint32_t dense_rows = 0; // see ScanCol::dense_rows
 
     ScanCol view(uint32_t col_id) const;
 };
@@ -164,6 +165,7 @@ struct Options {
     // Hash tables on a plain scan are partitioned straight from the pages
     // (no staged tuples).
     bool     scan_build            = true;
+    bool     block_prefilter       = true; // early bloom filters tested four 64-row blocks at a time first
     uint32_t scan_workgroups       = 512; // workgroups of the scan-driven partition (16..4096)
 };
 

@@ -24,13 +24,20 @@ namespace cgj {
 // ---------------------------------------------------------------------------
 constexpr int kScanBlock = 1024;
 
-__global__ void k_page_counts(const uint8_t* pages, uint64_t npages, int32_t type, uint32_t* counts) {
+// Rows per page; dense[0] becomes 1 unless every page holds no NULL and as
+// many rows as page 0 (the last page at most as many), dense[1] = rows of page 0.
+__global__ void k_page_counts(const uint8_t* pages, uint64_t npages, int32_t type, uint32_t* counts,
+                              uint32_t* dense) {
     uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= npages) return;
     uint32_t nr = *reinterpret_cast<const uint16_t*>(pages + p * kPageSize);
     uint32_t c  = nr;
     if (type == T_VARCHAR && nr >= 0xfffe) c = nr - 0xfffe; // long string: 1 row on its first page
     counts[p] = c;
+    const uint32_t nv = *reinterpret_cast<const uint16_t*>(pages + p * kPageSize + 2);
+    const uint32_t n0 = *reinterpret_cast<const uint16_t*>(pages);
+    if (p == 0) dense[1] = n0;
+    if (type == T_VARCHAR || nv != nr || (p + 1 < npages ? nr != n0 : nr > n0)) atomicOr(dense, 1u);
 }
 
 __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
@@ -177,12 +184,12 @@ hipError_t launch_exclusive_scan(uint32_t* data, uint64_t n, uint32_t* scratch, 
 
 hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t type, uint64_t nrows,
                                  uint32_t* row_start, BlockIdx* blocks, uint32_t* scratch, uint64_t* total_rows_dev,
-                                 hipStream_t s) {
+                                 uint32_t* dense, hipStream_t s) {
     if (npages == 0) return hipSuccess;
     uint32_t* counts  = scratch;
     uint32_t* sums    = scratch + npages;
     uint64_t  nblocks = (npages + kScanBlock - 1) / kScanBlock;
-    k_page_counts<<<dim3((npages + 255) / 256), dim3(256), 0, s>>>(pages, npages, type, counts);
+    k_page_counts<<<dim3((npages + 255) / 256), dim3(256), 0, s>>>(pages, npages, type, counts, dense);
     k_scan_local<<<dim3(nblocks), dim3(kScanBlock), 0, s>>>(counts, npages, sums);
     k_scan_sums<<<dim3(1), dim3(kScanBlock), 0, s>>>(sums, nblocks, total_rows_dev);
     k_scan_add<<<dim3((npages + 255) / 256), dim3(256), 0, s>>>(counts, npages, sums, total_rows_dev, row_start,
@@ -1063,8 +1070,65 @@ __device__ inline void finish_wave(const PipeArgs& a, const WaveState& ws, uint6
     }
 }
 
+// Early bloom filters of a pages-driven pipeline, tested before the full
+// per-block work. Returns the blocks (bit j = block `first + j`, j < nblk <=
+// 64) that hold a row passing every filter tested. Blocks go through eight at
+// a time, so a wave has eight blocks' index, key and filter loads in flight
+// where the full body has one. Column c (a bit of `bmask`) is tested against
+// the filter of probe (bprobe >> 4c) & 15, the first probe keyed on it; the
+// full cascade tests every filter again, so no block it would keep is
+// dropped. Only k_pipeline<8> uses it: in the other variants its registers
+// would cost a wave per SIMD (<4> would pass 128 VGPRs, <16> 168). A later
+// column's keys are decoded only for the rows that passed the earlier
+// filters: loading them ahead for every row costs more than the round trip
+// it saves.
 template <int NA>
-__global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
+__device__ inline uint64_t prefilter_blocks(const PipeArgs& a, uint64_t first, uint32_t nblk, uint64_t end,
+                                            uint32_t bmask, uint64_t bprobe) {
+    constexpr int  U    = 8;
+    const uint32_t lane = lane_id();
+    uint64_t       live = 0;
+    for (uint32_t g = 0; g < nblk; g += U) {
+        bool alive[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) alive[j] = g + j < nblk && (first + g + j) * 64 + lane < end;
+#pragma unroll
+        for (int c = 0; c < kMaxScan; ++c) {
+            if (c >= NA || !((bmask >> c) & 1u)) continue;
+            uint64_t any = 0;
+#pragma unroll
+            for (int j = 0; j < U; ++j) any |= __ballot(alive[j]);
+            if (!any) break;
+            const ProbeSpec& P     = a.probes[(bprobe >> (4 * c)) & 15u];
+            const bool       dense = a.scan[c].dense_rows != 0;
+            BlockIdx         bi[U] = {};
+            if (!dense) // index entries of all U blocks first, then their values
+#pragma unroll
+                for (int j = 0; j < U; ++j) bi[j] = load_block_idx(a.scan[c].blocks, first + g + (g + j < nblk ? j : 0));
+            uint64_t w[U], bb[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const uint64_t v  = dense ? decode_dense(a.scan[c], first + g + j, alive[j])
+                                          : decode_with(a.scan[c], bi[j], first + g + j, alive[j]);
+                alive[j]          = alive[j] && v != kNull;
+                const uint64_t h2 = hash_bloom(hash_key(v));
+                bb[j]             = bloom_bits(h2);
+                w[j]              = alive[j] ? P.bloom[bloom_word(h2, P.bloom_log)] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) alive[j] = alive[j] && (w[j] & bb[j]) == bb[j];
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+            if (__ballot(alive[j])) live |= 1ull << (g + j);
+    }
+    return live;
+}
+
+// The narrow variant is held to 4 waves per SIMD (at most 128 VGPRs): left
+// alone the allocator lands one register above and loses a wave.
+template <int NA>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4 : 1, 8))) void k_pipeline(PipeArgs a) {
     extern __shared__ uint64_t lds[];
     const uint32_t wib    = wave_in_block();
     const uint32_t stream = blockIdx.x * kWavesPerBlock + wib;
@@ -1097,10 +1161,36 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
         m_end         = hi;
         m_step        = uint64_t(bpx) * kWavesPerBlock;
     }
+    // Scanned columns keying an early bloom filter, and per column the first
+    // probe keyed on it (4 bits each).
+    uint32_t bmask  = 0;
+    uint64_t bprobe = 0;
+    if (a.driver == DRIVER_PAGES)
+        for (uint32_t p = a.nprobes; p-- > 0;)
+            if (a.probes[p].bloom && a.probes[p].early && a.probes[p].key_slot < uint32_t(NA)) {
+                const uint32_t c = a.probes[p].key_slot;
+                bmask |= 1u << c;
+                bprobe = (bprobe & ~(15ull << (4 * c))) | (uint64_t(p) << (4 * c));
+            }
+    // A wave tests the filters ahead of the full per-block work while at most
+    // half of the blocks it has tested survive them.
+    const bool try_prefilter = NA == 8 && bmask != 0 && a.prefilter;
+    uint32_t   pf_seen = 0, pf_live = 0;
+
     for (uint64_t m = m_first; m < m_end; m += m_step) {
         uint64_t r0  = m * a.morsel_rows;
         uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
-        for (uint64_t rb = r0; rb < end; rb += 64) {
+        for (uint64_t cb = r0; cb < end; cb += 64 * 64) { // runs of 64 blocks (one morsel at most 4096 rows)
+        const uint64_t cend = cb + 64 * 64 < end ? cb + 64 * 64 : end;
+        const uint32_t nblk = uint32_t((cend - cb + 63) / 64);
+        uint64_t       live = nblk >= 64 ? ~0ull : ((1ull << nblk) - 1);
+        if (try_prefilter && (pf_seen < 256 || 2 * pf_live <= pf_seen)) {
+            live = prefilter_blocks<NA>(a, cb / 64, nblk, end, bmask, bprobe);
+            pf_seen += nblk;
+            pf_live += uint32_t(__popcll(live));
+        }
+        for (; live; live &= live - 1) {
+            const uint64_t rb = cb + 64 * uint64_t(__builtin_ctzll(live));
             uint64_t row    = rb + lane;
             bool     active = row < end;
             uint64_t s[NA];
@@ -1115,9 +1205,6 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                 // Filters commute, so the cascade runs in column order; every
                 // column index is a compile-time constant (no runtime-indexed
                 // register array).
-                uint32_t bmask = 0; // scanned columns keying an early bloom filter
-                for (uint32_t p = 0; p < a.nprobes; ++p)
-                    if (a.probes[p].bloom && a.probes[p].early) bmask |= 1u << a.probes[p].key_slot;
                 uint32_t decoded = 0; // wave-uniform bitmask of decoded columns
 #pragma unroll
                 for (int c = 0; c < kMaxScan; ++c) {
@@ -1320,6 +1407,7 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a) {
                 if (!__ballot(alive)) break;
             }
         }
+        } // runs of 64 blocks
     }
 
     if (a.target == TARGET_PAGES) {

@@ -94,6 +94,7 @@ struct PipeArgs {
     PipeCounters*   counters;
     uint32_t        page_chunk;  // pool pages reserved per atomic (1..kPageChunk)
     uint32_t        grid_per_cu; // host only: waves per CU of the launch (0 = default)
+    uint32_t        prefilter;   // pages driver: test early bloom filters four blocks at a time first
 };
 
 // ---- lean one-probe pipeline ----
@@ -213,10 +214,13 @@ struct RadixArgs {
 hipError_t launch_radix_partition(const RadixArgs& a, uint32_t wgs, uint32_t* scan_scratch, uint64_t* total,
                                   hipStream_t s);
 
-// Column preparation: rows per page prefix and the block index.
+// Column preparation: rows per page prefix and the block index. `dense`
+// (two words, zeroed by the caller) receives 1 in word 0 unless every page is
+// NULL-free with as many rows as page 0 (the last at most as many), and the
+// rows of page 0 in word 1.
 hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t type, uint64_t nrows,
                                  uint32_t* row_start, BlockIdx* blocks, uint32_t* scratch, uint64_t* total_rows_dev,
-                                 hipStream_t s);
+                                 uint32_t* dense, hipStream_t s);
 size_t     prepare_scratch_words(uint64_t npages);
 
 hipError_t launch_decode(const ScanCol& c, uint64_t nrows, uint64_t* out, hipStream_t s);

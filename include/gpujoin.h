@@ -142,7 +142,9 @@ typedef struct {
  *   "force_grouped": build every hash table grouped by key (1) or only when needed (0);
  *   "scan_build": hash tables on a plain scan are partitioned straight from its pages (1,
  *       default) or from staged tuples (0);
- *   "scan_workgroups": workgroups of that partition (16..4096, default 512). */
+ *   "scan_workgroups": workgroups of that partition (16..4096, default 512);
+ *   "block_prefilter": a pipeline over pages tests its early bloom filters on four 64-row
+ *       blocks at a time before the full per-block work (1, default) or not (0). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */

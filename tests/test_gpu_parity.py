@@ -48,6 +48,32 @@ def test_random_plans_vs_oracle(ctx, seed):
     assert got == exp
 
 
+@pytest.mark.parametrize("seed", range(16))
+def test_block_prefilter_with_bloom_filters_on_every_join(ctx, seed):
+    """Every join gets a bloom filter, so pipelines over pages with eight
+    tuple slots test their early filters on four blocks at a time before the
+    per-block work. Sparse key domains leave most blocks without a match.
+    Rows equal the oracle's and those with the block pre-filter off."""
+    rng = random.Random(7000 + seed)
+    ntables = rng.randint(3, 6)
+    max_rows = rng.choice([3000, 20000, 60000])
+    key_type = rng.choice([INT32, INT64])
+    domain = rng.choice([max_rows, 4 * max_rows, 40 * max_rows])
+    plan = random_plan(rng, ntables=ntables, max_rows=max_rows, key_type=key_type, domain=domain)
+    types = [t for _, t in plan.nodes[plan.root].output_attrs]
+    say(f"block pre-filter seed={seed} tables={ntables} rows<={max_rows} domain={domain}")
+    ctx.set_option("bloom_bits", 16)
+    try:
+        got = check_table(ctx.execute(plan), types)
+        ctx.set_option("block_prefilter", 0)
+        off = check_table(ctx.execute(plan), types)
+    finally:
+        ctx.set_option("block_prefilter", 1)
+        ctx.set_option("bloom_bits", -1)
+    assert got == off
+    assert got == check_table(oracle.execute(plan), types)
+
+
 @pytest.fixture
 def grouped_ctx(ctx):
     """The context with every hash table built grouped by key."""
@@ -205,6 +231,29 @@ def test_decode_ragged_sizes(ctx, n):
     col = gj.encode_column(INT32, rng.integers(0, 1000, n), valid)
     dt = ctx.upload(gj.ColumnarTable(n, [col]))
     assert np.array_equal(dt.decode(0), oracle.decode_fixed(col, n))
+
+
+@pytest.mark.parametrize("n", [1, 64, 1006, 1007, 1008, 1983, 1984, 1985, 4097, 70_001])
+@pytest.mark.parametrize("t", [INT32, INT64])
+def test_decode_dense_columns(ctx, t, n):
+    """NULL-free columns with full pages are read without the block index
+    (row r is value r % d of page r / d): sizes around the 1984-row INT32 and
+    1007-row INT64 pages, and blocks that cross a page boundary."""
+    rng = np.random.default_rng(n + 7 * t)
+    col = gj.encode_column(t, rng.integers(0, 1 << 31, n), np.ones(n, dtype=bool))
+    dt = ctx.upload(gj.ColumnarTable(n, [col]))
+    assert np.array_equal(dt.decode(0), oracle.decode_fixed(col, n))
+
+
+def test_decode_uneven_pages_not_taken_as_dense(ctx):
+    """NULL-free pages of unequal fill (a 100-row page before full ones) must
+    be read through the block index, not as fixed rows per page."""
+    rng = np.random.default_rng(3)
+    a = gj.encode_column(INT32, rng.integers(0, 1 << 31, 100), np.ones(100, dtype=bool))
+    b = gj.encode_column(INT32, rng.integers(0, 1 << 31, 5000), np.ones(5000, dtype=bool))
+    col = gj.Column(INT32, a.pages + b.pages)
+    dt = ctx.upload(gj.ColumnarTable(5100, [col]))
+    assert np.array_equal(dt.decode(0), oracle.decode_fixed(col, 5100))
 
 
 def c1_plan(nb, np_, seed=0x5EED0001):
