@@ -413,7 +413,9 @@ private:
         const TreeNode& p = t_.nodes[join.probe];
         return uint64_t(std::min(size_bound(p.set, p.card), 1.8e19));
     }
-    const HashTable* make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est);
+    // known_grouped: the key column is known to need the grouped layout.
+    const HashTable* make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est,
+                                bool known_grouped);
     // Hash table on a plain scan of relation `rel`, partitioned straight from
     // its pages (no staged tuples). False when that path does not apply or the
     // table needs more than the plain partitioned build (the caller stages).
@@ -559,7 +561,8 @@ uint64_t Executor::launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& 
     return waves;
 }
 
-const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est) {
+const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t>& eqs, uint64_t probe_est,
+                                      bool known_grouped) {
     auto ht    = std::make_unique<HashTable>();
     ht->eqs    = eqs;
     ht->words  = uint32_t(eqs.size());
@@ -599,7 +602,7 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
     // rebuilt grouped instead of finishing a quadratic insert.
     const bool     can_group = ctx_.options.group_walk > 0;
     const uint32_t walk_cap  = can_group ? ctx_.options.group_walk : 0;
-    bool           done = false, group = ctx_.options.force_grouped;
+    bool           done = false, group = ctx_.options.force_grouped || (known_grouped && can_group);
     auto           read_flags = [&](uint32_t* f) {
         CGJ_HIP(hipMemcpyAsync(f, flags.get(), 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx_.stream()));
         CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
@@ -725,7 +728,12 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
         probes.push_back(ProbeStep{ht, sh.front(), bn.set});
         cur |= bn.set;
     }
-    if (probes.empty() && ctx_.options.scan_build) {
+    // A plain scan's key column may be known to need the grouped layout.
+    const bool     plain   = probes.empty();
+    const uint64_t key_uid = plain ? inputs_[q_.rels[leaf].table]->uid : 0;
+    const uint64_t key_col = plain ? q_.attrs[q_.attr_of(uint32_t(leaf), key_eq)].col : 0;
+    const bool     known   = plain && ctx_.build_hints().grouped(key_uid, key_col);
+    if (plain && ctx_.options.scan_build && !known) {
         const HashTable* ht = nullptr;
         if (scan_build(uint32_t(leaf), eqs, probe_rows, ht)) return ht; // nullptr: empty build side
     }
@@ -734,7 +742,9 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
     d.set = 1ull << leaf;
     pipeline(d, probes, t);
     if (t.count == 0) return nullptr; // empty build side: the inner join is empty
-    return make_table(t, eqs, probe_rows);
+    const HashTable* ht = make_table(t, eqs, probe_rows, known);
+    if (plain && !known && ht->grouped && !ctx_.options.force_grouped) ctx_.build_hints().mark_grouped(key_uid, key_col);
+    return ht;
 }
 
 bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64_t probe_est, const HashTable*& out) {
@@ -1092,6 +1102,7 @@ std::vector<std::unique_ptr<DevResult>> Context::execute_many(const std::vector<
     while (workers_.size() < workers) {
         workers_.push_back(std::make_unique<Context>(device_));
         workers_.back()->options = options;
+        workers_.back()->hints_  = hints_;
     }
     for (uint32_t w = 0; w < workers; ++w) workers_[w]->options = options;
     // Each worker plans its query with this context's estimator (statistics

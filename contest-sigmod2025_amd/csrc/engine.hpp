@@ -8,6 +8,8 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -96,6 +98,26 @@ struct DevColumn {
 
 // Process-wide distinct id of a device table (key of cached join statistics).
 uint64_t next_table_uid();
+
+// Key columns (table uid, column) of base tables on which a hash table had to
+// be built grouped: a key repeated past the walk cap. A later build on such a
+// column starts grouped instead of failing the plain layouts first. A context
+// shares its hints with its worker contexts.
+class BuildHints {
+public:
+    bool grouped(uint64_t uid, uint64_t col) const {
+        std::lock_guard<std::mutex> lock(mu_);
+        return grouped_.count({uid, col}) != 0;
+    }
+    void mark_grouped(uint64_t uid, uint64_t col) {
+        std::lock_guard<std::mutex> lock(mu_);
+        grouped_.insert({uid, col});
+    }
+
+private:
+    mutable std::mutex                       mu_;
+    std::set<std::pair<uint64_t, uint64_t>> grouped_;
+};
 
 struct DevTable {
     uint64_t               nrows = 0;
@@ -236,6 +258,7 @@ public:
     // Cardinality model of the join ordering (the max-of-inputs rule unless
     // configured otherwise) and its cached join statistics.
     Estimator& estimator() { return *est_; }
+    BuildHints& build_hints() { return *hints_; }
     // Result rows of an imported query (any join order; no pages written).
     uint64_t count_query(const Query& q, const std::vector<const DevTable*>& inputs);
     // Result rows of the join of the connected relation set `relset` of a plan
@@ -271,6 +294,7 @@ private:
     hipStream_t stream_ = nullptr;
     std::shared_ptr<DeviceHeap> heap_ = std::make_shared<DeviceHeap>();
     std::unique_ptr<Estimator>  est_;
+    std::shared_ptr<BuildHints> hints_ = std::make_shared<BuildHints>();
     std::vector<std::unique_ptr<Context>> workers_;
 };
 
