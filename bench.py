@@ -271,7 +271,7 @@ def run_c5(args):
             ctx.execute_device(ps.plans[pid].plan, est.plan_inputs(tabs, ps.plans[pid])).free()
         t_rows = 0
         ms_plan = 0.0
-        pipelines = retries = 0
+        pipelines = retries = retries_build = 0
         ms_build = ms_probe = 0.0
         per_step = []
         for step in range(args.steps):
@@ -285,6 +285,7 @@ def run_c5(args):
                 ms_build += st["ms_build"]
                 ms_probe += st["ms_probe"]
                 retries += st["retries"]
+                retries_build += st["retries_build"]
                 t_rows += r.num_rows
                 r.free()
                 if k % 100 == 99:
@@ -293,6 +294,7 @@ def run_c5(args):
         results[name] = {"s_per_step": min(per_step), "ms_plan_per_step": ms_plan / args.steps,
                          "rows_per_step": t_rows // args.steps, "s_statistics_untimed": round(s_stats, 2),
                          "pipelines_per_step": pipelines // args.steps, "retries_per_step": retries // args.steps,
+                         "build_retries_per_step": retries_build // args.steps,
                          "ms_device_build_per_step": round(ms_build / args.steps, 1),
                          "ms_device_probe_per_step": round(ms_probe / args.steps, 1)}
         log(f"{name}: {results[name]}")

@@ -362,7 +362,7 @@ public:
     std::unique_ptr<DevResult> run(bool count_only = false);
 
     double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0, ms_partition = 0;
-    uint32_t launches = 0, retries = 0, probe_lean = 0, probe_waves = 0;
+    uint32_t launches = 0, retries = 0, retries_build = 0, probe_lean = 0, probe_waves = 0;
     uint64_t build_tuples = 0, probe_rows = 0, build_bytes = 0, out_ref_pages = 0, probe_input_pages = 0;
 
 private:
@@ -629,7 +629,8 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
             ht->rmask  = (1ull << rs_log) - 1;
             ht->unique = f[0] == 0;
         } else {
-            ++retries; // a region overflowed or an insert walked too far (heavy duplicates)
+            ++retries;
+            ++retries_build; // a region overflowed or an insert walked too far (heavy duplicates)
             group = can_group;
         }
     }
@@ -645,6 +646,7 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
             ht->unique = f[0] == 0;
         } else {
             ++retries;
+            ++retries_build;
             group = true;
         }
     }
@@ -686,6 +688,7 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
             if (!f[1]) break;
             if (attempt >= 3) throw std::runtime_error("grouped hash table: regions overflow");
             ++retries;
+            ++retries_build;
             ++lg;
         }
         ht->grouped = true;
@@ -728,11 +731,20 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
         probes.push_back(ProbeStep{ht, sh.front(), bn.set});
         cur |= bn.set;
     }
-    // A plain scan's key column may be known to need the grouped layout.
-    const bool     plain   = probes.empty();
-    const uint64_t key_uid = plain ? inputs_[q_.rels[leaf].table]->uid : 0;
-    const uint64_t key_col = plain ? q_.attrs[q_.attr_of(uint32_t(leaf), key_eq)].col : 0;
-    const bool     known   = plain && ctx_.build_hints().grouped(key_uid, key_col);
+    // The key column may be known to need the grouped layout: for a plain
+    // scan its own column, for a join result the base column its key values
+    // come from (the first relation of the set in the key's class).
+    const bool plain   = probes.empty();
+    uint32_t   key_rel = uint32_t(leaf);
+    if (!plain)
+        for (uint32_t r = 0; r < q_.rels.size(); ++r)
+            if (((tn.set >> r) & 1u) && std::binary_search(q_.rels[r].eqs.begin(), q_.rels[r].eqs.end(), key_eq)) {
+                key_rel = r;
+                break;
+            }
+    const uint64_t key_uid = inputs_[q_.rels[key_rel].table]->uid;
+    const uint64_t key_col = q_.attrs[q_.attr_of(key_rel, key_eq)].col;
+    const bool     known   = ctx_.build_hints().grouped(key_uid, key_col);
     if (plain && ctx_.options.scan_build && !known) {
         const HashTable* ht = nullptr;
         if (scan_build(uint32_t(leaf), eqs, probe_rows, ht)) return ht; // nullptr: empty build side
@@ -743,7 +755,7 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
     pipeline(d, probes, t);
     if (t.count == 0) return nullptr; // empty build side: the inner join is empty
     const HashTable* ht = make_table(t, eqs, probe_rows, known);
-    if (plain && !known && ht->grouped && !ctx_.options.force_grouped) ctx_.build_hints().mark_grouped(key_uid, key_col);
+    if (!known && ht->grouped && !ctx_.options.force_grouped) ctx_.build_hints().mark_grouped(key_uid, key_col);
     return ht;
 }
 
@@ -820,6 +832,7 @@ bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64
     if (n == 0) return true; // empty build side
     if (f[1] || f[2]) { // a region overflowed or a key repeats heavily: the staged path handles it
         ++retries;
+        ++retries_build;
         return false;
     }
     ht->tuples = n;
@@ -1253,6 +1266,7 @@ std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const st
     stats.output_ref_pages  = ex.out_ref_pages;
     stats.pipelines         = ex.launches;
     stats.retries           = ex.retries;
+    stats.retries_build     = ex.retries_build;
     uint64_t in_pages       = 0;
     for (uint32_t ri = 0; ri < q.rels.size(); ++ri) {
         const QRel& r = q.rels[ri];

@@ -96,7 +96,7 @@ class Stats(C.Structure):
         ("est_sets", C.c_uint32),
         ("est_fallbacks", C.c_uint32),
         ("est_stat_joins", C.c_uint32),
-        ("est_pad", C.c_uint32),
+        ("retries_build", C.c_uint32),
     ]
 
     def as_dict(self):

@@ -106,7 +106,8 @@ typedef struct {
     uint64_t probe_rows;     /* rows streamed by the output pipeline's scan    */
     uint64_t output_rows;
     uint32_t pipelines;      /* number of pipeline launches                    */
-    uint32_t retries;        /* re-runs after a page-pool or staging overflow  */
+    uint32_t retries;        /* re-runs: hash-table builds that changed layout, staging or
+                                page-pool overflows                             */
     uint64_t probe_input_pages; /* input pages scanned by the output pipeline  */
     uint64_t output_ref_pages;  /* output pages at the reference's packing:
                                    ceil((rows + width*nonnull)/bit budget) per
@@ -122,7 +123,7 @@ typedef struct {
     uint32_t est_sets;          /* connected relation sets estimated by the network */
     uint32_t est_fallbacks;     /* ... of them, by the independence rule (low confidence) */
     uint32_t est_stat_joins;    /* 2-way counts run (pairs not yet in the catalog) */
-    uint32_t est_pad;
+    uint32_t retries_build;     /* ... of them, builds that changed layout (walk cap, region overflow) */
 } cgj_stats;
 
 /* Execution options of a context (0 = success, -1 = unknown name):
