@@ -1,4 +1,5 @@
 // C ABI (include/gpujoin.h) and the C++ contest surface (include/contest/plan.h).
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -99,13 +100,23 @@ int cgj_execute(cgj_context* ctx, const cgj_plan* plan, const cgj_host_table* in
                 if (nd.output_attrs[a].index < used[nd.base_table_id].size())
                     used[nd.base_table_id][nd.output_attrs[a].index] = true;
         }
+        using clock  = std::chrono::steady_clock;
+        auto ms      = [](clock::time_point a, clock::time_point b) {
+            return std::chrono::duration<double, std::milli>(b - a).count();
+        };
+        const auto t0 = clock::now();
         for (uint32_t i = 0; i < num_inputs; ++i) {
             auto dt = ctx->impl->upload(inputs[i], &used[i]);
             views.push_back(dt.get());
             owned.push_back(std::move(dt));
         }
-        auto res = ctx->impl->execute(*plan, views);
+        const auto t1  = clock::now();
+        auto       res = ctx->impl->execute(*plan, views);
+        const auto t2  = clock::now();
         ctx->impl->download(*res, *out);
+        const auto t3 = clock::now();
+        ctx->impl->stats.ms_host_upload   = ms(t0, t1);
+        ctx->impl->stats.ms_host_download = ms(t2, t3);
     });
 }
 

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <numeric>
 #include <thread>
@@ -109,6 +110,10 @@ Context::Context(int device) : device_(device), est_(std::make_unique<Estimator>
 
 Context::~Context() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (int b = 0; b < 2; ++b) {
+        if (pin_ev_[b]) (void)hipEventDestroy(pin_ev_[b]);
+        if (pin_[b]) (void)hipHostFree(pin_[b]);
+    }
     workers_.clear();
     est_.reset(); // its network's buffers go back to the heap
     heap_->trim();
@@ -140,42 +145,78 @@ void Context::prepare_column(DevColumn& c) {
     c.dense_rows = (!uneven && n0 >= 64 && c.nrows < (uint64_t(1) << 32)) ? n0 : 0;
 }
 
+uint8_t* Context::staging(int b) {
+    if (!pin_[b]) {
+        CGJ_HIP(hipHostMalloc(&pin_[b], kStagingBytes, hipHostMallocDefault));
+        CGJ_HIP(hipEventCreateWithFlags(&pin_ev_[b], hipEventDisableTiming));
+    } else {
+        CGJ_HIP(hipEventSynchronize(pin_ev_[b]));
+    }
+    return static_cast<uint8_t*>(pin_[b]);
+}
+
+void Context::staging_done(int b) { CGJ_HIP(hipEventRecord(pin_ev_[b], stream_)); }
+
+namespace {
+// Runs f(i) for i in [0, n) on up to `max_threads` host threads (one below
+// 512 items): gathers and scatters of 8 KiB pages through the staging
+// buffers. The first exception of any thread is rethrown after all have
+// finished.
+void parallel_pages(uint64_t n, const std::function<void(uint64_t)>& f, uint32_t max_threads) {
+    const uint32_t nt =
+        n < 512 ? 1u : std::min<uint32_t>(max_threads, std::max(1u, std::thread::hardware_concurrency()));
+    if (nt == 1) {
+        for (uint64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread>        th;
+    std::vector<std::exception_ptr> err(nt);
+    for (uint32_t t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            try {
+                for (uint64_t i = n * t / nt; i < n * (t + 1) / nt; ++i) f(i);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    for (auto& x : th) x.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+} // namespace
+
 std::unique_ptr<DevTable> Context::upload(const cgj_host_table& t, const std::vector<bool>* used) {
     auto dt   = std::make_unique<DevTable>();
     dt->nrows = t.num_rows;
-    const size_t kChunk = size_t(64) << 20;
-    void*        pinned = nullptr;
-    CGJ_HIP(hipHostMalloc(&pinned, kChunk, hipHostMallocDefault));
-    try {
-        for (uint32_t ci = 0; ci < t.num_columns; ++ci) {
-            const cgj_host_column& hc = t.columns[ci];
-            DevColumn              c;
-            c.type   = hc.type;
-            c.nrows  = t.num_rows;
-            if (used && !(*used)[ci]) { // never scanned: keep an empty placeholder
-                dt->cols.push_back(std::move(c));
-                continue;
-            }
-            c.npages = hc.num_pages;
-            c.pages  = DevBuf(heap_.get(), c.npages * kPageSize + 64);
-            uint64_t per = kChunk / kPageSize;
-            for (uint64_t p0 = 0; p0 < c.npages; p0 += per) {
-                uint64_t n = std::min(per, c.npages - p0);
-                CGJ_HIP(hipStreamSynchronize(stream_));
-                for (uint64_t i = 0; i < n; ++i)
-                    std::memcpy(static_cast<uint8_t*>(pinned) + i * kPageSize, hc.pages[p0 + i], kPageSize);
-                CGJ_HIP(hipMemcpyAsync(c.pages.get<uint8_t>() + p0 * kPageSize, pinned, n * kPageSize,
-                                       hipMemcpyHostToDevice, stream_));
-            }
-            prepare_column(c);
+    // Chunks alternate between the two staging buffers: gathering chunk k+1
+    // on the host overlaps the copy of chunk k.
+    const uint64_t per   = kStagingBytes / kPageSize;
+    int            chunk = 0;
+    for (uint32_t ci = 0; ci < t.num_columns; ++ci) {
+        const cgj_host_column& hc = t.columns[ci];
+        DevColumn              c;
+        c.type  = hc.type;
+        c.nrows = t.num_rows;
+        if (used && !(*used)[ci]) { // never scanned: keep an empty placeholder
             dt->cols.push_back(std::move(c));
+            continue;
         }
-        CGJ_HIP(hipStreamSynchronize(stream_));
-    } catch (...) {
-        (void)hipHostFree(pinned);
-        throw;
+        c.npages = hc.num_pages;
+        c.pages  = DevBuf(heap_.get(), c.npages * kPageSize + 64);
+        for (uint64_t p0 = 0; p0 < c.npages; p0 += per, ++chunk) {
+            const uint64_t n   = std::min(per, c.npages - p0);
+            const int      b   = chunk & 1;
+            uint8_t*       buf = staging(b);
+            parallel_pages(
+                n, [&](uint64_t i) { std::memcpy(buf + i * kPageSize, hc.pages[p0 + i], kPageSize); }, 4);
+            CGJ_HIP(hipMemcpyAsync(c.pages.get<uint8_t>() + p0 * kPageSize, buf, n * kPageSize,
+                                   hipMemcpyHostToDevice, stream_));
+            staging_done(b);
+        }
+        prepare_column(c);
+        dt->cols.push_back(std::move(c));
     }
-    CGJ_HIP(hipHostFree(pinned));
+    CGJ_HIP(hipStreamSynchronize(stream_));
     return dt;
 }
 
@@ -1324,38 +1365,43 @@ void Context::download_pool(const PoolView& r, cgj_result& out) {
     uint64_t used = 0;
     for (auto& col : pages)
         for (uint32_t p : col) used = std::max<uint64_t>(used, uint64_t(p) + 1);
-    const uint64_t kChunkPages = 8192; // 64 MiB staging
-    void*          pinned      = nullptr;
-    CGJ_HIP(hipHostMalloc(&pinned, kChunkPages * kPageSize, hipHostMallocDefault));
-    try {
-        for (size_t c = 0; c < r.types.size(); ++c) {
-            delete[] out.columns[c].pages;
-            out.columns[c].pages     = new void*[pages[c].size() ? pages[c].size() : 1];
-            out.columns[c].num_pages = pages[c].size();
-            for (size_t i = 0; i < pages[c].size(); ++i) out.columns[c].pages[i] = nullptr;
-        }
-        // copy the pool in chunks and scatter pages to their columns
-        std::vector<std::pair<uint32_t, uint32_t>> where(used, {~0u, ~0u});
-        for (size_t c = 0; c < r.types.size(); ++c)
-            for (size_t i = 0; i < pages[c].size(); ++i) where[pages[c][i]] = {uint32_t(c), uint32_t(i)};
-        for (uint64_t p0 = 0; p0 < used; p0 += kChunkPages) {
-            uint64_t n = std::min(kChunkPages, used - p0);
-            CGJ_HIP(hipMemcpyAsync(pinned, r.pool + p0 * kPageSize, n * kPageSize,
-                                   hipMemcpyDeviceToHost, stream_));
-            CGJ_HIP(hipStreamSynchronize(stream_));
-            for (uint64_t i = 0; i < n; ++i) {
-                auto w = where[p0 + i];
-                if (w.first == ~0u) continue;
-                void* pg = ::operator new(kPageSize);
-                std::memcpy(pg, static_cast<uint8_t*>(pinned) + i * kPageSize, kPageSize);
-                out.columns[w.first].pages[w.second] = pg;
-            }
-        }
-    } catch (...) {
-        (void)hipHostFree(pinned);
-        throw;
+    const uint64_t kChunkPages = kStagingBytes / kPageSize;
+    for (size_t c = 0; c < r.types.size(); ++c) {
+        delete[] out.columns[c].pages;
+        out.columns[c].pages     = new void*[pages[c].size() ? pages[c].size() : 1];
+        out.columns[c].num_pages = pages[c].size();
+        for (size_t i = 0; i < pages[c].size(); ++i) out.columns[c].pages[i] = nullptr;
     }
-    CGJ_HIP(hipHostFree(pinned));
+    // Copy the pool in chunks, alternating between the two staging buffers,
+    // and scatter each chunk's pages to their columns while the next chunk's
+    // copy runs.
+    std::vector<std::pair<uint32_t, uint32_t>> where(used, {~0u, ~0u});
+    for (size_t c = 0; c < r.types.size(); ++c)
+        for (size_t i = 0; i < pages[c].size(); ++i) where[pages[c][i]] = {uint32_t(c), uint32_t(i)};
+    const uint64_t nchunks = (used + kChunkPages - 1) / kChunkPages;
+    auto issue = [&](uint64_t k) {
+        const uint64_t p0 = k * kChunkPages, n = std::min(kChunkPages, used - p0);
+        CGJ_HIP(hipMemcpyAsync(staging(int(k & 1)), r.pool + p0 * kPageSize, n * kPageSize, hipMemcpyDeviceToHost,
+                               stream_));
+        staging_done(int(k & 1));
+    };
+    if (nchunks) issue(0);
+    for (uint64_t k = 0; k < nchunks; ++k) {
+        if (k + 1 < nchunks) issue(k + 1);
+        const uint8_t* buf = staging(int(k & 1)); // waits for chunk k's copy
+        const uint64_t p0 = k * kChunkPages, n = std::min(kChunkPages, used - p0);
+        // fresh operator-new pages fault on first touch: more threads than the gather
+        parallel_pages(
+            n,
+            [&](uint64_t i) {
+                const auto w = where[p0 + i];
+                if (w.first == ~0u) return;
+                void* pg = ::operator new(kPageSize);
+                std::memcpy(pg, buf + i * kPageSize, kPageSize);
+                out.columns[w.first].pages[w.second] = pg;
+            },
+            8);
+    }
 }
 
 } // namespace cgj

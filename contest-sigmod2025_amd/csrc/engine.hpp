@@ -247,6 +247,12 @@ public:
     uint64_t count(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
     void     download(const DevResult& r, cgj_result& out);
     void     download_pool(const PoolView& v, cgj_result& out);
+    // Pinned host staging of page uploads and downloads: two buffers of
+    // kStagingBytes, allocated on first use and kept for the context's
+    // lifetime; staging(b) waits until the last copy through buffer b is done.
+    static constexpr size_t kStagingBytes = size_t(64) << 20;
+    uint8_t*                staging(int b);
+    void                    staging_done(int b); // records the copy just issued through buffer b
     double   copy_probe(uint64_t bytes, uint32_t reps);
 
     cgj_stats   stats{};
@@ -295,6 +301,8 @@ private:
     std::shared_ptr<DeviceHeap> heap_ = std::make_shared<DeviceHeap>();
     std::unique_ptr<Estimator>  est_;
     std::shared_ptr<BuildHints> hints_ = std::make_shared<BuildHints>();
+    void*                       pin_[2]    = {nullptr, nullptr};
+    hipEvent_t                  pin_ev_[2] = {nullptr, nullptr};
     std::vector<std::unique_ptr<Context>> workers_;
 };
 

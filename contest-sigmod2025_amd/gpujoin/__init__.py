@@ -97,6 +97,8 @@ class Stats(C.Structure):
         ("est_fallbacks", C.c_uint32),
         ("est_stat_joins", C.c_uint32),
         ("retries_build", C.c_uint32),
+        ("ms_host_upload", C.c_double),
+        ("ms_host_download", C.c_double),
     ]
 
     def as_dict(self):
@@ -291,7 +293,10 @@ class _Marshal:
         for t, tab in enumerate(tables):
             cols = (HostColumn * max(len(tab.columns), 1))()
             for c, col in enumerate(tab.columns):
-                bufs = [C.create_string_buffer(bytes(p), PAGE_SIZE) for p in col.pages]
+                # the library only reads input pages: a full-size bytes page is
+                # passed by the address of its own buffer, anything else copied
+                bufs = [C.c_char_p(p) if type(p) is bytes and len(p) == PAGE_SIZE
+                        else C.create_string_buffer(bytes(p), PAGE_SIZE) for p in col.pages]
                 ptrs = (C.c_void_p * max(len(bufs), 1))(*[C.cast(b, C.c_void_p) for b in bufs])
                 self.keep.extend(bufs)
                 self.keep.append(ptrs)

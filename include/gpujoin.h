@@ -124,6 +124,9 @@ typedef struct {
     uint32_t est_fallbacks;     /* ... of them, by the independence rule (low confidence) */
     uint32_t est_stat_joins;    /* 2-way counts run (pairs not yet in the catalog) */
     uint32_t retries_build;     /* ... of them, builds that changed layout (walk cap, region overflow) */
+    double   ms_host_upload;    /* cgj_execute only: wall time of copying the scanned host pages in */
+    double   ms_host_download;  /* cgj_execute only: wall time of copying the result pages out into
+                                   one operator-new block per page */
 } cgj_stats;
 
 /* Execution options of a context (0 = success, -1 = unknown name):
