@@ -776,16 +776,16 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
     // scan its own column, for a join result the base column its key values
     // come from (the first relation of the set in the key's class).
     const bool plain   = probes.empty();
-    uint32_t   key_rel = uint32_t(leaf);
-    if (!plain)
-        for (uint32_t r = 0; r < q_.rels.size(); ++r)
-            if (((tn.set >> r) & 1u) && std::binary_search(q_.rels[r].eqs.begin(), q_.rels[r].eqs.end(), key_eq)) {
-                key_rel = r;
-                break;
-            }
-    const uint64_t key_uid = inputs_[q_.rels[key_rel].table]->uid;
-    const uint64_t key_col = q_.attrs[q_.attr_of(key_rel, key_eq)].col;
-    const bool     known   = ctx_.build_hints().grouped(key_uid, key_col);
+    bool       has_key = false; // a relation of the set produces the key class (always, for a join key)
+    uint32_t   key_rel = 0;
+    for (uint32_t r = 0; r < q_.rels.size() && !has_key; ++r)
+        if (((tn.set >> r) & 1u) && std::binary_search(q_.rels[r].eqs.begin(), q_.rels[r].eqs.end(), key_eq)) {
+            key_rel = r;
+            has_key = true;
+        }
+    const uint64_t key_uid = has_key ? inputs_[q_.rels[key_rel].table]->uid : 0;
+    const uint64_t key_col = has_key ? q_.attrs[q_.attr_of(key_rel, key_eq)].col : 0;
+    const bool     known   = has_key && ctx_.build_hints().grouped(key_uid, key_col);
     if (plain && ctx_.options.scan_build && !known) {
         const HashTable* ht = nullptr;
         if (scan_build(uint32_t(leaf), eqs, probe_rows, ht)) return ht; // nullptr: empty build side
@@ -796,7 +796,8 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
     pipeline(d, probes, t);
     if (t.count == 0) return nullptr; // empty build side: the inner join is empty
     const HashTable* ht = make_table(t, eqs, probe_rows, known);
-    if (!known && ht->grouped && !ctx_.options.force_grouped) ctx_.build_hints().mark_grouped(key_uid, key_col);
+    if (has_key && !known && ht->grouped && !ctx_.options.force_grouped)
+        ctx_.build_hints().mark_grouped(key_uid, key_col);
     return ht;
 }
 
