@@ -51,15 +51,18 @@ def test_random_plans_vs_oracle(ctx, seed):
 @pytest.mark.parametrize("seed", range(16))
 def test_block_prefilter_with_bloom_filters_on_every_join(ctx, seed):
     """Every join gets a bloom filter, so pipelines over pages with eight
-    tuple slots test their early filters on four blocks at a time before the
-    per-block work. Sparse key domains leave most blocks without a match.
-    Rows equal the oracle's and those with the block pre-filter off."""
+    tuple slots test their early filters on eight blocks at a time before the
+    per-block work. Sparse key domains leave most blocks without a match;
+    odd seeds have no NULLs, so their fixed-width columns are read as dense
+    columns (no block index). Rows equal the oracle's and those with the block
+    pre-filter off."""
     rng = random.Random(7000 + seed)
     ntables = rng.randint(3, 6)
     max_rows = rng.choice([3000, 20000, 60000])
     key_type = rng.choice([INT32, INT64])
     domain = rng.choice([max_rows, 4 * max_rows, 40 * max_rows])
-    plan = random_plan(rng, ntables=ntables, max_rows=max_rows, key_type=key_type, domain=domain)
+    plan = random_plan(rng, ntables=ntables, max_rows=max_rows, key_type=key_type, domain=domain,
+                       null_frac=0.0 if seed % 2 else 0.15)
     types = [t for _, t in plan.nodes[plan.root].output_attrs]
     say(f"block pre-filter seed={seed} tables={ntables} rows<={max_rows} domain={domain}")
     ctx.set_option("bloom_bits", 16)
