@@ -321,6 +321,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "force_grouped") options.force_grouped = v != 0;
     else if (name == "scan_build") options.scan_build = v != 0;
     else if (name == "block_prefilter") options.block_prefilter = v != 0;
+    else if (name == "scan_insert_max_bytes") options.scan_insert_max_bytes = v > 0 ? uint64_t(v) : 0;
     else if (name == "scan_workgroups") options.scan_workgroups = v >= 16 && v <= 4096 ? uint32_t(v) : 512u;
     else return false;
     return true;
@@ -461,6 +462,13 @@ private:
     // its pages (no staged tuples). False when that path does not apply or the
     // table needs more than the plain partitioned build (the caller stages).
     bool scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64_t probe_est, const HashTable*& out);
+    // Small hash table on a plain scan inserted straight from its pages (one
+    // kernel, one round trip). False when the table is too large for it or a
+    // key repeats past the walk cap (`walked`: the caller builds grouped).
+    bool scan_insert(uint32_t rel, const std::vector<uint32_t>& eqs, uint64_t probe_est, const HashTable*& out,
+                     bool& walked);
+    // Tuple words of relation `rel`'s columns for classes `eqs`, NULL-checked join columns.
+    ScanBuildArgs scan_args(uint32_t rel, const std::vector<uint32_t>& eqs);
     void pipeline(Driver d, std::vector<ProbeStep> probes, Target& t);
     uint64_t launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& ms);
     uint64_t grid_waves(uint64_t nrows, bool xcd_order = false, uint32_t per_cu = 16) const;
@@ -785,10 +793,17 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
         }
     const uint64_t key_uid = has_key ? inputs_[q_.rels[key_rel].table]->uid : 0;
     const uint64_t key_col = has_key ? q_.attrs[q_.attr_of(key_rel, key_eq)].col : 0;
-    const bool     known   = has_key && ctx_.build_hints().grouped(key_uid, key_col);
+    bool           known   = has_key && ctx_.build_hints().grouped(key_uid, key_col);
     if (plain && ctx_.options.scan_build && !known) {
-        const HashTable* ht = nullptr;
-        if (scan_build(uint32_t(leaf), eqs, probe_rows, ht)) return ht; // nullptr: empty build side
+        const HashTable* ht     = nullptr;
+        bool             walked = false;
+        if (scan_insert(uint32_t(leaf), eqs, probe_rows, ht, walked)) return ht; // nullptr: empty build side
+        if (walked && has_key) { // a key repeats past the walk cap: build grouped at once
+            ctx_.build_hints().mark_grouped(key_uid, key_col);
+            known = true;
+        } else if (scan_build(uint32_t(leaf), eqs, probe_rows, ht)) {
+            return ht;
+        }
     }
     Driver d;
     d.rel = leaf;
@@ -799,6 +814,79 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
     if (has_key && !known && ht->grouped && !ctx_.options.force_grouped)
         ctx_.build_hints().mark_grouped(key_uid, key_col);
     return ht;
+}
+
+ScanBuildArgs Executor::scan_args(uint32_t rel, const std::vector<uint32_t>& eqs) {
+    ScanBuildArgs sa{};
+    sa.words = uint32_t(eqs.size());
+    for (uint32_t w = 0; w < sa.words; ++w) {
+        const uint32_t a = q_.attr_of(rel, eqs[w]);
+        const uint64_t c = q_.attrs[a].col;
+        sa.col[w]        = column(rel, c).view(col_id(rel, c));
+        if (q_.is_join_eq(eqs[w])) sa.notnull |= 1u << w;
+    }
+    sa.notnull |= 1u; // the key
+    sa.nrows = q_.rels[rel].rows;
+    return sa;
+}
+
+bool Executor::scan_insert(uint32_t rel, const std::vector<uint32_t>& eqs, uint64_t probe_est, const HashTable*& out,
+                           bool& walked) {
+    out               = nullptr;
+    walked            = false;
+    const uint64_t n0 = q_.rels[rel].rows; // upper bound on the tuples (NULL keys are dropped)
+    if (n0 == 0 || eqs.size() > size_t(kMaxWords) || ctx_.options.force_grouped) return false;
+    const uint32_t words = uint32_t(eqs.size());
+    JoinTuning     tn    = ctx_.tune(n0, probe_est, 8);
+    uint32_t       lg    = log2_ceil(uint64_t(std::ceil(double(n0) / tn.max_load)));
+    if (lg < 6) lg = 6;
+    if ((uint64_t(1) << lg) * words * sizeof(uint64_t) > ctx_.options.scan_insert_max_bytes) return false;
+    ScanBuildArgs sa = scan_args(rel, eqs);
+    auto          ht = std::make_unique<HashTable>();
+    ht->eqs          = eqs;
+    ht->words        = words;
+    ht->cap          = 1ull << lg;
+    ht->shift        = 64 - lg;
+    ht->rmask        = ht->cap - 1;
+    ht->slots        = DevBuf(&ctx_.heap(), ht->cap * words * sizeof(uint64_t));
+    DevBuf fc(&ctx_.heap(), 4 * sizeof(uint32_t) + sizeof(uint64_t)); // flags, then the count
+    if (tn.bloom_bits) {
+        uint64_t bw   = std::max<uint64_t>(1, n0 * tn.bloom_bits / 64);
+        ht->bloom_log = log2_ceil(bw);
+        ht->bloom     = DevBuf(&ctx_.heap(), (1ull << ht->bloom_log) * sizeof(uint64_t));
+        ht->has_bloom = true;
+    }
+    uint64_t*   bloom = ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr;
+    hipStream_t s     = ctx_.stream();
+    CGJ_HIP(hipEventRecord(ev0_, s));
+    CGJ_HIP(hipMemsetAsync(fc.get(), 0, 4 * sizeof(uint32_t) + sizeof(uint64_t), s));
+    CGJ_HIP(launch_fill_u64(ht->slots.get<uint64_t>(), ht->cap * words, kEmpty, s));
+    if (bloom) CGJ_HIP(launch_fill_u64(bloom, 1ull << ht->bloom_log, 0, s));
+    auto* count = reinterpret_cast<unsigned long long*>(fc.get<uint32_t>() + 4);
+    CGJ_HIP(launch_scan_insert(sa, ht->slots.get<uint64_t>(), lg, bloom, ht->bloom_log, fc.get<uint32_t>(),
+                               ctx_.options.group_walk, count, s));
+    uint32_t back[6];
+    CGJ_HIP(hipMemcpyAsync(back, fc.get(), sizeof(back), hipMemcpyDeviceToHost, s));
+    CGJ_HIP(hipEventRecord(ev1_, s));
+    CGJ_HIP(hipEventSynchronize(ev1_));
+    float fms = 0;
+    CGJ_HIP(hipEventElapsedTime(&fms, ev0_, ev1_));
+    ms_build += fms;
+    const uint64_t n = uint64_t(back[4]) | (uint64_t(back[5]) << 32);
+    if (back[2]) { // an insert passed the walk cap: the table is incomplete
+        ++retries;
+        ++retries_build;
+        walked = true;
+        return false;
+    }
+    if (n == 0) return true; // empty build side
+    ht->tuples = n;
+    ht->unique = back[0] == 0;
+    build_tuples += n;
+    build_bytes += n * words * sizeof(uint64_t);
+    tables_.push_back(std::move(ht));
+    out = tables_.back().get();
+    return true;
 }
 
 bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64_t probe_est, const HashTable*& out) {
@@ -815,16 +903,7 @@ bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64
     if (!tn.partitioned || lg <= rs_log || (uint64_t(1) << rs_log) * words * 8 > ctx_.lds_bytes() ||
         ctx_.options.force_grouped)
         return false;
-    ScanBuildArgs sa{};
-    sa.words = words;
-    for (uint32_t w = 0; w < words; ++w) {
-        const uint32_t a = q_.attr_of(rel, eqs[w]);
-        const uint64_t c = q_.attrs[a].col;
-        sa.col[w]        = column(rel, c).view(col_id(rel, c));
-        if (q_.is_join_eq(eqs[w])) sa.notnull |= 1u << w;
-    }
-    sa.notnull |= 1u; // the key
-    sa.nrows                    = n0;
+    ScanBuildArgs sa = scan_args(rel, eqs);
     const uint32_t log_regions  = lg - rs_log;
     sa.nregions                 = 1u << log_regions;
     sa.rshift                   = 64 - log_regions;

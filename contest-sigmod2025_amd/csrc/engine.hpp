@@ -187,7 +187,9 @@ struct Options {
     // Hash tables on a plain scan are partitioned straight from the pages
     // (no staged tuples).
     bool     scan_build            = true;
-    bool     block_prefilter       = true; // early bloom filters tested four 64-row blocks at a time first
+    bool     block_prefilter       = true; // early bloom filters tested eight 64-row blocks at a time first
+    uint64_t scan_insert_max_bytes = uint64_t(8) << 20; // hash tables on a plain scan up to this size are
+                                                        // inserted straight from the pages (0: never)
     uint32_t scan_workgroups       = 512; // workgroups of the scan-driven partition (16..4096)
 };
 

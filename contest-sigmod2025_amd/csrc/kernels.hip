@@ -520,6 +520,71 @@ __global__ __launch_bounds__(1024) void k_scan_scatter(ScanBuildArgs a, const ui
     }
 }
 
+// Hash table on a plain scan inserted straight from its pages, for tables
+// small enough that random inserts stay in the caches (larger ones take the
+// partitioned build): each lane decodes its row's tuple words (a NULL join
+// column drops the row) and inserts them as k_ht_insert does, with the same
+// flags; `count` receives the rows inserted. One 64-row block per wave step.
+__global__ __launch_bounds__(256) void k_scan_insert(ScanBuildArgs a, uint64_t* ht, uint32_t shift, uint64_t mask,
+                                                     uint64_t* bloom, uint32_t bloom_log, uint32_t* flags,
+                                                     uint32_t walk_cap, unsigned long long* count) {
+    const uint64_t nb     = (a.nrows + 63) / 64;
+    const uint32_t wpb    = blockDim.x / 64;
+    const uint64_t nwaves = uint64_t(gridDim.x) * wpb;
+    const uint32_t words  = a.words;
+    uint32_t       mine   = 0;
+    for (uint64_t b = uint64_t(blockIdx.x) * wpb + wave_in_block(); b < nb; b += nwaves) {
+        uint64_t   v[kMaxWords];
+        const bool valid = scan_build_row(a, b, b * 64 + lane_id() < a.nrows, v);
+        if (!valid) continue;
+        const uint64_t key = v[0];
+        const uint64_t h   = hash_key(key);
+        uint64_t       pos = h >> shift;
+        bool           dup = false, stop = false;
+        for (uint32_t same = 0;;) {
+            const unsigned long long prev =
+                atomicCAS(reinterpret_cast<unsigned long long*>(ht + pos * words), kEmpty, key);
+            if (prev == kEmpty) break;
+            if (prev == key) {
+                dup = true;
+                if (walk_cap && ++same >= walk_cap) {
+                    stop = true;
+                    break;
+                }
+            }
+            pos = (pos + 1) & mask;
+        }
+        if (dup) flags[0] = 1;
+        if (stop) {
+            flags[2] = 1;
+            continue;
+        }
+#pragma unroll
+        for (int w = 1; w < kMaxWords; ++w)
+            if (uint32_t(w) < words) ht[pos * words + w] = v[w];
+        if (bloom) {
+            const uint64_t h2 = hash_bloom(h);
+            atomicOr(reinterpret_cast<unsigned long long*>(bloom + bloom_word(h2, bloom_log)), bloom_bits(h2));
+        }
+        ++mine;
+    }
+    const uint32_t total = wave_sum(mine);
+    if (lane_id() == 0 && total) atomicAdd(count, (unsigned long long)total);
+}
+
+hipError_t launch_scan_insert(const ScanBuildArgs& a, uint64_t* ht, uint32_t log_cap, uint64_t* bloom,
+                              uint32_t bloom_log, uint32_t* flags, uint32_t walk_cap, unsigned long long* count,
+                              hipStream_t s) {
+    if (a.nrows == 0) return hipSuccess;
+    if (a.words == 0 || a.words > uint32_t(kMaxWords) || log_cap < 1 || log_cap > 40) return hipErrorInvalidValue;
+    const uint64_t nb     = (a.nrows + 63) / 64;
+    uint64_t       blocks = (nb + 3) / 4;
+    if (blocks > 4096) blocks = 4096;
+    k_scan_insert<<<dim3(blocks), dim3(256), 0, s>>>(a, ht, 64 - log_cap, (uint64_t(1) << log_cap) - 1, bloom,
+                                                     bloom_log, flags, walk_cap, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_scan_partition(const ScanBuildArgs& a, uint32_t blocks, uint32_t* bh, uint32_t* scan_scratch,
                                  uint64_t* total, uint64_t* parts, uint32_t threads, hipStream_t s) {
     if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;

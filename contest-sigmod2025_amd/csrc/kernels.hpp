@@ -269,6 +269,12 @@ struct ScanBuildArgs {
 };
 hipError_t launch_scan_partition(const ScanBuildArgs& a, uint32_t blocks, uint32_t* bh, uint32_t* scan_scratch,
                                  uint64_t* total, uint64_t* parts, uint32_t threads, hipStream_t s);
+// Small hash table (2^log_cap slots of a.words words, filled with kEmpty by
+// the caller) inserted straight from the scan's pages: same layout and flags
+// as launch_ht_insert; *count += rows inserted (zeroed by the caller).
+hipError_t launch_scan_insert(const ScanBuildArgs& a, uint64_t* ht, uint32_t log_cap, uint64_t* bloom,
+                              uint32_t bloom_log, uint32_t* flags, uint32_t walk_cap, unsigned long long* count,
+                              hipStream_t s);
 // The LDS region build alone, over an already partitioned `parts` holding
 // *n_dev tuples (read on the device: no host round trip in between).
 hipError_t launch_region_build(const uint64_t* parts, const uint32_t* bh, uint32_t blocks, const uint64_t* n_dev,

@@ -147,8 +147,10 @@ typedef struct {
  *   "scan_build": hash tables on a plain scan are partitioned straight from its pages (1,
  *       default) or from staged tuples (0);
  *   "scan_workgroups": workgroups of that partition (16..4096, default 512);
- *   "block_prefilter": a pipeline over pages tests its early bloom filters on four 64-row
- *       blocks at a time before the full per-block work (1, default) or not (0). */
+ *   "block_prefilter": a pipeline over pages tests its early bloom filters on eight 64-row
+ *       blocks at a time before the full per-block work (1, default) or not (0);
+ *   "scan_insert_max_bytes": a hash table on a plain scan of at most this many bytes is
+ *       inserted straight from the pages by one kernel (default 8 MiB, 0 = never). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
