@@ -73,8 +73,9 @@ def test_execute_many_reports_errors(ctx):
     assert len(ok) == 2
 
 
+@pytest.mark.parametrize("workers", [4, 12])
 @pytest.mark.parametrize("mode", [gj.EST_INDEPENDENCE, gj.EST_LEARNED])
-def test_execute_many_planned_here_matches_single_runs(ctx, mode):
+def test_execute_many_planned_here_matches_single_runs(ctx, mode, workers):
     """Under a model other than the max rule, each worker plans its queries
     with the context's estimator (shared statistics catalog, inference on the
     worker's stream) and executes them: the rows equal those of one
@@ -91,8 +92,8 @@ def test_execute_many_planned_here_matches_single_runs(ctx, mode):
     mlp = ctx.mlp(ctx.num_features(), seed=4) if mode == gj.EST_LEARNED else None
     ctx.set_estimator(mode, mlp, max_sigma=100.0)
     try:
-        say(f"execute_many planned by the workers, estimator {mode}")
-        res = ctx.execute_many(plans, tables, workers=4)
+        say(f"execute_many planned by the workers, estimator {mode}, {workers} workers")
+        res = ctx.execute_many(plans, tables, workers=workers)
     finally:
         ctx.set_estimator(gj.EST_MAX)
     for p, r, e in zip(plans, res, expect):
