@@ -114,6 +114,7 @@ Context::~Context() {
         if (pin_ev_[b]) (void)hipEventDestroy(pin_ev_[b]);
         if (pin_[b]) (void)hipHostFree(pin_[b]);
     }
+    if (spin_) (void)hipHostFree(spin_);
     workers_.clear();
     est_.reset(); // its network's buffers go back to the heap
     heap_->trim();
@@ -156,6 +157,19 @@ uint8_t* Context::staging(int b) {
 }
 
 void Context::staging_done(int b) { CGJ_HIP(hipEventRecord(pin_ev_[b], stream_)); }
+
+uint8_t* Context::small_pinned(size_t bytes) {
+    if (bytes > spin_bytes_) {
+        CGJ_HIP(hipStreamSynchronize(stream_)); // no copy through the old buffer is in flight
+        if (spin_) CGJ_HIP(hipHostFree(spin_));
+        spin_       = nullptr;
+        spin_bytes_ = 0;
+        const size_t want = std::max<size_t>(bytes, size_t(256) << 10);
+        CGJ_HIP(hipHostMalloc(&spin_, want, hipHostMallocDefault));
+        spin_bytes_ = want;
+    }
+    return static_cast<uint8_t*>(spin_);
+}
 
 namespace {
 // Runs f(i) for i in [0, n) on up to `max_threads` host threads (one below

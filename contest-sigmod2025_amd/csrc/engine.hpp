@@ -254,6 +254,10 @@ public:
     // lifetime; staging(b) waits until the last copy through buffer b is done.
     static constexpr size_t kStagingBytes = size_t(64) << 20;
     uint8_t*                staging(int b);
+    // Pinned host buffer of at least `bytes` for small transfers (the
+    // estimator's features and outputs), grown on demand and kept for the
+    // context's lifetime.
+    uint8_t* small_pinned(size_t bytes);
     void                    staging_done(int b); // records the copy just issued through buffer b
     double   copy_probe(uint64_t bytes, uint32_t reps);
 
@@ -305,6 +309,8 @@ private:
     std::shared_ptr<BuildHints> hints_ = std::make_shared<BuildHints>();
     void*                       pin_[2]    = {nullptr, nullptr};
     hipEvent_t                  pin_ev_[2] = {nullptr, nullptr};
+    void*                       spin_       = nullptr;
+    size_t                      spin_bytes_ = 0;
     std::vector<std::unique_ptr<Context>> workers_;
 };
 

@@ -177,14 +177,23 @@ void Mlp::predict(const float* x, uint32_t n, float* out) {
 void Mlp::predict_on(Context& on, const float* x, uint32_t n, float* out) const {
     if (n == 0) return;
     if (dirty_) throw std::runtime_error("network weights changed since their BF16 conversion");
-    hipStream_t s = on.stream();
-    DevBuf X(&on.heap(), uint64_t(n) * in_ * sizeof(float)), Y(&on.heap(), uint64_t(n) * kMlpOut * sizeof(float));
-    CGJ_HIP(hipMemcpyAsync(X.get(), x, uint64_t(n) * in_ * sizeof(float), hipMemcpyHostToDevice, s));
+    // One device block for features and outputs, and the copies through the
+    // context's pinned buffer (direct transfers, no staging of pageable memory).
+    hipStream_t    s   = on.stream();
+    const size_t   xb  = size_t(n) * in_ * sizeof(float);
+    const size_t   yb  = size_t(n) * kMlpOut * sizeof(float);
+    const size_t   yo  = (xb + 255) / 256 * 256;
+    DevBuf         dev(&on.heap(), yo + yb);
+    uint8_t*       pin = on.small_pinned(yo + yb);
+    std::memcpy(pin, x, xb);
+    CGJ_HIP(hipMemcpyAsync(dev.get(), pin, xb, hipMemcpyHostToDevice, s));
     const float* P = params_.get<float>();
-    CGJ_HIP(launch_mlp_infer(X.get<float>(), n, in_, bf1_.get<uint16_t>(), P + off_b1(), bf2_.get<uint16_t>(),
-                             P + off_b2(), P + off_w3(), P + off_b3(), Y.get<float>(), s));
-    CGJ_HIP(hipMemcpyAsync(out, Y.get(), uint64_t(n) * kMlpOut * sizeof(float), hipMemcpyDeviceToHost, s));
+    float*       Y = reinterpret_cast<float*>(dev.get<uint8_t>() + yo);
+    CGJ_HIP(launch_mlp_infer(dev.get<float>(), n, in_, bf1_.get<uint16_t>(), P + off_b1(), bf2_.get<uint16_t>(),
+                             P + off_b2(), P + off_w3(), P + off_b3(), Y, s));
+    CGJ_HIP(hipMemcpyAsync(pin + yo, Y, yb, hipMemcpyDeviceToHost, s));
     CGJ_HIP(hipStreamSynchronize(s));
+    std::memcpy(out, pin + yo, yb);
 }
 
 void Mlp::predict_f32(const float* x, uint32_t n, float* out) {
