@@ -301,15 +301,14 @@ void Batch::launch_once() {
 }
 
 void Batch::run() {
-    // The batched kernels did not finish on their first GPU runs and the cause
-    // is not found yet; until it is, running a batch is an error rather than
-    // a launch that may not return.
-    throw std::runtime_error("batched execution is disabled: its kernels did not finish on the GPU "
-                             "(cause not found); run the plans with cgj_execute_many instead");
     for (int attempt = 0;; ++attempt) {
         launch_once();
         bool grow = false;
         for (size_t j = 0; j < jobs_.size(); ++j) {
+            if (sync_[j].dup & 2u)
+                throw std::runtime_error("batch: job " + std::to_string(j) + ": hash-table insert found no empty slot");
+            if (counters_[j].fault)
+                throw std::runtime_error("batch: job " + std::to_string(j) + ": hash-table probe found no empty slot");
             if (counters_[j].overflow) {
                 // the pool was sized for one row per probe row; duplicates made more
                 Job& J = jobs_[j];

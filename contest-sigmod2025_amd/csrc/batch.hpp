@@ -3,10 +3,12 @@
 //
 // Each plan of the batch is one join of two scans (the shape of the contest's
 // single-join queries, reference engine/query/QueryPlan.cpp:414-678 run once
-// per query). Its build side is inserted and its probe side streamed through
-// the one-probe pipeline by tasks of one queue; the queue lists every build
-// task before any probe task, and a probe task waits (bounded) for its own
-// job's build. Per-job tunings come from BatchTuner.
+// per query). One graph holds: fill kernels that clear the hash tables and
+// counters, one persistent launch that takes every job's build tasks from a
+// device queue, then one persistent launch per output width that takes the
+// probe tasks (the one-probe pipeline over a row range) from their queue. The
+// kernel boundary orders every probe after every build; no wave waits on
+// another. Per-job tunings come from BatchTuner.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -81,6 +81,38 @@ __device__ inline uint32_t wave_sum(uint32_t v) {
 }
 __device__ inline uint32_t highest_lane(uint64_t m) { return 63 - __clzll(m); }
 
+__device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint64_t uniform_u64(uint64_t v) {
+    return uint64_t(uniform_u32(uint32_t(v))) | (uint64_t(uniform_u32(uint32_t(v >> 32))) << 32);
+}
+
+// Lane 0 test for one-lane side effects (header stores, counter updates).
+// The empty asm makes every call an independent value: two such tests in a
+// loop cannot be correlated by the compiler. Correlated `lane == 0` branches
+// at the end of one iteration and the start of the next let jump threading
+// route lanes 1..63 around the loop head on their own, after which a
+// readfirstlane in the loop reads lane 1 instead of lane 0 (a task loop built
+// that way re-ran its first task on lanes 1..63 without end).
+__device__ inline bool lead_lane() {
+    uint32_t l = lane_id();
+    asm volatile("" : "+v"(l));
+    return l == 0;
+}
+
+// Adds `n` to *p once for the whole wave and returns the previous value to
+// every lane, without a lane-guarded branch: every active lane takes part in
+// the atomic, the first active one with `n`, the others with 0, and the first
+// active lane's result is broadcast. Work queues and reservations use this.
+__device__ inline unsigned long long wave_fetch_add(unsigned long long* p, unsigned long long n) {
+    const uint32_t           lead = uniform_u32(lane_id());
+    const unsigned long long old  = atomicAdd(p, lane_id() == lead ? n : 0ull);
+    return uniform_u64(old);
+}
+__device__ inline uint32_t wave_fetch_or(uint32_t* p, uint32_t v) {
+    const uint32_t lead = uniform_u32(lane_id());
+    return uniform_u32(atomicOr(p, lane_id() == lead ? v : 0u));
+}
+
 // Load of data that no kernel writes while the reading kernel runs (column
 // indexes, hash tables built by an earlier launch). Through the constant
 // address space a wave-uniform address becomes a scalar load; otherwise the

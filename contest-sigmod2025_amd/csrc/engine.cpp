@@ -1117,6 +1117,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             launch(a, nslots, nrows, *ms);
             PipeCounters c;
             CGJ_HIP(hipMemcpy(&c, counters_.get(), sizeof(c), hipMemcpyDeviceToHost));
+            if (c.fault) throw std::runtime_error("hash-table probe met no empty slot in its region");
             if (!c.overflow) {
                 t.count = c.tuples;
                 t.slots = std::min<uint64_t>(c.pages, cap);
@@ -1132,6 +1133,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         launch(a, nslots, nrows, *ms);
         PipeCounters c;
         CGJ_HIP(hipMemcpy(&c, counters_.get(), sizeof(c), hipMemcpyDeviceToHost));
+        if (c.fault) throw std::runtime_error("hash-table probe met no empty slot in its region");
         t.count = c.tuples;
         return;
     }
@@ -1172,6 +1174,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         launch(a, nslots, nrows, *ms);
         PipeCounters c;
         CGJ_HIP(hipMemcpy(&c, counters_.get(), sizeof(c), hipMemcpyDeviceToHost));
+        if (c.fault) throw std::runtime_error("hash-table probe met no empty slot in its region");
         if (!c.overflow) {
             res->nrows          = c.tuples;
             res->total_pages    = 0;
@@ -1451,6 +1454,14 @@ void Context::download_pool(const PoolView& r, cgj_result& out) {
         if (a0[1] != b0[1]) return a0[1] < b0[1];
         return a0[2] < b0[2];
     });
+    // Every (column, stream, sequence) key names one page: a repeated key
+    // would make the order of that column's pages ambiguous.
+    for (size_t k = 1; k < order.size(); ++k) {
+        const uint32_t* a0 = &dir[4 * order[k - 1]];
+        const uint32_t* b0 = &dir[4 * order[k]];
+        if (a0[0] == b0[0] && a0[1] == b0[1] && a0[2] == b0[2])
+            throw std::runtime_error("page directory repeats a (column, stream, sequence) key");
+    }
     std::vector<std::vector<uint32_t>> pages(r.types.size());
     for (uint64_t i : order) {
         if (dir[4 * i] >= pages.size()) throw std::runtime_error("corrupt page directory");

@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256) void k_scan_insert(ScanBuildArgs a, uint64_t* 
         ++mine;
     }
     const uint32_t total = wave_sum(mine);
-    if (lane_id() == 0 && total) atomicAdd(count, (unsigned long long)total);
+    if (lead_lane() && total) atomicAdd(count, (unsigned long long)total);
 }
 
 hipError_t launch_scan_insert(const ScanBuildArgs& a, uint64_t* ht, uint32_t log_cap, uint64_t* bloom,
@@ -889,16 +889,14 @@ __device__ inline int32_t fixed_budget(int32_t type) { return 8 * (int32_t(kPage
 __device__ inline void page_open(const PipeArgs& a, WaveState& ws, OutState& st, uint64_t* bits, int32_t budget) {
     uint64_t next = ws.next, end = ws.end;
     if (next == end) {
-        unsigned long long base = 0;
-        if (lane_id() == 0) base = atomicAdd(&a.counters->pages, (unsigned long long)a.page_chunk);
-        base   = __shfl(base, 0, 64);
+        const unsigned long long base = wave_fetch_add(&a.counters->pages, (unsigned long long)a.page_chunk);
         next   = base;
         ws.end = base + a.page_chunk;
     }
     uint64_t idx = next;
     ws.next      = next + 1;
     st.bad = idx >= a.pool_pages ? 1u : 0u;
-    if (st.bad && lane_id() == 0) atomicExch(&a.counters->overflow, 1ull);
+    if (st.bad && lead_lane()) atomicExch(&a.counters->overflow, 1ull);
     st.page  = st.bad ? 0u : uint32_t(idx);
     st.nr    = 0;
     st.nv    = 0;
@@ -929,7 +927,7 @@ __device__ inline void page_close(const PipeArgs& a, OutState& st, uint64_t* bit
 #pragma unroll
         for (uint32_t b = 0; b < 8; ++b)
             if (b0 + b < nb) bm[b0 + b] = uint8_t(word >> (8 * b));
-        if (lane_id() == 0) {
+        if (lead_lane()) {
             *reinterpret_cast<uint32_t*>(page) = st.nr | (st.nv << 16);
             dir_put(a, st.page, col, stream, st.seq);
         }
@@ -1066,7 +1064,7 @@ __device__ void append_varchar(const PipeArgs& a, WaveState& ws, OutState& st, u
                     const uint64_t* s64 = reinterpret_cast<const uint64_t*>(base + p * kPageSize);
                     uint64_t*       d64 = reinterpret_cast<uint64_t*>(a.pool + uint64_t(st.page) * kPageSize);
                     for (uint32_t i = lane_id(); i < kPageSize / 8; i += 64) d64[i] = s64[i];
-                    if (lane_id() == 0) dir_put(a, st.page, col, stream, st.seq);
+                    if (lead_lane()) dir_put(a, st.page, col, stream, st.seq);
                     st.npages++;
                 }
                 st.seq++;
@@ -1087,10 +1085,8 @@ __device__ inline void stage_put(const PipeArgs& a, WaveState& ws, bool have, ui
     uint64_t rem  = end - next;
     uint64_t base2 = 0;
     if (cnt > rem) {
-        unsigned long long b = 0;
-        if (lane_id() == 0) b = atomicAdd(&a.counters->pages, (unsigned long long)a.stage_chunk);
-        base2 = __shfl(b, 0, 64);
-        if (base2 + a.stage_chunk > a.stage_cap && lane_id() == 0) atomicExch(&a.counters->overflow, 1ull);
+        base2 = wave_fetch_add(&a.counters->pages, (unsigned long long)a.stage_chunk);
+        if (base2 + a.stage_chunk > a.stage_cap && lead_lane()) atomicExch(&a.counters->overflow, 1ull);
     }
     if (have) {
         uint64_t pos = rank < rem ? next + rank : base2 + (rank - rem);
@@ -1115,9 +1111,9 @@ __device__ inline void finish_column(const PipeArgs& a, OutState& st, uint64_t* 
                                      const uint8_t* stg) {
     if (st.page != kNoPage && st.nr > 0)
         page_close(a, st, bits, c, stream, stg);
-    else if (st.page != kNoPage && !st.bad && lane_id() == 0)
+    else if (st.page != kNoPage && !st.bad && lead_lane())
         a.dir[4 * uint64_t(st.page) + 3] = 0; // opened, never filled
-    if (lane_id() == 0) {
+    if (lead_lane()) {
         if (st.nnz) atomicAdd(&a.counters->nnz[c], (unsigned long long)st.nnz);
         if (st.npages) atomicAdd(&a.counters->col_pages[c], (unsigned long long)st.npages);
     }
@@ -1127,7 +1123,7 @@ __device__ inline void finish_column(const PipeArgs& a, OutState& st, uint64_t* 
 // reservation (staging slots become holes, pool pages invalid directory entries).
 __device__ inline void finish_wave(const PipeArgs& a, const WaveState& ws, uint64_t local) {
     uint32_t lane = lane_id();
-    if (lane == 0 && local) atomicAdd(&a.counters->tuples, (unsigned long long)local);
+    if (lead_lane() && local) atomicAdd(&a.counters->tuples, (unsigned long long)local);
     if (a.target == TARGET_STAGE) {
         for (uint64_t i = ws.next + lane; i < ws.end && i < a.stage_cap; i += 64) a.stage[i * a.stage_words] = kEmpty;
     } else if (a.target == TARGET_PAGES) {
@@ -1540,19 +1536,13 @@ __device__ inline const T& args_at(uintptr_t addr) {
 }
 __device__ inline uintptr_t kernarg_addr() { return (uintptr_t)__builtin_amdgcn_kernarg_segment_ptr(); }
 
-__device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ inline uint64_t uniform_u64(uint64_t v) {
-    return uint64_t(uniform_u32(uint32_t(v))) | (uint64_t(uniform_u32(uint32_t(v >> 32))) << 32);
-}
 __device__ inline uint64_t low_mask(uint32_t n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
 
 __device__ inline void lean_open(const LeanArgs& a, WaveState& ws, LeanPage& st, uint64_t& bits, int32_t budget) {
     uint64_t next = ws.next;
     if (next == ws.end) {
-        const uint32_t     chunk = a.page_chunk;
-        unsigned long long base  = 0;
-        if (lane_id() == 0) base = atomicAdd(&a.counters->pages, (unsigned long long)chunk);
-        next   = uniform_u64(base);
+        const uint32_t chunk = a.page_chunk;
+        next   = wave_fetch_add(&a.counters->pages, (unsigned long long)chunk);
         ws.end = next + chunk;
     }
     ws.next  = next + 1;
@@ -1561,7 +1551,7 @@ __device__ inline void lean_open(const LeanArgs& a, WaveState& ws, LeanPage& st,
     // of a divergent join
     st.bad   = uniform_u32(next >= a.pool_pages ? 1u : 0u);
     st.page  = uniform_u32(st.bad ? 0u : uint32_t(next));
-    if (st.bad && lane_id() == 0) atomicExch(&a.counters->overflow, 1ull);
+    if (st.bad && lead_lane()) atomicExch(&a.counters->overflow, 1ull);
     st.nr    = 0;
     st.nv    = 0;
     st.avail = budget;
@@ -1578,7 +1568,7 @@ __device__ inline void lean_close(const LeanArgs& a, LeanPage& st, uint64_t bits
 #pragma unroll
         for (uint32_t b = 0; b < 8; ++b)
             if (b0 + b < nb) bm[b0 + b] = uint8_t(bits >> (8 * b));
-        if (lane_id() == 0) {
+        if (lead_lane()) {
             *reinterpret_cast<uint32_t*>(page) = st.nr | (st.nv << 16);
             uint4 e = make_uint4(col, stream, st.seq, 1u);
             *reinterpret_cast<uint4*>(a.dir + 4 * uint64_t(st.page)) = e;
@@ -1646,9 +1636,9 @@ __device__ inline void lean_append(const LeanArgs& a, WaveState& ws, LeanPage& s
 __device__ inline void lean_finish(const LeanArgs& a, LeanPage& st, uint64_t bits, uint32_t c, uint32_t stream) {
     if (st.page != kNoPage && st.nr > 0)
         lean_close(a, st, bits, c, stream);
-    else if (st.page != kNoPage && !st.bad && lane_id() == 0)
+    else if (st.page != kNoPage && !st.bad && lead_lane())
         a.dir[4 * uint64_t(st.page) + 3] = 0; // opened, never filled
-    if (lane_id() == 0) {
+    if (lead_lane()) {
         if (st.nnz) atomicAdd(&a.counters->nnz[c], (unsigned long long)st.nnz);
         if (st.npages) atomicAdd(&a.counters->col_pages[c], (unsigned long long)st.npages);
     }
@@ -1732,6 +1722,12 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
             const uint64_t  rmask = a.mask;
             uint64_t        q     = h >> a.shift; // next slot to inspect
             bool            live  = alive;
+            // While the region holds an empty slot, no lane takes more than
+            // rmask + 2 steps over all its rounds, so the wave takes at most 64
+            // times that; past it the table is corrupt or full, and the block
+            // stops with the fault flag set instead of walking on.
+            const uint64_t walk_limit = 64 * (rmask + 2);
+            uint64_t       walk       = 0;
             for (;;) {
                 // Each step inspects two consecutive slots of every searching
                 // lane; a lane stops at a match or an empty slot. The wave steps
@@ -1741,6 +1737,11 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                 uint64_t hslot = 0, k1 = 0;
                 const uint64_t  odd   = a.aligned_pairs ? 1 : 0; // uniform
                 while (__ballot(searching)) {
+                    if (++walk > walk_limit) {
+                        if (lead_lane()) atomicExch(&a.counters->fault, 1ull);
+                        live = false;
+                        break;
+                    }
                     if (searching) {
                         // slots qa (ignored when `skip`) and qb, then continue at qn
                         const uint64_t qa   = q & ~odd;
@@ -1812,7 +1813,7 @@ __device__ inline void lean_flush(const LeanArgs& a, LeanWave<NS>& W, uint32_t s
     const uint32_t lane = lane_id();
 #pragma unroll
     for (int c = 0; c < NO; ++c) lean_finish(a, W.st[c], W.bits[c], uint32_t(c), stream);
-    if (lane == 0 && W.local) atomicAdd(&a.counters->tuples, (unsigned long long)W.local);
+    if (lead_lane() && W.local) atomicAdd(&a.counters->tuples, (unsigned long long)W.local);
     if (NO > 0)
         for (uint64_t i = W.ws.next + lane; i < W.ws.end && i < a.pool_pages; i += 64) a.dir[4 * i + 3] = 0;
     lean_wave_init(W);
@@ -1827,7 +1828,6 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
     const uint32_t wib    = wave_in_block();
     const uint32_t stream = blockIdx.x * kWavesPerBlock + wib;
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-    const uint32_t lane   = lane_id();
     LeanWave<NS>   W;
     lean_wave_init(W);
 
@@ -1846,9 +1846,7 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
     for (uint64_t m = m_first; m < m_end;) {
         uint64_t m_next = m + m_step;
         if (!a.xcd_order) {
-            unsigned long long t = 0;
-            if (lane == 0) t = atomicAdd(&a.counters->morsel, 1ull);
-            m_next = nwaves + uniform_u64(t);
+            m_next = nwaves + wave_fetch_add(&a.counters->morsel, 1ull);
         }
         const uint64_t r0  = m * a.morsel_rows;
         const uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
@@ -1873,7 +1871,7 @@ __device__ inline void batch_build_task(uintptr_t job_addr, uint64_t index) {
     const BatchJob& J0   = args_at<BatchJob>(job_addr);
     const uint64_t  r0   = index * J0.build.morsel_rows;
     const uint64_t  end  = r0 + J0.build.morsel_rows < J0.build.rows ? r0 + J0.build.morsel_rows : J0.build.rows;
-    bool            dup  = false;
+    bool            dup  = false, lost = false;
     for (uint64_t rb = r0; rb < end; rb += 64) {
         const BatchBuild& B      = args_at<BatchJob>(job_addr).build;
         const uint64_t    row    = rb + lane;
@@ -1887,37 +1885,49 @@ __device__ inline void batch_build_task(uintptr_t job_addr, uint64_t index) {
             const uint64_t h     = hash_key(key);
             const uint32_t words = B.words;
             uint64_t       pos   = h >> B.shift;
-            for (;;) {
+            bool           full  = false;
+            for (uint64_t step = 0;; ++step) {
+                if (step > B.mask) { // no empty slot left: report instead of walking on
+                    full = true;
+                    break;
+                }
                 const unsigned long long prev =
                     atomicCAS(reinterpret_cast<unsigned long long*>(B.ht + pos * words), kEmpty, key);
                 if (prev == kEmpty) break;
                 dup = dup || prev == key;
                 pos = (pos + 1) & B.mask;
             }
+            lost = lost || full;
+            if (!full) {
 #pragma unroll
-            for (int t = 0; t < kBatchAttrs; ++t)
-                if (uint32_t(t) < B.nattr) B.ht[pos * words + 1 + t] = at[t];
-            if (B.bloom) {
-                const uint64_t h2 = hash_bloom(h);
-                atomicOr(reinterpret_cast<unsigned long long*>(B.bloom + bloom_word(h2, B.bloom_log)), bloom_bits(h2));
+                for (int t = 0; t < kBatchAttrs; ++t)
+                    if (uint32_t(t) < B.nattr) B.ht[pos * words + 1 + t] = at[t];
+                if (B.bloom) {
+                    const uint64_t h2 = hash_bloom(h);
+                    atomicOr(reinterpret_cast<unsigned long long*>(B.bloom + bloom_word(h2, B.bloom_log)),
+                             bloom_bits(h2));
+                }
             }
         }
     }
     BatchSync* sync = args_at<BatchJob>(job_addr).sync;
-    if (__ballot(dup) && lane == 0) atomicOr(&sync->dup, 1u);
+    if (__ballot(dup)) wave_fetch_or(&sync->dup, 1u);
+    if (__ballot(lost)) wave_fetch_or(&sync->dup, 2u);
 }
 
 // Probe task (a later launch than every build task): runs the one-probe
 // pipeline over rows [index * morsel_rows, ...) of the probe table and flushes
-// its pages; a table without duplicate keys stops at the first match.
+// its pages; a table without duplicate keys stops at the first match. The
+// task's pages carry the task index as their stream id: a wave that runs two
+// tasks of one job starts its page sequence at 0 for each, and the job's
+// directory keys (column, task, sequence) stay unique; a download orders the
+// pages by task, i.e. by probe row range.
 template <int NO, int NS>
-__device__ inline void batch_probe_task(uintptr_t job_addr, uint64_t index, uint32_t stream, LeanWave<NS>& W) {
-    const uint32_t  lane = lane_id();
+__device__ inline void batch_probe_task(uintptr_t job_addr, uint64_t index, LeanWave<NS>& W) {
+    const uint32_t  stream = uint32_t(index);
     const BatchJob& J    = args_at<BatchJob>(job_addr);
     BatchSync*      sync = J.sync;
-    uint32_t        dup  = 0;
-    if (lane == 0) dup = __hip_atomic_load(&sync->dup, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    dup                   = uniform_u32(dup);
+    const uint32_t  dup  = uniform_u32(__hip_atomic_load(&sync->dup, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     const uintptr_t pa    = job_addr + offsetof(BatchJob, probe);
     const LeanArgs& a     = args_at<LeanArgs>(pa);
     const uint64_t  r0    = index * a.morsel_rows;
@@ -1930,11 +1940,8 @@ __device__ inline void batch_probe_task(uintptr_t job_addr, uint64_t index, uint
 // time (wall clock) is added to its job's counters.
 __global__ __launch_bounds__(256) void k_batch_build(const BatchJob* jobs, const BatchTask* tasks, uint64_t ntasks,
                                                      unsigned long long* queue) {
-    const uint32_t lane = lane_id();
     for (;;) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(queue, 1ull);
-        t = uniform_u64(t);
+        const unsigned long long t = wave_fetch_add(queue, 1ull);
         if (t >= ntasks) break;
         const uint4     tk  = load_invariant_u4(tasks + t);
         const uintptr_t ja  = uintptr_t(jobs + tk.x);
@@ -1942,32 +1949,28 @@ __global__ __launch_bounds__(256) void k_batch_build(const BatchJob* jobs, const
         const uint64_t  t0  = wall_clock64();
         batch_build_task(ja, idx);
         const uint64_t dt   = wall_clock64() - t0;
-        BatchSync*     sync = args_at<BatchJob>(ja).sync; // outside the lane-0 branch: scalar operand
-        if (lane == 0) atomicAdd(&sync->build_clocks, (unsigned long long)dt);
+        BatchSync*     sync = args_at<BatchJob>(ja).sync;
+        wave_fetch_add(&sync->build_clocks, (unsigned long long)dt);
     }
 }
 
 template <int NO>
 __global__ __launch_bounds__(256) void k_batch_probe(const BatchJob* jobs, const BatchTask* tasks, uint64_t ntasks,
                                                      unsigned long long* queue) {
-    constexpr int  NS     = NO > 0 ? NO : 1;
-    const uint32_t stream = blockIdx.x * kWavesPerBlock + wave_in_block();
-    const uint32_t lane   = lane_id();
-    LeanWave<NS>   W;
+    constexpr int NS = NO > 0 ? NO : 1;
+    LeanWave<NS>  W;
     lean_wave_init(W);
     for (;;) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(queue, 1ull);
-        t = uniform_u64(t);
+        const unsigned long long t = wave_fetch_add(queue, 1ull);
         if (t >= ntasks) break;
         const uint4     tk  = load_invariant_u4(tasks + t);
         const uintptr_t ja  = uintptr_t(jobs + tk.x);
         const uint64_t  idx = uint64_t(tk.z) | (uint64_t(tk.w) << 32);
         const uint64_t  t0  = wall_clock64();
-        batch_probe_task<NO>(ja, idx, stream, W);
+        batch_probe_task<NO>(ja, idx, W);
         const uint64_t dt   = wall_clock64() - t0;
         BatchSync*     sync = args_at<BatchJob>(ja).sync;
-        if (lane == 0) atomicAdd(&sync->probe_clocks, (unsigned long long)dt);
+        wave_fetch_add(&sync->probe_clocks, (unsigned long long)dt);
     }
 }
 
@@ -2159,9 +2162,9 @@ __global__ __launch_bounds__(256) void k_gen_fill(GenSpec g, const uint32_t* row
         uint64_t r     = uint64_t(r0) + j;
         bool     valid = in && !gen_is_null(g.seed, g.null_stream, g.null_frac, r);
         // block prefix of valid flags (4 waves)
-        uint32_t lane = lane_id(), wv = threadIdx.x / 64;
+        uint32_t wv = threadIdx.x / 64;
         uint64_t b    = __ballot(valid);
-        if (lane == 0) {
+        if (lead_lane()) {
             sh[wv] = __popcll(b);
             if (in) bm[j / 64] = b; // j is a multiple of 64 for lane 0
         }

@@ -54,6 +54,7 @@ struct PipeCounters {
     unsigned long long pages;     // pool pages (or staging slots) reserved
     unsigned long long morsel;    // morsels handed out beyond each wave's first (dynamic schedule)
     unsigned long long overflow;  // nonzero: a buffer was too small, re-run
+    unsigned long long fault;     // nonzero: a hash-table walk met no empty slot within its bound
     unsigned long long nnz[kMaxOut];       // non-NULL values written per output column
     unsigned long long col_pages[kMaxOut]; // pages written per output column
 };
@@ -165,7 +166,8 @@ struct BatchBuild {
 
 // Per-job flags and timing, in device memory (zeroed before every run).
 struct BatchSync {
-    unsigned int       dup;          // a duplicate build key was seen
+    unsigned int       dup;          // bit 0: a duplicate build key was seen; bit 1: an insert found no
+                                     // empty slot (the table is incomplete)
     unsigned int       pad[3];
     unsigned long long build_clocks; // wall-clock ticks spent in build tasks
     unsigned long long probe_clocks; // ... in probe tasks

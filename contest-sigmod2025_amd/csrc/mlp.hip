@@ -117,7 +117,7 @@ __global__ void k_nll(const float* out, const float* y, uint32_t M, float* dout,
     // wave sum, then one atomic per wave
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) l += __shfl_xor(l, d, 64);
-    if (lane_id() == 0) atomicAdd(loss_sum, l);
+    if (lead_lane()) atomicAdd(loss_sum, l);
 }
 
 // Adam step (Kingma & Ba, 2015) with L2 weight decay added to the gradient.

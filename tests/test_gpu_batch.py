@@ -13,11 +13,6 @@ from helpers import check_table
 
 pytestmark = pytest.mark.gpu
 
-# The batched build/probe kernels did not finish on their first two GPU runs
-# (test_batch_mixed_shapes_vs_oracle); the library refuses to launch them
-# until the cause is found, and these tests stay skipped meanwhile.
-HANG = pytest.mark.skip(reason="batched kernels hung on the GPU; launch disabled until the cause is found")
-
 
 def say(msg):
     print(msg, flush=True)
@@ -81,7 +76,6 @@ def _run_and_check(ctx, plans, label):
     return batch, tables
 
 
-@HANG
 def test_batch_mixed_shapes_vs_oracle(ctx):
     plans = [_mixed_job(s) for s in range(24)]
     say("batch of 24 mixed joins")
@@ -90,7 +84,6 @@ def test_batch_mixed_shapes_vs_oracle(ctx):
     assert info["jobs"] == 24 and info["tasks"] > 0 and info["waves"] > 0
 
 
-@HANG
 def test_batch_matches_one_launch_per_join(ctx):
     """The c4 recipe at 1% scale: each job's rows as the plain path returns them."""
     from gpujoin import workloads
@@ -116,7 +109,6 @@ def test_batch_matches_one_launch_per_join(ctx):
     assert check_table(batch.download(3), types) == check_table(oracle.execute(host), types)
 
 
-@HANG
 def test_batch_autotune_keeps_results(ctx):
     plans = [_mixed_job(s) for s in range(6, 14)]
     tables = [[ctx.upload(t) for t in p.inputs] for p in plans]
@@ -129,14 +121,6 @@ def test_batch_autotune_keeps_results(ctx):
     for j, plan in enumerate(plans):
         types = [t for _, t in plan.nodes[plan.root].output_attrs]
         assert check_table(batch.download(j), types) == check_table(before[j], types)
-
-
-def test_batch_run_is_refused_while_disabled(ctx):
-    keys = np.arange(100, dtype=np.int32)
-    plan = _two_table_plan([(INT32, keys, None)], [(INT32, keys, None)], 0, 0, [(0, INT32)])
-    batch = ctx.batch([plan], [[ctx.upload(t) for t in plan.inputs]])
-    with pytest.raises(RuntimeError, match="disabled"):
-        batch.run()
 
 
 def test_batch_rejects_other_shapes(ctx):
