@@ -108,11 +108,15 @@ def pcie_inclusive(ctx, name, scale, reps=3):
 
 def run_c4(args):
     """256 independent INT32 joins (build log-uniform [1K, 2M] rows, probe 10x)
-    on device-resident tables, all in one cgj_execute_many call on concurrent
-    worker streams; the same joins as one execute_device call each are timed
-    beside it. (The persistent-kernel batch over a device work queue is
-    disabled: cgj_batch_run refuses to launch it.) A step is one pass over
-    all joins."""
+    on device-resident tables. The batched form runs all of them from one
+    hipGraph: fill kernels, one persistent build launch over a device queue
+    of every job's build tasks, one persistent probe launch over the probe
+    tasks; task sizes, load factor and bloom bits come from the batch
+    autotuner (tuned once before the timed steps). Beside it: the same joins
+    as one execute_device call each (the plain one-join-per-launch path, as
+    the reference runs one Contest::execute per query), and all joins in one
+    cgj_execute_many call on worker streams. A step is one pass over all
+    joins; every form's row counts must agree."""
     import gpujoin as gj
     from gpujoin import workloads
 
@@ -129,55 +133,104 @@ def run_c4(args):
     log(f"c4: {len(jobs)} joins, {tuples} tuples generated in {time.perf_counter() - t0:.1f} s")
     plans = [p for p, _ in jobs]
     tabs_all = [t for _, t in jobs]
+
+    # batched: one graph launch per step
+    t0 = time.perf_counter()
+    batch = ctx.batch(plans, tabs_all)
+    batch.run()
+    t_first = time.perf_counter() - t0
+    t_tune = 0.0
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        batch.autotune()
+        t_tune = time.perf_counter() - t0
+        log(f"c4 batch autotuned in {t_tune:.1f} s: {batch.info()}")
     for _ in range(args.warmup):
-        for r in ctx.execute_many(plans, tabs_all, workers=args.workers):
-            r.free()
-    # the batched form: all joins in one call, on `workers` concurrent streams
-    per_step = []
-    rows = 0
+        batch.run()
+    per_step, dev_ms = [], []
     for step in range(args.steps):
         t_start = time.perf_counter()
-        res = ctx.execute_many(plans, tabs_all, workers=args.workers)
+        batch.run()
         per_step.append(time.perf_counter() - t_start)
-        rows = sum(r.num_rows for r in res)
-        for r in res:
-            r.free()
-        log(f"c4 step {step}: {per_step[-1] * 1000:.1f} ms ({args.workers} workers)")
-    s_step = sum(per_step) / len(per_step)
-    # one execute_device per join, one after another
+        dev_ms.append(batch.info()["ms_last"])
+        log(f"c4 batch step {step}: {per_step[-1] * 1000:.2f} ms host, {dev_ms[-1]:.2f} ms graph")
+    s_batch = sum(per_step) / len(per_step)
+    info = batch.info()
+    rows_batch = [batch.num_rows(j) for j in range(len(jobs))]
+
+    # one execute_device per join, one after another (plain path)
     seq = []
     ms_dev = 0.0
+    rows_plain = []
     for step in range(max(1, args.steps // 2)):
         t_start = time.perf_counter()
+        rows_plain = []
         for plan, tabs in jobs:
             r = ctx.execute_device(plan, tabs)
             ms_dev += ctx.stats()["ms_total"]
+            rows_plain.append(r.num_rows)
             r.free()
         seq.append(time.perf_counter() - t_start)
     s_seq = sum(seq) / len(seq)
     log(f"c4 one join per call: {s_seq * 1000:.1f} ms")
+    if rows_plain != rows_batch:
+        raise SystemExit("c4: batched row counts differ from one call per join")
+
+    # all joins in one cgj_execute_many call on worker streams
+    for r in ctx.execute_many(plans, tabs_all, workers=args.workers):
+        r.free()
+    many = []
+    for step in range(max(1, args.steps // 2)):
+        t_start = time.perf_counter()
+        res = ctx.execute_many(plans, tabs_all, workers=args.workers)
+        many.append(time.perf_counter() - t_start)
+        for r in res:
+            r.free()
+    s_many = sum(many) / len(many)
+    log(f"c4 execute_many ({args.workers} workers): {s_many * 1000:.1f} ms")
+
+    # algorithmic bytes: input pages read once, output pages at the reference's
+    # packing written once, compact build tuples (INT32 key + INT32 payload)
+    # written and read once
+    budget32 = 8 * (PAGE - 4)
+    in_pages = sum(t.num_pages(c) for _, tabs in jobs for t in tabs for c in range(2))
+    out_pages = sum(2 * ((r + 32 * r + budget32 - 1) // budget32) for r in rows_batch)
+    build_rows = sum(w.tables[0].rows for w in bw.jobs)
+    b_alg = (in_pages + out_pages) * PAGE + 2 * 8 * build_rows
+    g_ms = sum(dev_ms) / len(dev_ms)
+    achieved = b_alg / (g_ms / 1000.0) / 1e9
     line = {
-        "metric": "c4 batched joins: Gtuples/s over 256 independent joins in one call",
-        "value": round(tuples / s_step / 1e9, 4),
+        "metric": "c4 batched joins: Gtuples/s over 256 independent joins in one graph launch",
+        "value": round(tuples / s_batch / 1e9, 4),
         "unit": "Gtuples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(s_step * 1000.0, 2),
+        "ms_per_step": round(s_batch * 1000.0, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": f"c4: {bw.description}",
-                   "path": f"cgj_execute_many: all joins in one call on {args.workers} worker streams"},
-        "joins_per_s": round(len(jobs) / s_step, 1),
-        "output_rows_per_step": rows,
+                   "path": "cgj_batch: one hipGraph per step (fills, persistent build launch, persistent probe "
+                           "launch over device task queues), autotuned task sizes / load factor / bloom bits"},
+        "joins_per_s": round(len(jobs) / s_batch, 1),
+        "output_rows_per_step": int(sum(rows_batch)),
+        "batch": {"graph_ms": round(g_ms, 3), "tasks": info["tasks"], "waves": info["waves"],
+                  "launches": info["launches"], "s_first_run": round(t_first, 2), "s_autotune": round(t_tune, 2)},
         "one_call_per_join": {"ms_per_step": round(s_seq * 1000.0, 2), "joins_per_s": round(len(jobs) / s_seq, 1),
                               "ms_device_per_step": round(ms_dev / len(seq), 2)},
-        "speedup_vs_one_call_per_join": round(s_seq / s_step, 3),
+        "execute_many": {"ms_per_step": round(s_many * 1000.0, 2), "workers": args.workers},
+        "speedup_vs_one_call_per_join": round(s_seq / s_batch, 3),
+        "speedup_vs_execute_many": round(s_many / s_batch, 3),
+        "roofline": {"kernel": "the batch graph (k_batch_build + k_batch_probe<2>)", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": int(b_alg),
+                     "kernel_ms": round(g_ms, 3)},
     }
     print(json.dumps(line), flush=True)
+    batch.free()
     for _, tabs in jobs:
         for t in tabs:
             t.free()
@@ -368,6 +421,7 @@ def main():
     ap.add_argument("--scale", type=float, default=None,
                     help="table scale (default 1; c5: 0.1, tables of 1K-4M rows)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="c4: skip the batch autotuner")
     ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
                     help="execution option (cgj_set_option), repeatable")
     ap.add_argument("--workers", type=int, default=None,
