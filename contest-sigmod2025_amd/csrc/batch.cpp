@@ -205,6 +205,7 @@ void Batch::layout() {
         L.dir        = reinterpret_cast<uint32_t*>(L.pool + J.pool_pages * kPageSize);
         L.counters   = counters + j;
         L.page_chunk = J.page_chunk;
+        L.abort_pages = ctx_.mem_bytes() / kPageSize;
         BatchBuild& B  = H.build;
         B.key          = J.bt->cols[J.bkey].view(0);
         B.nattr        = uint32_t(J.battr.size());
@@ -309,6 +310,9 @@ void Batch::run() {
                 throw std::runtime_error("batch: job " + std::to_string(j) + ": hash-table insert found no empty slot");
             if (counters_[j].fault)
                 throw std::runtime_error("batch: job " + std::to_string(j) + ": hash-table probe found no empty slot");
+            if (counters_[j].overflow && counters_[j].pages >= ctx_.mem_bytes() / kPageSize)
+                throw std::runtime_error("batch: job " + std::to_string(j) +
+                                         ": result too large, more output pages than device memory holds");
             if (counters_[j].overflow) {
                 // the pool was sized for one row per probe row; duplicates made more
                 Job& J = jobs_[j];

@@ -142,6 +142,27 @@ void cgj_dtable_free(cgj_context*, cgj_dtable* t) { delete t; }
 
 uint64_t cgj_dtable_num_rows(const cgj_dtable* t) { return t ? t->impl->nrows : 0; }
 
+int cgj_dtable_download(cgj_context* ctx, const cgj_dtable* t, uint32_t column, cgj_result_column* out) {
+    return guarded(ctx, [&] {
+        std::memset(out, 0, sizeof(*out));
+        if (!t || column >= t->impl->cols.size()) throw std::out_of_range("no such column");
+        try {
+            ctx->impl->download_column(t->impl->cols[column], *out);
+        } catch (...) {
+            free_column(*out);
+            throw;
+        }
+    });
+}
+
+int cgj_dtable_prepare(cgj_context* ctx, cgj_dtable* t) {
+    return guarded(ctx, [&] {
+        if (!t) throw std::runtime_error("null table");
+        for (auto& c : t->impl->cols)
+            if (c.npages) ctx->impl->prepare_column(c); // placeholders of unscanned columns have none
+    });
+}
+
 uint64_t cgj_dtable_num_pages(const cgj_dtable* t, uint32_t column) {
     if (!t || column >= t->impl->cols.size()) return 0;
     return t->impl->cols[column].npages;

@@ -105,6 +105,7 @@ Context::Context(int device) : device_(device), est_(std::make_unique<Estimator>
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         if (prop.multiProcessorCount > 0) cus_ = uint32_t(prop.multiProcessorCount);
         if (prop.sharedMemPerBlock > 0) lds_ = uint64_t(prop.sharedMemPerBlock);
+        if (prop.totalGlobalMem > 0) mem_ = uint64_t(prop.totalGlobalMem);
     }
 }
 
@@ -1022,6 +1023,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
 
     PipeArgs a;
     std::memset(&a, 0, sizeof(a));
+    a.abort_pages = ~uint64_t(0);
     a.xcd_order = xcd_order ? 1 : 0;
     a.prefilter = ctx_.options.block_prefilter ? 1 : 0;
     std::map<uint32_t, uint32_t> slot; // class -> tuple slot
@@ -1109,6 +1111,9 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         uint64_t chunk = std::min<uint64_t>(4096, std::max<uint64_t>(64, uint64_t(per)));
         uint64_t cap   = probes.empty() ? nrows : uint64_t(std::min(std::max(t.est * 1.25, 1024.0), 4e9));
         cap            = std::max<uint64_t>(cap, 1) + waves * chunk;
+        // staged tuples beyond what device memory holds cannot be built: the
+        // pipeline stops reserving slots there instead of enumerating them all
+        a.abort_pages  = ctx_.mem_bytes() / (uint64_t(a.stage_words) * sizeof(uint64_t));
         for (int attempt = 0;; ++attempt) {
             t.stage       = DevBuf(&ctx_.heap(), cap * a.stage_words * sizeof(uint64_t));
             a.stage       = t.stage.get<uint64_t>();
@@ -1123,9 +1128,15 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
                 t.slots = std::min<uint64_t>(c.pages, cap);
                 break;
             }
+            if (c.pages >= a.abort_pages)
+                throw std::runtime_error("join result too large: more than " + std::to_string(a.abort_pages) +
+                                         " staged tuples, beyond device memory");
             if (attempt > 2) throw std::runtime_error("staging overflow persists");
             ++retries;
             cap = c.pages + waves * chunk + 64;
+            if (cap * a.stage_words * sizeof(uint64_t) > ctx_.mem_bytes())
+                throw std::runtime_error("join result too large: " + std::to_string(c.pages) +
+                                         " staged tuples need more than the device memory");
         }
         return;
     }
@@ -1165,6 +1176,8 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
     a.str_stage      = stage_s.get<uint8_t>();
     uint64_t slack   = waves * a.page_chunk + 64;
     uint64_t pages   = uint64_t(dp) + waves * a.nout + slack;
+    // more output pages than device memory holds: the pipeline stops there
+    a.abort_pages    = ctx_.mem_bytes() / kPageSize;
     for (int attempt = 0;; ++attempt) {
         res->pool    = DevBuf(&ctx_.heap(), pages * kPageSize + pages * 16 + 64);
         a.pool       = res->pool.get<uint8_t>();
@@ -1197,9 +1210,16 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             }
             break;
         }
+        if (c.pages >= a.abort_pages)
+            throw std::runtime_error("query result too large: more than " + std::to_string(a.abort_pages) +
+                                     " output pages (" + std::to_string(a.abort_pages * kPageSize >> 30) +
+                                     " GiB), beyond device memory");
         if (attempt > 2) throw std::runtime_error("output page pool overflow persists");
         ++retries;
         pages = c.pages + slack;
+        if (pages * (kPageSize + 16) > ctx_.mem_bytes())
+            throw std::runtime_error("query result too large: " + std::to_string(c.pages) +
+                                     " output pages need more than the device memory");
     }
     probe_rows += nrows;
     if (d.rel >= 0)
@@ -1424,6 +1444,34 @@ void Context::download(const DevResult& r, cgj_result& out) {
     v.reserved_pages = r.reserved_pages;
     v.total_pages    = r.total_pages;
     download_pool(v, out);
+}
+
+void Context::download_column(const DevColumn& c, cgj_result_column& out) {
+    out.type      = c.type;
+    out.num_pages = c.npages; // entries stay null until their page is copied
+    out.pages     = new void*[c.npages ? c.npages : 1]();
+    const uint64_t per     = kStagingBytes / kPageSize;
+    const uint64_t nchunks = (c.npages + per - 1) / per;
+    auto           issue   = [&](uint64_t k) {
+        const uint64_t p0 = k * per, n = std::min(per, c.npages - p0);
+        CGJ_HIP(hipMemcpyAsync(staging(int(k & 1)), c.pages.get<uint8_t>() + p0 * kPageSize, n * kPageSize,
+                               hipMemcpyDeviceToHost, stream_));
+        staging_done(int(k & 1));
+    };
+    if (nchunks) issue(0);
+    for (uint64_t k = 0; k < nchunks; ++k) {
+        if (k + 1 < nchunks) issue(k + 1);
+        const uint8_t* buf = staging(int(k & 1)); // waits for chunk k's copy
+        const uint64_t p0 = k * per, n = std::min(per, c.npages - p0);
+        parallel_pages(
+            n,
+            [&](uint64_t i) {
+                void* pg = ::operator new(kPageSize);
+                std::memcpy(pg, buf + i * kPageSize, kPageSize);
+                out.pages[p0 + i] = pg;
+            },
+            8);
+    }
 }
 
 void Context::download_pool(const PoolView& r, cgj_result& out) {

@@ -249,6 +249,8 @@ public:
     uint64_t count(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
     void     download(const DevResult& r, cgj_result& out);
     void     download_pool(const PoolView& v, cgj_result& out);
+    // Host copy of a device column's pages, one operator-new block per page.
+    void     download_column(const DevColumn& c, cgj_result_column& out);
     // Pinned host staging of page uploads and downloads: two buffers of
     // kStagingBytes, allocated on first use and kept for the context's
     // lifetime; staging(b) waits until the last copy through buffer b is done.
@@ -286,6 +288,7 @@ public:
     std::vector<PlannedJoin> plan_tree(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
     uint32_t   cu_count() const { return cus_; }
     uint64_t   lds_bytes() const { return lds_; } // LDS per workgroup
+    uint64_t   mem_bytes() const { return mem_; } // device memory
 
     // A query imported and ordered by this context's estimator, and its
     // execution (on any context of the device) as planned.
@@ -303,6 +306,7 @@ private:
     int         device_;
     uint32_t    cus_    = 256;
     uint64_t    lds_    = 65536;
+    uint64_t    mem_    = uint64_t(64) << 30;
     hipStream_t stream_ = nullptr;
     std::shared_ptr<DeviceHeap> heap_ = std::make_shared<DeviceHeap>();
     std::unique_ptr<Estimator>  est_;

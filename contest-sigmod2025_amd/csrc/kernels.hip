@@ -91,76 +91,93 @@ __global__ void k_scan_add(uint32_t* data, uint64_t n, const uint32_t* sums, con
     if (i == 0) out[out_n] = uint32_t(*total);
 }
 
-__device__ inline bool page_bit(const uint8_t* page, uint32_t nr, uint32_t nv, uint32_t j) {
-    if (nv == nr) return true;
-    const uint8_t* bm = page + kPageSize - (nr + 7) / 8;
-    return (bm[j >> 3] >> (j & 7)) & 1;
+
+// Validity of rows [bit, bit + 64) of a page (nr rows, nv non-NULL values),
+// bit i = row bit + i; rows at or past nr read as 0. A page without NULLs is
+// all valid and its bitmap is not read; otherwise at most 9 bitmap bytes are,
+// none past the page end.
+__device__ inline uint64_t page_bits64(const uint8_t* page, uint32_t nr, uint32_t nv, uint32_t bit) {
+    if (bit >= nr) return 0;
+    const uint32_t n = nr - bit < 64 ? nr - bit : 64;
+    const uint64_t m = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+    if (nv == nr) return m;
+    const uint32_t nbytes = (nr + 7) / 8;
+    const uint8_t* bm     = page + kPageSize - nbytes;
+    const uint32_t b0 = bit >> 3, sh = bit & 7;
+    uint64_t       lo = 0, hi = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i)
+        if (b0 + i < nbytes) lo |= uint64_t(bm[b0 + i]) << (8 * i);
+    if (b0 + 8 < nbytes) hi = bm[b0 + 8];
+    const uint64_t w = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+    return w & m;
 }
 
-// Block index: one entry per 64-row block (see BlockIdx).
-__global__ void k_block_index(const uint8_t* pages, uint64_t npages, int32_t type, const uint32_t* row_start,
-                              uint64_t nrows, BlockIdx* blocks) {
-    uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b * 64 >= nrows) return;
-    uint32_t r0 = uint32_t(b * 64);
-    uint32_t r1 = uint32_t(r0 + 64 < nrows ? r0 + 64 : nrows);
-    // last page whose first row is <= r0 (it holds r0; empty pages are skipped)
-    uint64_t lo = 0, hi = npages;
-    while (hi - lo > 1) {
-        uint64_t mid = (lo + hi) / 2;
-        if (row_start[mid] <= r0) lo = mid; else hi = mid;
-    }
-    uint32_t       p    = uint32_t(lo);
-    const uint8_t* page = pages + uint64_t(p) * kPageSize;
-    uint32_t       nr   = *reinterpret_cast<const uint16_t*>(page);
-    uint32_t       nv   = *reinterpret_cast<const uint16_t*>(page + 2);
-    BlockIdx       e;
-    e.page    = p;
-    e.nn      = 0;
-    e.split   = 64;
-    e.complex = 0;
-    e.valid   = 0;
-    bool is_long = type == T_VARCHAR && nr >= 0xfffe;
-    if (is_long) {
-        e.complex = 1;
-        blocks[b] = e;
-        return;
-    }
-    uint32_t j0 = r0 - row_start[p];
-    uint32_t re = row_start[p + 1];
-    uint32_t n1 = (r1 < re ? r1 : re) - r0;
-    uint32_t nn = 0;
-    if (nv == nr) {
-        nn = j0;
-    } else {
-        const uint8_t* bm = page + kPageSize - (nr + 7) / 8;
-        for (uint32_t k = 0; k < (j0 >> 3); ++k) nn += __popc(bm[k]);
-        if (j0 & 7) nn += __popc(bm[j0 >> 3] & ((1u << (j0 & 7)) - 1));
-    }
-    e.nn = uint16_t(nn);
-    for (uint32_t t = 0; t < n1; ++t)
-        if (page_bit(page, nr, nv, j0 + t)) e.valid |= 1ull << t;
-    if (r0 + n1 < r1) {
-        // the block continues on the next page, which must hold all its remaining rows
-        // (rows remain, so p is not the last page: row_start[npages] = nrows >= r1)
-        uint32_t q = p + 1;
-        if (q >= npages || row_start[q + 1] < r1 || row_start[q] != r0 + n1) {
-            e.complex = 1;
-            blocks[b] = e;
-            return;
+// Block index, one wave per page: the blocks whose first row lies in page p
+// (64 at a time, one per lane) get their entries from the page's bitmap; a
+// block's non-NULL offset is the page's valid rows before it, from an
+// exclusive wave scan of the blocks' in-page valid counts. A block that runs
+// past the page continues on page p + 1 when that page holds all its remaining
+// rows (and is not a long-string page), else it is marked complex, as are
+// blocks on long-string pages. Reads only headers and bitmaps.
+__global__ __launch_bounds__(256) void k_block_index_pages(const uint8_t* pages, uint64_t npages, int32_t type,
+                                                          const uint32_t* row_start, uint64_t nrows,
+                                                          BlockIdx* blocks) {
+    const uint64_t p = uint64_t(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
+    if (p >= npages) return; // the whole wave
+    const uint32_t s = row_start[p], e = row_start[p + 1];
+    if (e <= s) return; // an empty page starts no block
+    const uint64_t b_first = (uint64_t(s) + 63) / 64, b_last = (uint64_t(e) - 1) / 64;
+    if (b_first > b_last) return;
+    const uint8_t* page    = pages + p * kPageSize;
+    const uint32_t nr_raw  = *reinterpret_cast<const uint16_t*>(page);
+    const uint32_t nv      = *reinterpret_cast<const uint16_t*>(page + 2);
+    const bool     is_long = type == T_VARCHAR && nr_raw >= 0xfffe;
+    const uint32_t nr      = e - s;
+    const uint32_t lane    = lane_id();
+    // valid rows of the page before the first block
+    const uint32_t j00   = uint32_t(b_first * 64 - s);
+    uint32_t       carry = is_long ? 0 : uint32_t(__popcll(page_bits64(page, nr, nv, 0) & (j00 >= 64 ? ~0ull : ((1ull << j00) - 1ull))));
+    for (uint64_t bb = b_first; bb <= b_last; bb += 64) {
+        const uint64_t b    = bb + lane;
+        const bool     have = b <= b_last;
+        BlockIdx       x;
+        x.page    = uint32_t(p);
+        x.nn      = 0;
+        x.split   = 64;
+        x.complex = is_long ? 1 : 0;
+        x.valid   = 0;
+        uint32_t cnt = 0;
+        if (have && !is_long) {
+            const uint64_t r0 = b * 64;
+            const uint64_t r1 = r0 + 64 < nrows ? r0 + 64 : nrows;
+            const uint32_t j0 = uint32_t(r0 - s);
+            x.valid           = page_bits64(page, nr, nv, j0);
+            cnt               = uint32_t(__popcll(x.valid));
+            if (r1 > e) {
+                const uint32_t n1 = uint32_t(e - r0);
+                const uint64_t q  = p + 1;
+                bool           ok = q < npages && row_start[q + 1] >= r1;
+                const uint8_t* pq = pages + q * kPageSize;
+                if (ok) {
+                    const uint32_t nrq = *reinterpret_cast<const uint16_t*>(pq);
+                    const uint32_t nvq = *reinterpret_cast<const uint16_t*>(pq + 2);
+                    ok = !(type == T_VARCHAR && nrq >= 0xfffe);
+                    if (ok) {
+                        const uint32_t rows_q = row_start[q + 1] - row_start[q];
+                        const uint32_t n2     = uint32_t(r1 - e);
+                        x.split               = uint8_t(n1);
+                        x.valid |= (page_bits64(pq, rows_q, nvq, 0) & ((1ull << n2) - 1ull)) << n1;
+                    }
+                }
+                if (!ok) x.complex = 1;
+            }
         }
-        const uint8_t* pq  = pages + uint64_t(q) * kPageSize;
-        uint32_t       nrq = *reinterpret_cast<const uint16_t*>(pq);
-        uint32_t       nvq = *reinterpret_cast<const uint16_t*>(pq + 2);
-        if (type == T_VARCHAR && nrq >= 0xfffe) {
-            e.complex = 1;
-        } else {
-            e.split = uint8_t(n1);
-            for (uint32_t t = 0; t < r1 - r0 - n1; ++t)
-                if (page_bit(pq, nrq, nvq, t)) e.valid |= 1ull << (n1 + t);
-        }
+        const uint32_t incl = wave_incl_scan(cnt);
+        x.nn                = uint16_t(carry + incl - cnt);
+        carry += __shfl(incl, 63, 64);
+        if (have) blocks[b] = x;
     }
-    blocks[b] = e;
 }
 
 size_t prepare_scratch_words(uint64_t npages) { return npages + (npages + kScanBlock - 1) / kScanBlock + 2; }
@@ -195,7 +212,9 @@ hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t 
     k_scan_add<<<dim3((npages + 255) / 256), dim3(256), 0, s>>>(counts, npages, sums, total_rows_dev, row_start,
                                                                 npages);
     uint64_t nb = (nrows + 63) / 64;
-    if (nb) k_block_index<<<dim3((nb + 255) / 256), dim3(256), 0, s>>>(pages, npages, type, row_start, nrows, blocks);
+    if (nb)
+        k_block_index_pages<<<dim3((npages + 3) / 4), dim3(256), 0, s>>>(pages, npages, type, row_start, nrows,
+                                                                         blocks);
     return hipGetLastError();
 }
 
@@ -865,7 +884,7 @@ struct WaveState {
     uint64_t rows;       // rows emitted (output) / tuples staged
     uint64_t next;       // next reserved page index / staging slot
     uint64_t end;        // end of the reserved range
-    uint64_t pad;
+    uint64_t stop;       // a reservation passed the abort bound: the result cannot fit, the wave stops
 };
 
 // Per-wave state of one output column's open page (wave-uniform values).
@@ -892,6 +911,7 @@ __device__ inline void page_open(const PipeArgs& a, WaveState& ws, OutState& st,
         const unsigned long long base = wave_fetch_add(&a.counters->pages, (unsigned long long)a.page_chunk);
         next   = base;
         ws.end = base + a.page_chunk;
+        if (base >= a.abort_pages) ws.stop = 1;
     }
     uint64_t idx = next;
     ws.next      = next + 1;
@@ -1086,6 +1106,7 @@ __device__ inline void stage_put(const PipeArgs& a, WaveState& ws, bool have, ui
     uint64_t base2 = 0;
     if (cnt > rem) {
         base2 = wave_fetch_add(&a.counters->pages, (unsigned long long)a.stage_chunk);
+        if (base2 >= a.abort_pages) ws.stop = 1;
         if (base2 + a.stage_chunk > a.stage_cap && lead_lane()) atomicExch(&a.counters->overflow, 1ull);
     }
     if (have) {
@@ -1238,10 +1259,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4
     const bool try_prefilter = NA == 8 && bmask != 0 && a.prefilter;
     uint32_t   pf_seen = 0, pf_live = 0;
 
-    for (uint64_t m = m_first; m < m_end; m += m_step) {
+    for (uint64_t m = m_first; m < m_end && !ws.stop; m += m_step) {
         uint64_t r0  = m * a.morsel_rows;
         uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
-        for (uint64_t cb = r0; cb < end; cb += 64 * 64) { // runs of 64 blocks (one morsel at most 4096 rows)
+        for (uint64_t cb = r0; cb < end && !ws.stop; cb += 64 * 64) { // runs of 64 blocks (a morsel: <= 4096 rows)
         const uint64_t cend = cb + 64 * 64 < end ? cb + 64 * 64 : end;
         const uint32_t nblk = uint32_t((cend - cb + 63) / 64);
         uint64_t       live = nblk >= 64 ? ~0ull : ((1ull << nblk) - 1);
@@ -1250,7 +1271,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4
             pf_seen += nblk;
             pf_live += uint32_t(__popcll(live));
         }
-        for (; live; live &= live - 1) {
+        for (; live && !ws.stop; live &= live - 1) {
             const uint64_t rb = cb + 64 * uint64_t(__builtin_ctzll(live));
             uint64_t row    = rb + lane;
             bool     active = row < end;
@@ -1465,7 +1486,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4
                     }
                 }
                 mult = 1;
-                if (!__ballot(alive)) break;
+                if (!__ballot(alive) || ws.stop) break;
             }
         }
         } // runs of 64 blocks
@@ -1544,6 +1565,7 @@ __device__ inline void lean_open(const LeanArgs& a, WaveState& ws, LeanPage& st,
         const uint32_t chunk = a.page_chunk;
         next   = wave_fetch_add(&a.counters->pages, (unsigned long long)chunk);
         ws.end = next + chunk;
+        if (next >= a.abort_pages) ws.stop = 1;
     }
     ws.next  = next + 1;
     // readfirstlane keeps these scalar: the compiler otherwise folds the
@@ -1688,7 +1710,7 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                                  int32_t unique) {
     const uint32_t lane = lane_id();
     {
-        for (uint64_t rb = r0; rb < end; rb += 64) {
+        for (uint64_t rb = r0; rb < end && !W.ws.stop; rb += 64) {
             const LeanArgs& a      = args_at<LeanArgs>(args_addr); // re-read per block (scalar cache)
             const uint64_t  row    = rb + lane;
             const bool      active = row < end;
@@ -1800,7 +1822,7 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                         }
                     }
                 }
-                if (!__ballot(live)) break;
+                if (!__ballot(live) || W.ws.stop) break;
             }
         }
     }
@@ -1843,7 +1865,7 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
         m_end        = a.nmorsels * (x + 1) / 8;
         m_step       = uint64_t(bpx) * kWavesPerBlock;
     }
-    for (uint64_t m = m_first; m < m_end;) {
+    for (uint64_t m = m_first; m < m_end && !W.ws.stop;) {
         uint64_t m_next = m + m_step;
         if (!a.xcd_order) {
             m_next = nwaves + wave_fetch_add(&a.counters->morsel, 1ull);
@@ -2087,6 +2109,7 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
     l.counters   = a.counters;
     l.page_chunk = a.page_chunk ? a.page_chunk : kPageChunk;
     l.aligned_pairs = aligned_pairs && P.words <= 2 ? 1u : 0u;
+    l.abort_pages   = a.abort_pages;
     // With the dynamic morsel schedule a grid larger than one resident wave
     // set only adds a tail: cap it at what the kernel's occupancy fits.
     static const void* const kernels[kLeanOut + 1] = {

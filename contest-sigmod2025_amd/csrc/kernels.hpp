@@ -96,6 +96,8 @@ struct PipeArgs {
     uint32_t        page_chunk;  // pool pages reserved per atomic (1..kPageChunk)
     uint32_t        grid_per_cu; // host only: waves per CU of the launch (0 = default)
     uint32_t        prefilter;   // pages driver: test early bloom filters four blocks at a time first
+    uint64_t        abort_pages; // a wave whose reservation starts at or past this many pool pages (or
+                                 // staging slots) stops: the result cannot fit device memory
 };
 
 // ---- lean one-probe pipeline ----
@@ -141,6 +143,7 @@ struct LeanArgs {
     // (slot q & ~1 ignored when q is odd) instead of {q, q + 1}: with 16-byte
     // entries a step then never spans two 64-byte lines. Entries of <= 2 words.
     uint32_t        aligned_pairs;
+    uint64_t        abort_pages; // as PipeArgs::abort_pages
 };
 
 // ---- batched joins: one persistent launch over a device work queue ----

@@ -147,6 +147,8 @@ SIGNATURES = {
     "cgj_dtable_free": (None, [C.c_void_p, C.c_void_p]),
     "cgj_dtable_num_rows": (C.c_uint64, [C.c_void_p]),
     "cgj_dtable_num_pages": (C.c_uint64, [C.c_void_p, C.c_uint32]),
+    "cgj_dtable_prepare": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cgj_dtable_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(ResultColumn)]),
     "cgj_execute_device": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
                                      C.POINTER(C.c_void_p)]),
     "cgj_dresult_num_rows": (C.c_uint64, [C.c_void_p]),
@@ -379,6 +381,32 @@ class DeviceTable:
     def num_pages(self, column):
         return lib().cgj_dtable_num_pages(self.handle, column)
 
+    def download_pages(self, columns) -> HostPages:
+        """Host copies of the given columns' pages (the pages a query scanned)."""
+        n = lib().cgj_dtable_num_rows(self.handle)
+        cols = []
+        try:
+            for c in columns:
+                rc = ResultColumn()
+                if lib().cgj_dtable_download(self.ctx.handle, self.handle, c, C.byref(rc)) != 0:
+                    raise RuntimeError(self.ctx.error())
+                cols.append(rc)
+        except Exception:
+            for rc in cols:
+                lib().cgj_column_release(C.byref(rc))
+            raise
+
+        def rel():
+            for rc in cols:
+                lib().cgj_column_release(C.byref(rc))
+        return HostPages(n, cols, rel)
+
+    def prepare(self):
+        """Rebuilds the row prefix and block index of every column (the
+        reference's per-query search vectors)."""
+        if lib().cgj_dtable_prepare(self.ctx.handle, self.handle) != 0:
+            raise RuntimeError(self.ctx.error())
+
     def decode(self, column):
         """Decodes a column on the GPU into uint64 slots (NULL = all ones)."""
         out = np.zeros(max(self.num_rows, 1), dtype=np.uint64)
@@ -394,6 +422,39 @@ class DeviceTable:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class HostPages:
+    """Library-owned host pages of a table (a downloaded result or device
+    table), kept as page pointers without per-page Python objects: a checker
+    reads them through host_table(). Call release() when done."""
+
+    def __init__(self, num_rows, columns, release):
+        self.num_rows = num_rows
+        self._cols = columns          # ResultColumn structures
+        self._release = release
+        self._view = None
+
+    def host_table(self) -> HostTable:
+        if self._view is None:
+            cols = (HostColumn * max(len(self._cols), 1))()
+            for i, c in enumerate(self._cols):
+                cols[i] = HostColumn(c.pages, c.num_pages, c.type)
+            self._view = (cols, HostTable(self.num_rows, cols, len(self._cols)))
+        return self._view[1]
+
+    def release(self):
+        if self._release:
+            self._release()
+            self._release = None
+            self._cols = []
+            self._view = None
+
+    def __del__(self):
+        try:
+            self.release()
         except Exception:
             pass
 
@@ -419,6 +480,15 @@ class DeviceResult:
             return result_to_table(res)
         finally:
             lib().cgj_result_release(C.byref(res))
+
+    def download_pages(self) -> HostPages:
+        """The result's pages in host memory as the contest receives them,
+        without Python copies."""
+        res = Result()
+        if lib().cgj_dresult_download(self.ctx.handle, self.handle, C.byref(res)) != 0:
+            raise RuntimeError(self.ctx.error())
+        cols = [res.columns[i] for i in range(res.num_columns)]
+        return HostPages(res.num_rows, cols, lambda: lib().cgj_result_release(C.byref(res)))
 
     def free(self):
         if self.handle:
@@ -473,6 +543,14 @@ class Batch:
             return result_to_table(res)
         finally:
             lib().cgj_result_release(C.byref(res))
+
+    def download_pages(self, j: int) -> "HostPages":
+        """Job j's result pages in host memory, without Python copies."""
+        res = Result()
+        if lib().cgj_batch_download(self.ctx.handle, self.handle, j, C.byref(res)) != 0:
+            raise RuntimeError(self.ctx.error())
+        cols = [res.columns[i] for i in range(res.num_columns)]
+        return HostPages(res.num_rows, cols, lambda: lib().cgj_result_release(C.byref(res)))
 
     def free(self):
         if self.handle:

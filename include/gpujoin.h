@@ -183,6 +183,17 @@ cgj_dtable* cgj_dtable_upload(cgj_context* ctx, const cgj_host_table* table);
  * before or after their context is destroyed (`ctx` may then be NULL). */
 void        cgj_dtable_free(cgj_context* ctx, cgj_dtable* t);
 uint64_t    cgj_dtable_num_rows(const cgj_dtable* t);
+/* Rebuilds every column's rows-per-page prefix and 64-row block index, the
+ * per-query work of TableScan::prepareColumn / makeSearchVector (reference
+ * engine/op/TableScan.cpp:546-582, run from QueryPlan.cpp:195-198 inside every
+ * execute()). Tables are indexed when uploaded or generated; this entry point
+ * repeats that work so that a per-query measurement can include it. */
+int         cgj_dtable_prepare(cgj_context* ctx, cgj_dtable* t);
+/* Copies one column's pages of a device table to host memory (the inverse of
+ * cgj_dtable_upload), one operator-new block per page; release with
+ * cgj_column_release. Lets a checker read exactly the pages a query scanned
+ * (for example tables generated in HBM). */
+int         cgj_dtable_download(cgj_context* ctx, const cgj_dtable* t, uint32_t column, cgj_result_column* out);
 uint64_t    cgj_dtable_num_pages(const cgj_dtable* t, uint32_t column);
 
 /* Runs `num_plans` independent plans over device-resident inputs concurrently:

@@ -51,6 +51,14 @@ def lib():
         l.oracle_result_release.argtypes = [C.POINTER(gj.Result)]
         l.oracle_decode_fixed.restype = C.c_int
         l.oracle_decode_fixed.argtypes = [C.POINTER(gj.HostColumn), C.c_uint64, C.c_void_p, C.c_char_p, C.c_size_t]
+        l.oracle_table_fingerprint.restype = C.c_int
+        l.oracle_table_fingerprint.argtypes = [C.POINTER(gj.HostTable), C.c_uint32, C.POINTER(C.c_uint64),
+                                               C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]
+        l.oracle_join_fingerprint.restype = C.c_int
+        l.oracle_join_fingerprint.argtypes = [C.POINTER(gj.HostTable), C.c_uint32, C.POINTER(gj.HostTable),
+                                              C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint32,
+                                              C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                              C.c_char_p, C.c_size_t]
         l.oracle_join_count.restype = C.c_uint64
         l.oracle_join_count.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
         _lib = l
@@ -92,3 +100,32 @@ def join_count(build_keys, probe_keys):
     b = np.ascontiguousarray(build_keys, dtype=np.uint64)
     p = np.ascontiguousarray(probe_keys, dtype=np.uint64)
     return int(lib().oracle_join_count(b.ctypes.data, len(b), p.ctypes.data, len(p)))
+
+
+def _threads(threads):
+    return threads or max(1, min(16, os.cpu_count() or 1))
+
+
+def table_fingerprint(host_table, threads=0):
+    """(rows, fingerprint) of a gpujoin.HostTable of fixed-width columns: the
+    sum over rows of a hash of the row's slots, independent of row order and
+    page packing."""
+    n, fp = C.c_uint64(), C.c_uint64()
+    err = C.create_string_buffer(512)
+    if lib().oracle_table_fingerprint(C.byref(host_table), _threads(threads), C.byref(n), C.byref(fp), err, 512):
+        raise RuntimeError(err.value.decode())
+    return n.value, fp.value
+
+
+def join_fingerprint(build, bkey, probe, pkey, outputs, threads=0):
+    """(rows, fingerprint) of the inner equi-join build[bkey] = probe[pkey] of
+    two gpujoin.HostTables projected on `outputs` ((side, column) pairs, side 0
+    = build, 1 = probe), computed by a threaded CPU hash join."""
+    side = (C.c_uint32 * max(len(outputs), 1))(*[s for s, _ in outputs])
+    col = (C.c_uint32 * max(len(outputs), 1))(*[c for _, c in outputs])
+    n, fp = C.c_uint64(), C.c_uint64()
+    err = C.create_string_buffer(512)
+    if lib().oracle_join_fingerprint(C.byref(build), bkey, C.byref(probe), pkey, side, col, len(outputs),
+                                     _threads(threads), C.byref(n), C.byref(fp), err, 512):
+        raise RuntimeError(err.value.decode())
+    return n.value, fp.value

@@ -23,12 +23,15 @@
 //     short string; long strings are copied page by page.
 // The tree is evaluated in the order the Plan gives (build side per
 // build_left); the reference re-plans, which changes no result row.
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -304,6 +307,125 @@ struct PageWriter {
     }
 };
 
+// ---- full-size checks: threaded decode and order-independent fingerprints ----
+
+// Runs f(t) for t in [0, n) on n threads; rethrows the first failure.
+template <class F>
+void on_threads(uint32_t n, F&& f) {
+    std::vector<std::thread>        th;
+    std::vector<std::exception_ptr> err(n);
+    for (uint32_t t = 0; t < n; ++t)
+        th.emplace_back([&, t] {
+            try {
+                f(t);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    for (auto& x : th) x.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
+// Decodes a fixed-width column into slots with `threads` threads: page row
+// counts first (their prefix gives each thread's first output row), then each
+// thread decodes a contiguous range of pages as decode_column does.
+std::vector<uint64_t> decode_fixed_parallel(const cgj_host_column& col, uint64_t num_rows, uint32_t threads) {
+    if (col.type == CGJ_VARCHAR) throw std::runtime_error("fixed-width columns only");
+    const uint64_t        np = col.num_pages;
+    std::vector<uint64_t> start(np + 1, 0);
+    for (uint64_t p = 0; p < np; ++p) start[p + 1] = start[p] + rd16(static_cast<const uint8_t*>(col.pages[p]));
+    if (start[np] != num_rows) throw std::runtime_error("page rows do not sum to num_rows");
+    std::vector<uint64_t> out(num_rows);
+    threads = std::max<uint32_t>(1, std::min<uint64_t>(threads, np ? np : 1));
+    on_threads(threads, [&](uint32_t t) {
+        for (uint64_t p = np * t / threads; p < np * (t + 1) / threads; ++p) {
+            const uint8_t* page = static_cast<const uint8_t*>(col.pages[p]);
+            const uint16_t nr = rd16(page), nv = rd16(page + 2);
+            const uint8_t* bm = page + kPage - (nr + 7u) / 8u;
+            uint64_t*      o  = out.data() + start[p];
+            uint32_t       k  = 0;
+            for (uint32_t r = 0; r < nr; ++r) {
+                if (nv != nr && !((bm[r / 8] >> (r % 8)) & 1u)) {
+                    o[r] = kNull;
+                    continue;
+                }
+                if (col.type == CGJ_INT32) {
+                    uint32_t v;
+                    std::memcpy(&v, page + 4 + 4 * k, 4);
+                    o[r] = v;
+                } else {
+                    std::memcpy(&o[r], page + 8 + 8 * size_t(k), 8);
+                }
+                ++k;
+            }
+        }
+    });
+    return out;
+}
+
+// SplitMix64 finaliser (Steele et al. 2014).
+inline uint64_t smix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Row fingerprint: depends on every slot and its column position; a table's
+// fingerprint is the sum over its rows (order independent, duplicates count).
+inline uint64_t row_hash(const uint64_t* slots, uint32_t n) {
+    uint64_t h = 0x243f6a8885a308d3ull;
+    for (uint32_t c = 0; c < n; ++c) h = smix(h ^ (slots[c] + 0x9e3779b97f4a7c15ull * (c + 1)));
+    return h;
+}
+
+// Open-addressing multimap from a non-NULL key to build rows, split into
+// 2^pbits partitions by the key hash so that threads build disjoint parts.
+struct KeyRows {
+    uint32_t                           pbits = 0;
+    std::vector<std::vector<uint64_t>> keys, rows; // per partition: slots (kNull = empty)
+    std::vector<uint64_t>              masks;
+
+    uint32_t part(uint64_t h) const { return pbits ? uint32_t(h >> (64 - pbits)) : 0u; }
+
+    void build(const std::vector<uint64_t>& bk, uint32_t threads) {
+        pbits = 0;
+        while ((1u << pbits) < threads) ++pbits;
+        const uint32_t np = 1u << pbits;
+        keys.assign(np, {});
+        rows.assign(np, {});
+        masks.assign(np, 0);
+        std::vector<uint64_t> count(np, 0);
+        for (uint64_t v : bk)
+            if (v != kNull) ++count[part(smix(v))];
+        on_threads(np, [&](uint32_t p) {
+            uint64_t cap = 16;
+            while (cap < 2 * count[p]) cap <<= 1;
+            keys[p].assign(cap, kNull);
+            rows[p].assign(cap, 0);
+            masks[p] = cap - 1;
+            for (uint64_t r = 0; r < bk.size(); ++r) {
+                const uint64_t v = bk[r];
+                if (v == kNull) continue;
+                const uint64_t h = smix(v);
+                if (part(h) != p) continue;
+                uint64_t s = h & masks[p];
+                while (keys[p][s] != kNull) s = (s + 1) & masks[p];
+                keys[p][s] = v;
+                rows[p][s] = r;
+            }
+        });
+    }
+    // Calls f(build row) for every build row holding key v.
+    template <class F>
+    void each(uint64_t v, F&& f) const {
+        const uint64_t h = smix(v);
+        const uint32_t p = part(h);
+        for (uint64_t s = h & masks[p]; keys[p][s] != kNull; s = (s + 1) & masks[p])
+            if (keys[p][s] == v) f(rows[p][s]);
+    }
+};
+
 void set_error(char* err, size_t len, const char* msg) {
     if (err && len) std::snprintf(err, len, "%s", msg);
 }
@@ -367,6 +489,91 @@ int oracle_decode_fixed(const cgj_host_column* col, uint64_t num_rows, uint64_t*
         std::vector<uint64_t> out;
         decode_column(*col, num_rows, pool, out);
         std::memcpy(slots, out.data(), out.size() * 8);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+// Row count and fingerprint (sum of row_hash over rows) of a table of
+// fixed-width columns, decoded with `threads` threads. The order of rows and
+// the packing of pages do not change it.
+int oracle_table_fingerprint(const cgj_host_table* t, uint32_t threads, uint64_t* count, uint64_t* fp, char* err,
+                             size_t errlen) {
+    try {
+        std::vector<std::vector<uint64_t>> cols;
+        for (uint32_t c = 0; c < t->num_columns; ++c)
+            cols.push_back(decode_fixed_parallel(t->columns[c], t->num_rows, threads));
+        const uint32_t        n = t->num_columns, nt = std::max<uint32_t>(threads, 1);
+        std::vector<uint64_t> part(nt, 0);
+        on_threads(nt, [&](uint32_t k) {
+            uint64_t acc = 0, row[64];
+            for (uint64_t r = t->num_rows * k / nt; r < t->num_rows * (k + 1) / nt; ++r) {
+                for (uint32_t c = 0; c < n; ++c) row[c] = cols[c][r];
+                acc += row_hash(row, n);
+            }
+            part[k] = acc;
+        });
+        uint64_t total = 0;
+        for (uint64_t v : part) total += v;
+        *count = t->num_rows;
+        *fp    = total;
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+// Expected row count and fingerprint of the inner equi-join of two tables on
+// build column `bkey` = probe column `pkey` (NULL never matches, duplicates
+// multiply), projected on `nout` columns (side 0 = build, 1 = probe, and the
+// column of that table), by a partitioned open-addressing hash join on the
+// CPU with `threads` threads. Fixed-width columns only.
+int oracle_join_fingerprint(const cgj_host_table* build, uint32_t bkey, const cgj_host_table* probe, uint32_t pkey,
+                            const uint32_t* out_side, const uint32_t* out_col, uint32_t nout, uint32_t threads,
+                            uint64_t* count, uint64_t* fp, char* err, size_t errlen) {
+    try {
+        if (nout > 64) throw std::runtime_error("too many output columns");
+        const uint32_t nt = std::max<uint32_t>(threads, 1);
+        // decode every column that is read (key columns and outputs)
+        std::vector<std::vector<uint64_t>> bc(build->num_columns), pc(probe->num_columns);
+        auto need = [&](uint32_t side, uint32_t col) {
+            const cgj_host_table* t = side ? probe : build;
+            auto&                 v = side ? pc : bc;
+            if (col >= t->num_columns) throw std::runtime_error("bad column index");
+            if (v[col].empty() && t->num_rows) v[col] = decode_fixed_parallel(t->columns[col], t->num_rows, nt);
+        };
+        need(0, bkey);
+        need(1, pkey);
+        for (uint32_t o = 0; o < nout; ++o) need(out_side[o], out_col[o]);
+        KeyRows kr;
+        kr.build(bc[bkey], nt);
+        std::vector<uint64_t> cnt(nt, 0), acc(nt, 0);
+        const uint64_t        np = probe->num_rows;
+        if (np)
+            on_threads(nt, [&](uint32_t k) {
+                uint64_t row[64], c = 0, a = 0;
+                for (uint64_t r = np * k / nt; r < np * (k + 1) / nt; ++r) {
+                    const uint64_t v = pc[pkey][r];
+                    if (v == kNull) continue;
+                    kr.each(v, [&](uint64_t b) {
+                        for (uint32_t o = 0; o < nout; ++o) row[o] = out_side[o] ? pc[out_col[o]][r] : bc[out_col[o]][b];
+                        a += row_hash(row, nout);
+                        ++c;
+                    });
+                }
+                cnt[k] = c;
+                acc[k] = a;
+            });
+        uint64_t c = 0, a = 0;
+        for (uint32_t k = 0; k < nt; ++k) {
+            c += cnt[k];
+            a += acc[k];
+        }
+        *count = c;
+        *fp    = a;
         return 0;
     } catch (const std::exception& e) {
         set_error(err, errlen, e.what());

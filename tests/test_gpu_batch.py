@@ -10,6 +10,7 @@ import gpujoin as gj
 import oracle
 from gpujoin.pages import FP64, INT32, INT64
 from helpers import check_table
+from test_gpu_many import assert_join_result
 
 pytestmark = pytest.mark.gpu
 
@@ -100,6 +101,8 @@ def test_batch_matches_one_launch_per_join(ctx):
         plain = check_table(r.download(), types)
         r.free()
         assert check_table(batch.download(j), types) == plain, j
+        # every job against the oracle's CPU join over the same input pages
+        assert_join_result(batch.download_pages(j), tables[j], [(0, 1), (1, 1)], j)
     # and one job against the oracle on host-generated pages
     job = wl.jobs[3]
     host = job.make_plan()

@@ -129,9 +129,12 @@ def test_contest_surface_random_plans_vs_oracle():
     rng = random.Random(4242)
     plans = []
     for i in range(24):
-        plans.append(random_plan(rng, ntables=rng.choice([2, 3, 4, 5]), max_rows=rng.choice([30, 300, 3000]),
-                                 key_type=rng.choice([INT32, INT64]), domain=rng.choice([10, 200]),
-                                 long_strings=(i % 6 == 0)))
+        ntables, max_rows = rng.choice([2, 3, 4, 5]), rng.choice([30, 300, 3000])
+        # bounded results: with d key values per join column each join
+        # multiplies the row count by about rows/d
+        domain = max(rng.choice([10, 200]), max_rows if ntables >= 3 else max_rows // 8)
+        plans.append(random_plan(rng, ntables=ntables, max_rows=max_rows, key_type=rng.choice([INT32, INT64]),
+                                 domain=domain, long_strings=(i % 6 == 0)))
     got = run_driver(plans)
     for i, (p, rows) in enumerate(zip(plans, got)):
         assert rows == canon_rows(table_rows(oracle.execute(p))), i

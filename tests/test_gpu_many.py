@@ -14,6 +14,21 @@ from helpers import check_table, random_plan
 pytestmark = pytest.mark.gpu
 
 
+def assert_join_result(pages, tables, outputs, label):
+    """The downloaded result pages of a build/probe join of two device tables
+    (key in column 0 of both) hold the rows of the oracle's CPU join over the
+    same input pages."""
+    got = oracle.table_fingerprint(pages.host_table())
+    pages.release()
+    b = tables[0].download_pages([0, 1])
+    p = tables[1].download_pages([0, 1])
+    exp = oracle.join_fingerprint(b.host_table(), 0, p.host_table(), 0, outputs)
+    b.release()
+    p.release()
+    assert got == exp, (label, got, exp)
+    return got[0]
+
+
 def say(msg):
     print(msg, flush=True)
     sys.stdout.flush()
@@ -45,16 +60,19 @@ def test_execute_many_matches_oracle_and_single_runs(ctx, workers):
         assert got == check_table(oracle.execute(p), types), i
 
 
-def test_execute_many_on_generated_joins(ctx):
-    """Sixteen c4-shaped joins (generated in HBM) at 1/100 scale: row counts
-    equal those of one execute_device per join."""
+def test_execute_many_c4_recipe_vs_oracle(ctx):
+    """Sixteen c4-shaped joins (generated in HBM) at 1/100 scale in one
+    execute_many call: every job's result rows against the oracle, as the
+    oracle's fingerprint of the downloaded result pages and of a CPU hash join
+    over the same input pages."""
     bw = wl.c4(scale=0.01, njobs=16)
     plans = [w.make_plan() for w in bw.jobs]
     tables = [[ctx.generate(w.seed, t.rows, t.columns) for t in w.tables] for w in bw.jobs]
-    say("execute_many: 16 generated joins")
+    say("execute_many: 16 generated c4 joins vs the oracle")
     res = ctx.execute_many(plans, tables, workers=4)
-    for p, ts, r in zip(plans, tables, res):
-        assert r.num_rows == ctx.execute_device(p, ts).num_rows
+    for j, (ts, r) in enumerate(zip(tables, res)):
+        assert_join_result(r.download_pages(), ts, [(0, 1), (1, 1)], j)
+        r.free()
 
 
 def test_execute_many_reports_errors(ctx):

@@ -48,6 +48,29 @@ def test_random_plans_vs_oracle(ctx, seed):
     assert got == exp
 
 
+def test_result_beyond_device_memory_is_refused(ctx):
+    """A five-table plan with a 4-value join key domain whose result has
+    11,109,749,415,203 rows (about 10^14 bytes of pages; counted on the CPU by
+    tree message passing). The pipeline that stages or writes it stops once
+    its reservations pass what device memory holds, and execute raises a clear
+    error within seconds instead of enumerating the whole result first."""
+    import time
+
+    rng = random.Random(1002)
+    plan = random_plan(rng, ntables=rng.randint(1, 6), max_rows=rng.choice([5, 50, 400, 3000]),
+                       key_type=rng.choice([INT32, INT64]), domain=rng.choice([4, 30, 1000]))
+    assert [t.num_rows for t in plan.inputs] == [2545, 1462, 2832, 1495, 677]
+    say("result beyond device memory")
+    t0 = time.perf_counter()
+    with pytest.raises(RuntimeError, match="too large|out of memory"):
+        ctx.execute(plan)
+    assert time.perf_counter() - t0 < 60
+    # the context stays usable
+    small = random_plan(random.Random(3), ntables=2, max_rows=50, domain=100)
+    types = [t for _, t in small.nodes[small.root].output_attrs]
+    assert check_table(ctx.execute(small), types) == check_table(oracle.execute(small), types)
+
+
 @pytest.mark.parametrize("seed", range(16))
 def test_block_prefilter_with_bloom_filters_on_every_join(ctx, seed):
     """Every join gets a bloom filter, so pipelines over pages with eight
@@ -582,6 +605,41 @@ def test_c2_full_size_count_matches_pages(ctx):
     assert res.num_rows == count > 400_000_000
     st = ctx.stats()
     assert st["output_rows"] == count and st["retries"] == 0, st
+
+
+def test_c2_full_size_fingerprint_vs_oracle(ctx):
+    """Every row of the full c2 query (50M x 500M, ~407M output rows) against
+    the oracle: the result pages, downloaded as the contest receives them, are
+    fingerprinted by the oracle's page decoder (row count and an order-free
+    sum of row hashes), and the expected count and fingerprint come from a CPU
+    hash join over the same input pages, downloaded from HBM."""
+    from gpujoin import workloads
+
+    wl = workloads.c2(1.0)
+    say("c2 full size: result fingerprint vs CPU join over the same pages")
+    tables = [ctx.generate(wl.seed, t.rows, t.columns) for t in wl.tables]
+    plan = wl.make_plan()
+    for t in tables:
+        t.prepare()
+    res = ctx.execute_device(plan, tables)
+    n_dev = res.num_rows
+    pages = res.download_pages()
+    res.free()
+    say(f"c2 full size: {n_dev} rows downloaded, fingerprinting")
+    got = oracle.table_fingerprint(pages.host_table())
+    pages.release()
+    build = tables[0].download_pages([0, 1])
+    probe = tables[1].download_pages([0, 1])
+    for t in tables:
+        t.free()
+    say("c2 full size: CPU hash join over the downloaded input pages")
+    # plan output: build payload, probe payload, probe key
+    exp = oracle.join_fingerprint(build.host_table(), 0, probe.host_table(), 0, [(0, 1), (1, 1), (1, 0)])
+    build.release()
+    probe.release()
+    say(f"c2 full size: GPU {got}, CPU {exp}")
+    assert got[0] == n_dev > 400_000_000
+    assert got == exp
 
 
 def test_c3_recipe_scaled_vs_oracle(ctx):
