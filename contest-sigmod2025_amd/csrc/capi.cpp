@@ -12,6 +12,7 @@
 #include "engine.hpp"
 #include "estimator.hpp"
 #include "gen.hpp"
+#include "noalloc.hpp"
 
 struct cgj_context {
     std::unique_ptr<cgj::Context> impl;
@@ -315,6 +316,10 @@ int cgj_count_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* 
     });
 }
 
+int cgj_count_device_noalloc(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                             uint64_t* rows) {
+    return guarded(ctx, [&] { *rows = cgj::count_join_noalloc(*ctx->impl, *plan, views_of(inputs, num_inputs)); });
+}
 
 uint32_t cgj_num_features(void) { return cgj::kFeatures; }
 

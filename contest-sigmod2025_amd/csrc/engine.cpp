@@ -22,6 +22,7 @@
 
 #include "estimator.hpp"
 #include "gen.hpp"
+#include "noalloc.hpp"
 #include "planner.hpp"
 
 namespace cgj {
@@ -107,6 +108,7 @@ Context::Context(int device) : device_(device), est_(std::make_unique<Estimator>
         if (prop.sharedMemPerBlock > 0) lds_ = uint64_t(prop.sharedMemPerBlock);
         if (prop.totalGlobalMem > 0) mem_ = uint64_t(prop.totalGlobalMem);
     }
+    (void)noalloc_scratch(device); // per-device address of the no-allocation entry point's array
 }
 
 Context::~Context() {

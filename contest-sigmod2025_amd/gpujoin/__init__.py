@@ -163,6 +163,8 @@ SIGNATURES = {
     "cgj_dtable_generate": (C.c_void_p, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(GenColumn), C.c_uint32]),
     "cgj_host_generate": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(GenColumn),
                                     C.POINTER(ResultColumn)]),
+    "cgj_count_device_noalloc": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
+                                           C.POINTER(C.c_uint64)]),
     "cgj_count_device": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
                                    C.POINTER(C.c_uint64)]),
     "cgj_copy_probe": (C.c_double, [C.c_void_p, C.c_uint64, C.c_uint32]),

@@ -225,6 +225,19 @@ int      cgj_last_stats(const cgj_context* ctx, cgj_stats* out);
 int      cgj_count_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
                           uint64_t* rows);
 
+/* No counterpart in the reference: cgj_count_device for a plan that joins two
+ * scans on one key, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops). It calls no hipMalloc/hipFree: the
+ * hash table (key-only slots at load <= 0.5) and its counters live in a fixed
+ * device array of CGJ_NOALLOC_TABLE_BYTES (plus 4 KiB) that each GPU the
+ * library runs on holds until the process ends. A build side (the smaller
+ * input) whose table would not fit is rejected (-1, cgj_last_error names the
+ * limit), as is any other plan shape. Calls on one GPU share that array and
+ * each clears it first: calls that overlap on one GPU race. */
+#define CGJ_NOALLOC_TABLE_BYTES (64u << 20)
+int      cgj_count_device_noalloc(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs,
+                                  uint32_t num_inputs, uint64_t* rows);
+
 /* Bandwidth probe: copies `bytes` bytes between two HBM buffers with a plain
  * streaming kernel, `reps` times; returns the mean kernel time in ms (or a
  * negative value on error). Calibrates HBM byte counters and reports the
