@@ -159,8 +159,10 @@ int cgj_dtable_download(cgj_context* ctx, const cgj_dtable* t, uint32_t column, 
 int cgj_dtable_prepare(cgj_context* ctx, cgj_dtable* t) {
     return guarded(ctx, [&] {
         if (!t) throw std::runtime_error("null table");
+        std::vector<cgj::DevColumn*> cols;
         for (auto& c : t->impl->cols)
-            if (c.npages) ctx->impl->prepare_column(c); // placeholders of unscanned columns have none
+            if (c.npages) cols.push_back(&c); // placeholders of unscanned columns have none
+        ctx->impl->prepare_columns(cols);
     });
 }
 
@@ -316,6 +318,27 @@ int cgj_count_device(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* 
     });
 }
 
+int cgj_sample_cardinalities(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
+                             double* cards, uint32_t cap, uint32_t* num_rels) {
+    return guarded(ctx, [&] {
+        const auto views = views_of(inputs, num_inputs);
+        cgj::PlannedQuery pq;
+        std::vector<uint64_t> rows;
+        std::vector<uint32_t> cols;
+        for (auto* t : views) {
+            rows.push_back(t->nrows);
+            cols.push_back(uint32_t(t->cols.size()));
+        }
+        const cgj::Query q = cgj::import_plan(*plan, rows, cols);
+        *num_rels          = uint32_t(q.rels.size());
+        bool empty         = false;
+        for (const auto& r : q.rels) empty = empty || r.rows == 0;
+        std::shared_ptr<cgj::Samples> s;
+        if (!empty) s = ctx->impl->take_samples(q, views);
+        for (uint32_t r = 0; r < q.rels.size() && r < cap; ++r) cards[r] = s ? s->card(r) : 0.0;
+    });
+}
+
 int cgj_count_device_noalloc(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
                              uint64_t* rows) {
     return guarded(ctx, [&] { *rows = cgj::count_join_noalloc(*ctx->impl, *plan, views_of(inputs, num_inputs)); });
@@ -325,7 +348,7 @@ uint32_t cgj_num_features(void) { return cgj::kFeatures; }
 
 int cgj_set_estimator(cgj_context* ctx, uint32_t mode, cgj_mlp* mlp, float max_sigma) {
     return guarded(ctx, [&] {
-        if (mode > cgj::EST_LEARNED) throw std::runtime_error("unknown estimator mode");
+        if (mode > cgj::EST_SAMPLE) throw std::runtime_error("unknown estimator mode");
         if (mode == cgj::EST_LEARNED && (!mlp || mlp->impl->in() != cgj::kFeatures))
             throw std::runtime_error("the learned estimator needs a network with cgj_num_features() inputs");
         auto& e     = ctx->impl->estimator();

@@ -100,17 +100,23 @@ __device__ inline bool lead_lane() {
 }
 
 // Adds `n` to *p once for the whole wave and returns the previous value to
-// every lane, without a lane-guarded branch: every active lane takes part in
-// the atomic, the first active one with `n`, the others with 0, and the first
-// active lane's result is broadcast. Work queues and reservations use this.
+// every lane: the first active lane does the atomic and its result is
+// broadcast. The lane test is an opaque value per call (as in lead_lane), so
+// it cannot be correlated with another one, and the broadcast sits right
+// after it. Work queues and reservations use this.
 __device__ inline unsigned long long wave_fetch_add(unsigned long long* p, unsigned long long n) {
-    const uint32_t           lead = uniform_u32(lane_id());
-    const unsigned long long old  = atomicAdd(p, lane_id() == lead ? n : 0ull);
+    uint32_t l = lane_id();
+    asm volatile("" : "+v"(l));
+    unsigned long long old = 0;
+    if (l == uniform_u32(l)) old = atomicAdd(p, n);
     return uniform_u64(old);
 }
 __device__ inline uint32_t wave_fetch_or(uint32_t* p, uint32_t v) {
-    const uint32_t lead = uniform_u32(lane_id());
-    return uniform_u32(atomicOr(p, lane_id() == lead ? v : 0u));
+    uint32_t l = lane_id();
+    asm volatile("" : "+v"(l));
+    uint32_t old = 0;
+    if (l == uniform_u32(l)) old = atomicOr(p, v);
+    return uniform_u32(old);
 }
 
 // Load of data that no kernel writes while the reading kernel runs (column

@@ -124,29 +124,45 @@ Context::~Context() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-void Context::prepare_column(DevColumn& c) {
-    c.row_start = DevBuf(heap_.get(), (c.npages + 1) * sizeof(uint32_t));
-    c.blocks    = DevBuf(heap_.get(), ((c.nrows + 63) / 64 + 1) * sizeof(BlockIdx));
-    if (c.npages == 0) {
-        CGJ_HIP(hipMemsetAsync(c.row_start.get(), 0, sizeof(uint32_t), stream_));
-        if (c.nrows) throw std::runtime_error("column has rows but no pages");
-        return;
+void Context::prepare_column(DevColumn& c) { prepare_columns({&c}); }
+
+void Context::prepare_columns(const std::vector<DevColumn*>& cols) {
+    std::vector<DevColumn*> todo;
+    for (DevColumn* c : cols) {
+        c->row_start = DevBuf(heap_.get(), (c->npages + 1) * sizeof(uint32_t));
+        c->blocks    = DevBuf(heap_.get(), ((c->nrows + 63) / 64 + 1) * sizeof(BlockIdx));
+        if (c->npages == 0) {
+            CGJ_HIP(hipMemsetAsync(c->row_start.get(), 0, sizeof(uint32_t), stream_));
+            if (c->nrows) throw std::runtime_error("column has rows but no pages");
+            continue;
+        }
+        todo.push_back(c);
     }
-    DevBuf scratch(heap_.get(), prepare_scratch_words(c.npages) * sizeof(uint32_t) + 16);
-    DevBuf total(heap_.get(), 2 * sizeof(uint64_t)); // rows, then the two dense words
-    CGJ_HIP(hipMemsetAsync(total.get(), 0, 2 * sizeof(uint64_t), stream_));
-    CGJ_HIP(launch_prepare_column(c.pages.get<uint8_t>(), c.npages, c.type, c.nrows, c.row_start.get<uint32_t>(),
-                                  c.blocks.get<BlockIdx>(), scratch.get<uint32_t>(), total.get<uint64_t>(),
-                                  total.get<uint32_t>() + 2, stream_));
-    uint64_t back[2] = {0, 0};
-    CGJ_HIP(hipMemcpyAsync(back, total.get(), sizeof(back), hipMemcpyDeviceToHost, stream_));
+    if (todo.empty()) return;
+    // per column: rows, then the two dense words
+    DevBuf              total(heap_.get(), todo.size() * 2 * sizeof(uint64_t));
+    std::vector<DevBuf> scratch;
+    CGJ_HIP(hipMemsetAsync(total.get(), 0, todo.size() * 2 * sizeof(uint64_t), stream_));
+    for (size_t i = 0; i < todo.size(); ++i) {
+        DevColumn& c = *todo[i];
+        scratch.emplace_back(heap_.get(), prepare_scratch_words(c.npages) * sizeof(uint32_t) + 16);
+        uint64_t* t = total.get<uint64_t>() + 2 * i;
+        CGJ_HIP(launch_prepare_column(c.pages.get<uint8_t>(), c.npages, c.type, c.nrows, c.row_start.get<uint32_t>(),
+                                      c.blocks.get<BlockIdx>(), scratch.back().get<uint32_t>(), t,
+                                      reinterpret_cast<uint32_t*>(t + 1), stream_));
+    }
+    std::vector<uint64_t> back(todo.size() * 2);
+    CGJ_HIP(hipMemcpyAsync(back.data(), total.get(), back.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
     CGJ_HIP(hipStreamSynchronize(stream_));
-    const uint64_t rows = back[0];
-    if (rows != c.nrows)
-        throw std::runtime_error("column pages hold " + std::to_string(rows) + " rows, table has " +
-                                 std::to_string(c.nrows));
-    const uint32_t uneven = uint32_t(back[1]), n0 = uint32_t(back[1] >> 32);
-    c.dense_rows = (!uneven && n0 >= 64 && c.nrows < (uint64_t(1) << 32)) ? n0 : 0;
+    for (size_t i = 0; i < todo.size(); ++i) {
+        DevColumn&     c    = *todo[i];
+        const uint64_t rows = back[2 * i];
+        if (rows != c.nrows)
+            throw std::runtime_error("column pages hold " + std::to_string(rows) + " rows, table has " +
+                                     std::to_string(c.nrows));
+        const uint32_t uneven = uint32_t(back[2 * i + 1]), n0 = uint32_t(back[2 * i + 1] >> 32);
+        c.dense_rows          = (!uneven && n0 >= 64 && c.nrows < (uint64_t(1) << 32)) ? n0 : 0;
+    }
 }
 
 uint8_t* Context::staging(int b) {
@@ -230,9 +246,12 @@ std::unique_ptr<DevTable> Context::upload(const cgj_host_table& t, const std::ve
                                    hipMemcpyHostToDevice, stream_));
             staging_done(b);
         }
-        prepare_column(c);
         dt->cols.push_back(std::move(c));
     }
+    std::vector<DevColumn*> prep;
+    for (auto& c : dt->cols)
+        if (c.npages || c.nrows == 0) prep.push_back(&c); // placeholders of unscanned columns stay unprepared
+    prepare_columns(prep);
     CGJ_HIP(hipStreamSynchronize(stream_));
     return dt;
 }
@@ -340,6 +359,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "block_prefilter") options.block_prefilter = v != 0;
     else if (name == "scan_insert_max_bytes") options.scan_insert_max_bytes = v > 0 ? uint64_t(v) : 0;
     else if (name == "scan_workgroups") options.scan_workgroups = v >= 16 && v <= 4096 ? uint32_t(v) : 512u;
+    else if (name == "incremental") options.incremental = v != 0;
     else return false;
     return true;
 }
@@ -408,7 +428,7 @@ struct Target {
 class Executor {
 public:
     Executor(Context& ctx, const Query& q, const PlanTree& t, const std::vector<const DevTable*>& inputs)
-        : ctx_(ctx), q_(q), t_(t), inputs_(inputs) {
+        : ctx_(ctx), q_(q), t_(&t), inputs_(inputs) {
         counters_ = DevBuf(&ctx.heap(), sizeof(PipeCounters));
         CGJ_HIP(hipEventCreate(&ev0_));
         CGJ_HIP(hipEventCreate(&ev1_));
@@ -419,17 +439,45 @@ public:
     }
 
     std::unique_ptr<DevResult> run(bool count_only = false);
+    // The reference's incremental execution (engine/query/QueryPlan.cpp:748-782):
+    // plan the remaining inputs (base relations and hash tables built so far),
+    // run the first hash-table build of that plan, re-filter the other
+    // relations' samples through the new table, plan again; until the plan's
+    // next pipeline is the output pipeline. `card` prices vertex sets.
+    using VertexCard = std::function<double(const std::vector<PlanVertex>&, uint64_t)>;
+    std::unique_ptr<DevResult> run_incremental(bool count_only, const VertexCard& card,
+                                               std::shared_ptr<Samples> samples);
+    uint64_t group_rows(int g) const { return groups_[g].ht->tuples; }
 
-    double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0, ms_partition = 0;
-    uint32_t launches = 0, retries = 0, retries_build = 0, probe_lean = 0, probe_waves = 0;
+    double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0, ms_partition = 0, ms_replan = 0;
+    uint32_t launches = 0, retries = 0, retries_build = 0, probe_lean = 0, probe_waves = 0, replans = 0;
     uint64_t build_tuples = 0, probe_rows = 0, build_bytes = 0, out_ref_pages = 0, probe_input_pages = 0;
 
 private:
     double                                  last_ms_ = 0;
     Context&                                ctx_;
     const Query&                            q_;
-    const PlanTree&                         t_;
+    const PlanTree*                         t_;
     const std::vector<const DevTable*>&     inputs_;
+    // Hash tables built by earlier iterations of the incremental loop: the
+    // relations they cover, their key class and the table.
+    struct Group {
+        uint64_t         set;
+        uint32_t         key;
+        const HashTable* ht;
+    };
+    std::vector<Group>       groups_;
+    std::shared_ptr<Samples> samples_;
+    // The class a join uses as hash key: a built table's own key, else the
+    // first class the build side shares with the probe side so far.
+    uint32_t key_for(int build_node, uint64_t cur) const;
+    // The first hash table the plan under `node` needs that is not built yet
+    // (false: every build of node's probe chain is a built table).
+    bool     next_build(int node, int& x, uint32_t& key, uint64_t& probe_rows) const;
+    // Runs the probe chain of `node` into target t (builds first); false when a
+    // build side is empty (the inner join is empty).
+    bool     run_chain(int node, Target& t);
+    void     refilter(uint32_t key, const HashTable* ht);
     std::vector<std::unique_ptr<HashTable>> tables_;
     std::vector<std::unique_ptr<Tuples>>    tuples_;
     std::vector<const DevColumn*>           colreg_;
@@ -469,7 +517,7 @@ private:
     // `probe_rows` = rows expected to probe it (tunes load factor and bloom).
     const HashTable* build(int node, uint32_t key_eq, uint64_t probe_rows);
     uint64_t         probe_rows_of(const TreeNode& join) const {
-        const TreeNode& p = t_.nodes[join.probe];
+        const TreeNode& p = t_->nodes[join.probe];
         return uint64_t(std::min(size_bound(p.set, p.card), 1.8e19));
     }
     // known_grouped: the key column is known to need the grouped layout.
@@ -517,11 +565,12 @@ void Executor::walk(int node, std::vector<int>& joins, int& leaf) const {
     // node = Join(build, probe); the probe side continues the pipeline down to a leaf
     std::vector<int> stack;
     int              n = node;
-    while (!t_.leaf(n)) {
+    while (!t_->leaf(n)) {
         stack.push_back(n);
-        n = t_.nodes[n].probe;
+        n = t_->nodes[n].probe;
     }
-    leaf = t_.nodes[n].rel;
+    leaf = t_->nodes[n].rel;
+    if (leaf < 0) throw std::runtime_error("a built hash table cannot drive a pipeline");
     joins.assign(stack.rbegin(), stack.rend()); // innermost join first
 }
 
@@ -772,8 +821,22 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
     return tables_.back().get();
 }
 
+uint32_t Executor::key_for(int build_node, uint64_t cur) const {
+    const TreeNode& bn = t_->nodes[build_node];
+    const auto      sh = q_.shared(bn.set, cur);
+    if (sh.empty()) throw std::runtime_error("join without a predicate");
+    if (bn.group >= 0) {
+        const uint32_t k = groups_[bn.group].key;
+        if (std::find(sh.begin(), sh.end(), k) == sh.end())
+            throw std::runtime_error("a built hash table is joined off its key");
+        return k;
+    }
+    return sh.front();
+}
+
 const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows) {
-    const TreeNode&       tn = t_.nodes[node];
+    if (t_->nodes[node].group >= 0) return groups_[t_->nodes[node].group].ht;
+    const TreeNode&       tn = t_->nodes[node];
     std::vector<uint32_t> eqs{key_eq};
     for (uint32_t e : q_.needed(tn.set))
         if (e != key_eq) eqs.push_back(e);
@@ -788,13 +851,12 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
     std::vector<ProbeStep> probes;
     uint64_t               cur = 1ull << leaf;
     for (int j : joins) {
-        const TreeNode& jn  = t_.nodes[j];
-        const TreeNode& bn  = t_.nodes[jn.build];
-        auto            sh  = q_.shared(bn.set, cur);
-        if (sh.empty()) throw std::runtime_error("join without a predicate");
-        const HashTable* ht = build(jn.build, sh.front(), probe_rows_of(jn));
+        const TreeNode& jn  = t_->nodes[j];
+        const TreeNode& bn  = t_->nodes[jn.build];
+        const uint32_t  key = key_for(jn.build, cur);
+        const HashTable* ht = build(jn.build, key, probe_rows_of(jn));
         if (!ht) return nullptr;
-        probes.push_back(ProbeStep{ht, sh.front(), bn.set});
+        probes.push_back(ProbeStep{ht, key, bn.set});
         cur |= bn.set;
     }
     // The key column may be known to need the grouped layout: for a plain
@@ -983,6 +1045,39 @@ bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64
 }
 
 void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
+    // Probe order: where the driving relation's samples were probed into the
+    // tables, the most selective table goes first (fewest sample rows found),
+    // among those whose key the pipeline already carries; the reference
+    // orders a scan's filters by selectivity the same way
+    // (engine/op/TableScan.cpp:461-474).
+    if (samples_ && d.rel >= 0 && probes.size() > 1 && samples_->size[d.rel] > 0) {
+        const auto&            pass = samples_->pass[d.rel];
+        std::vector<ProbeStep> ordered;
+        std::vector<bool>      used(probes.size(), false);
+        uint64_t               have = d.set;
+        auto                   sel  = [&](const ProbeStep& p) {
+            auto it = pass.find(p.ht);
+            return it == pass.end() ? 2.0 : double(__builtin_popcountll(it->second)) / samples_->size[d.rel];
+        };
+        while (ordered.size() < probes.size()) {
+            int best = -1;
+            for (size_t i = 0; i < probes.size(); ++i) {
+                if (used[i]) continue;
+                const auto pr = q_.produced(have);
+                if (!pr.count(probes[i].key_eq)) continue;
+                if (best < 0 || sel(probes[i]) < sel(probes[size_t(best)])) best = int(i);
+            }
+            if (best < 0) { // keep the remaining ones in their order (keys come from them)
+                for (size_t i = 0; i < probes.size(); ++i)
+                    if (!used[i]) ordered.push_back(probes[i]);
+                break;
+            }
+            used[size_t(best)] = true;
+            have |= probes[size_t(best)].set;
+            ordered.push_back(probes[size_t(best)]);
+        }
+        probes = std::move(ordered);
+    }
     // Chains longer than the kernel's probe limit are cut: the first part is
     // materialised as tuples that drive the rest.
     while (probes.size() > size_t(kMaxProbes)) {
@@ -1228,6 +1323,28 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         for (uint32_t i = 0; i < a.nscan; ++i) probe_input_pages += column(uint32_t(d.rel), scan_cols[i]).npages;
 }
 
+bool Executor::run_chain(int node, Target& t) {
+    std::vector<int> joins;
+    int              leaf;
+    walk(node, joins, leaf);
+    std::vector<ProbeStep> probes;
+    uint64_t               cur = 1ull << leaf;
+    for (int j : joins) {
+        const TreeNode&  jn  = t_->nodes[j];
+        const TreeNode&  bn  = t_->nodes[jn.build];
+        const uint32_t   key = key_for(jn.build, cur);
+        const HashTable* ht  = build(jn.build, key, probe_rows_of(jn));
+        if (!ht) return false; // empty build side
+        probes.push_back(ProbeStep{ht, key, bn.set});
+        cur |= bn.set;
+    }
+    Driver d;
+    d.rel = leaf;
+    d.set = 1ull << leaf;
+    pipeline(d, probes, t);
+    return true;
+}
+
 std::unique_ptr<DevResult> Executor::run(bool count_only) {
     auto res = std::make_unique<DevResult>();
     for (uint32_t a : q_.out) res->types.push_back(q_.attrs[a].type);
@@ -1236,29 +1353,140 @@ std::unique_ptr<DevResult> Executor::run(bool count_only) {
     register_all_columns();
     Target t;
     t.kind   = count_only ? TARGET_COUNT : TARGET_PAGES;
-    t.est    = size_bound(t_.nodes[t_.root].set, t_.nodes[t_.root].card);
+    t.est    = size_bound(t_->nodes[t_->root].set, t_->nodes[t_->root].card);
     t.result = res.get();
-    std::vector<int> joins;
-    int              leaf;
-    walk(t_.root, joins, leaf);
-    std::vector<ProbeStep> probes;
-    uint64_t               cur = 1ull << leaf;
-    for (int j : joins) {
-        const TreeNode& jn = t_.nodes[j];
-        const TreeNode& bn = t_.nodes[jn.build];
-        auto            sh = q_.shared(bn.set, cur);
-        if (sh.empty()) throw std::runtime_error("join without a predicate");
-        const HashTable* ht = build(jn.build, sh.front(), probe_rows_of(jn));
-        if (!ht) return res; // empty build side
-        probes.push_back(ProbeStep{ht, sh.front(), bn.set});
-        cur |= bn.set;
-    }
-    Driver d;
-    d.rel = leaf;
-    d.set = 1ull << leaf;
-    pipeline(d, probes, t);
+    if (!run_chain(t_->root, t)) return res;
     if (count_only) res->nrows = t.count;
     return res;
+}
+
+bool Executor::next_build(int node, int& x, uint32_t& key, uint64_t& probe_rows) const {
+    std::vector<int> joins;
+    int              leaf;
+    walk(node, joins, leaf);
+    uint64_t cur = 1ull << leaf;
+    for (int j : joins) {
+        const TreeNode& jn = t_->nodes[j];
+        const int       b  = jn.build;
+        const uint32_t  k  = key_for(b, cur);
+        cur |= t_->nodes[b].set;
+        if (t_->nodes[b].group >= 0) continue; // built already
+        if (t_->leaf(b) || !next_build(b, x, key, probe_rows)) {
+            // a plain scan, or a chain whose builds are all built tables
+            x          = b;
+            key        = k;
+            probe_rows = probe_rows_of(jn);
+        }
+        return true;
+    }
+    return false;
+}
+
+void Executor::refilter(uint32_t key, const HashTable* ht) {
+    if (!samples_) return;
+    // relations still planned on their own that produce the key class
+    uint64_t grouped = 0;
+    for (const Group& g : groups_) grouped |= g.set;
+    std::vector<std::pair<uint32_t, const std::vector<uint64_t>*>> todo;
+    size_t                                                         n = 0;
+    for (uint32_t r = 0; r < q_.rels.size(); ++r) {
+        if (((grouped >> r) & 1) || samples_->size[r] == 0) continue;
+        auto it = samples_->keys[r].find(key);
+        if (it == samples_->keys[r].end()) continue;
+        todo.emplace_back(r, &it->second);
+        n += it->second.size();
+    }
+    if (!n) return;
+    uint8_t*  pin   = ctx_.small_pinned(n * sizeof(uint64_t) + n * sizeof(uint32_t));
+    uint64_t* keys  = reinterpret_cast<uint64_t*>(pin);
+    uint32_t* found = reinterpret_cast<uint32_t*>(pin + n * sizeof(uint64_t));
+    size_t    k     = 0;
+    for (auto& [r, v] : todo)
+        for (uint64_t x : *v) keys[k++] = x;
+    DevBuf      dev(&ctx_.heap(), n * (sizeof(uint64_t) + sizeof(uint32_t)));
+    hipStream_t s = ctx_.stream();
+    CGJ_HIP(hipMemcpyAsync(dev.get(), keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    SampleTable st;
+    st.ht    = ht->slots.get<uint64_t>();
+    st.mask  = ht->rmask;
+    st.shift = ht->shift;
+    st.words = ht->grouped ? 2 : ht->words;
+    uint32_t* dfound = reinterpret_cast<uint32_t*>(dev.get<uint8_t>() + n * sizeof(uint64_t));
+    CGJ_HIP(launch_sample_probe(st, dev.get<uint64_t>(), uint32_t(n), dfound, s));
+    CGJ_HIP(hipMemcpyAsync(found, dfound, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    CGJ_HIP(hipStreamSynchronize(s));
+    k = 0;
+    for (auto& [r, v] : todo) {
+        uint64_t m = 0;
+        for (size_t i = 0; i < v->size(); ++i, ++k)
+            if (found[k]) m |= 1ull << i;
+        samples_->matches[r] &= m;
+        samples_->pass[r][ht] = m;
+    }
+}
+
+std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const VertexCard& card,
+                                                     std::shared_ptr<Samples> samples) {
+    samples_ = std::move(samples);
+    auto res = std::make_unique<DevResult>();
+    for (uint32_t a : q_.out) res->types.push_back(q_.attrs[a].type);
+    for (auto& r : q_.rels)
+        if (r.rows == 0) return res; // an empty input empties the inner join
+    register_all_columns();
+    PlanTree tree;
+    for (;;) {
+        // the join graph now: built tables plus the relations none of them covers
+        const auto              t0 = std::chrono::steady_clock::now();
+        std::vector<PlanVertex> vx;
+        uint64_t                grouped = 0;
+        for (size_t g = 0; g < groups_.size(); ++g) {
+            PlanVertex v;
+            v.set   = groups_[g].set;
+            v.group = int(g);
+            v.key   = int32_t(groups_[g].key);
+            vx.push_back(v);
+            grouped |= groups_[g].set;
+        }
+        for (uint32_t r = 0; r < q_.rels.size(); ++r)
+            if (!((grouped >> r) & 1)) {
+                PlanVertex v;
+                v.set = 1ull << r;
+                v.rel = int(r);
+                vx.push_back(v);
+            }
+        tree = plan_joins_vertices(q_, vx, [&](uint64_t vmask) { return card(vx, vmask); });
+        t_   = &tree;
+        ms_replan += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        ++replans;
+        int      x   = -1;
+        uint32_t key = 0;
+        uint64_t prows = 0;
+        if (!next_build(tree.root, x, key, prows)) {
+            // every build of the root's chain is done: the output pipeline
+            Target t;
+            t.kind   = count_only ? TARGET_COUNT : TARGET_PAGES;
+            t.est    = size_bound(tree.nodes[tree.root].set, tree.nodes[tree.root].card);
+            t.result = res.get();
+            if (!run_chain(tree.root, t)) return res;
+            if (count_only) res->nrows = t.count;
+            return res;
+        }
+        const HashTable* ht = build(x, key, prows);
+        if (!ht) return res; // an empty build side empties the inner join
+        const uint64_t set = tree.nodes[x].set;
+        std::vector<Group> keep;
+        for (const Group& g : groups_)
+            if ((g.set & set) != g.set) keep.push_back(g); // tables the new one consumed drop out
+        keep.push_back(Group{set, key, ht});
+        groups_ = std::move(keep);
+        // the samples steer the next join order and the probe order of a
+        // pipeline with several probes: with one relation and one table left
+        // there is nothing for them to decide
+        uint64_t covered = 0;
+        for (const Group& g : groups_) covered |= g.set;
+        const uint32_t left = uint32_t(q_.rels.size()) - uint32_t(__builtin_popcountll(covered));
+        if (left >= 2 || groups_.size() >= 2) refilter(key, ht);
+    }
 }
 
 } // namespace
@@ -1364,7 +1592,11 @@ std::vector<Context::PlannedJoin> Context::plan_tree(const cgj_plan& plan, const
     Query q = import_inputs(plan, inputs);
     check_types(q, inputs);
     EstimateReport           rep;
-    PlanTree                 tree = plan_joins(q, est_->model(*this, q, inputs, rep));
+    std::shared_ptr<Samples> smp;
+    bool                     empty = false;
+    for (const auto& r : q.rels) empty = empty || r.rows == 0;
+    if ((est_->mode == EST_SAMPLE || est_->mode == EST_LEARNED) && !empty) smp = take_samples(q, inputs);
+    PlanTree                 tree = plan_joins(q, est_->model(*this, q, inputs, rep, smp));
     std::vector<PlannedJoin> out;
     std::function<void(int)> rec = [&](int n) {
         if (tree.leaf(n)) return;
@@ -1386,10 +1618,72 @@ PlannedQuery Context::plan_query(const cgj_plan& plan, const std::vector<const D
     pq.q = import_inputs(plan, inputs);
     check_types(pq.q, inputs);
     const auto t0 = std::chrono::steady_clock::now();
-    pq.tree       = plan_joins(pq.q, est.model(*this, pq.q, inputs, pq.rep));
+    pq.mode       = est.mode;
+    bool empty    = false;
+    for (const auto& r : pq.q.rels) empty = empty || r.rows == 0;
+    if ((est.mode == EST_SAMPLE || est.mode == EST_LEARNED) && !empty) pq.samples = take_samples(pq.q, inputs);
+    pq.card       = est.model(*this, pq.q, inputs, pq.rep, pq.samples);
+    pq.tree       = plan_joins(pq.q, pq.card);
     pq.ms_plan =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() - pq.rep.ms_stats;
     return pq;
+}
+
+std::shared_ptr<Samples> Context::take_samples(const Query& q, const std::vector<const DevTable*>& inputs) {
+    auto           sm = std::make_shared<Samples>();
+    const uint32_t n  = uint32_t(q.rels.size());
+    sm->rows.resize(n);
+    sm->size.resize(n);
+    sm->keys.resize(n);
+    sm->matches.resize(n);
+    sm->pass.resize(n);
+    std::vector<SampleCol>                     cols;
+    std::vector<uint64_t>                      rowids;
+    std::vector<std::pair<uint32_t, uint32_t>> where; // (relation, class) of each gathered column
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint64_t rows = q.rels[r].rows;
+        const uint32_t size = uint32_t(std::min<uint64_t>(rows, 64));
+        sm->rows[r]         = rows;
+        sm->size[r]         = size;
+        sm->matches[r]      = size >= 64 ? ~0ull : ((1ull << size) - 1);
+        if (!size) continue;
+        // sample rows drawn with replacement from a stream fixed by the
+        // relation's size and position: the same plan samples the same rows
+        std::vector<uint64_t> ids(size);
+        const uint64_t        seed = sm_mix(rows * 0x9e3779b97f4a7c15ull + r + 1);
+        for (uint32_t i = 0; i < size; ++i) ids[i] = sm_mix(seed + (i + 1) * kGolden) % rows;
+        std::sort(ids.begin(), ids.end());
+        for (uint32_t e : q.rels[r].eqs) {
+            if (!q.is_join_eq(e)) continue;
+            const DevColumn& c = inputs[q.rels[r].table]->cols[q.attrs[q.attr_of(r, e)].col];
+            cols.push_back(SampleCol{c.view(0), uint32_t(rowids.size()), size});
+            rowids.insert(rowids.end(), ids.begin(), ids.end());
+            where.emplace_back(r, e);
+        }
+    }
+    if (!cols.empty()) {
+        const size_t cb = cols.size() * sizeof(SampleCol), rb = rowids.size() * sizeof(uint64_t);
+        uint8_t*     pin = small_pinned(cb + 2 * rb);
+        std::memcpy(pin, cols.data(), cb);
+        std::memcpy(pin + cb, rowids.data(), rb);
+        DevBuf dev(heap_.get(), cb + 2 * rb);
+        CGJ_HIP(hipMemcpyAsync(dev.get(), pin, cb + rb, hipMemcpyHostToDevice, stream_));
+        uint8_t* d = dev.get<uint8_t>();
+        CGJ_HIP(launch_sample_gather(reinterpret_cast<const SampleCol*>(d), uint32_t(cols.size()),
+                                     reinterpret_cast<const uint64_t*>(d + cb), reinterpret_cast<uint64_t*>(d + cb + rb),
+                                     stream_));
+        CGJ_HIP(hipMemcpyAsync(pin + cb + rb, d + cb + rb, rb, hipMemcpyDeviceToHost, stream_));
+        CGJ_HIP(hipStreamSynchronize(stream_));
+        const uint64_t* out = reinterpret_cast<const uint64_t*>(pin + cb + rb);
+        for (size_t k = 0; k < cols.size(); ++k) {
+            const auto [r, e] = where[k];
+            std::vector<uint64_t> v(out + cols[k].first, out + cols[k].first + cols[k].n);
+            for (uint32_t i = 0; i < v.size(); ++i)
+                if (v[i] == kNull) sm->matches[r] &= ~(1ull << i); // NULL never joins
+            sm->keys[r][e] = std::move(v);
+        }
+    }
+    return sm;
 }
 
 std::unique_ptr<DevResult> Context::run_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs,
@@ -1403,9 +1697,46 @@ std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const st
     const Query&          q   = pq.q;
     const EstimateReport& rep = pq.rep;
     Executor              ex(*this, q, pq.tree, inputs);
-    auto                  res = ex.run(count_only);
+    std::unique_ptr<DevResult> res;
+    if (!options.incremental) {
+        res = ex.run(count_only);
+    } else {
+        // Vertex cardinalities for re-planning. The max-of-inputs rule (sampled
+        // or exact relation sizes) takes a built table's exact tuple count as
+        // its cardinality; the other models scale their estimate of the
+        // covered relations by each built table's measured / estimated size.
+        std::vector<uint64_t> group_tuples; // filled through the executor
+        const CardFn          base  = pq.card;
+        const uint32_t        mode  = pq.mode;
+        auto                  smp   = pq.samples;
+        Executor::VertexCard  vcard = [&, base, mode, smp](const std::vector<PlanVertex>& vx, uint64_t vmask) {
+            const bool max_rule = mode == EST_MAX || mode == EST_SAMPLE;
+            double     c = 0, corr = 1;
+            uint64_t   set = 0;
+            for (uint64_t t = vmask; t; t &= t - 1) {
+                const PlanVertex& v = vx[__builtin_ctzll(t)];
+                set |= v.set;
+                double vc;
+                if (v.group >= 0)
+                    vc = double(ex.group_rows(v.group));
+                else if (mode == EST_SAMPLE && smp)
+                    vc = smp->card(uint32_t(v.rel));
+                else
+                    vc = double(q.rels[v.rel].rows);
+                if (max_rule) c = std::max(c, vc);
+                else if (v.group >= 0) corr *= vc / std::max(base(v.set), 1.0);
+            }
+            if (max_rule) return c;
+            if (!(vmask & (vmask - 1))) {
+                const PlanVertex& v = vx[__builtin_ctzll(vmask)];
+                return v.group >= 0 ? double(ex.group_rows(v.group)) : base(v.set);
+            }
+            return base(set) * corr;
+        };
+        res = ex.run_incremental(count_only, vcard, smp);
+    }
     stats                   = cgj_stats{};
-    stats.ms_plan           = pq.ms_plan;
+    stats.ms_plan           = pq.ms_plan + ex.ms_replan;
     stats.ms_stats          = rep.ms_stats;
     stats.est_sets          = rep.sets;
     stats.est_fallbacks     = rep.fallbacks;
@@ -1427,6 +1758,7 @@ std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const st
     stats.pipelines         = ex.launches;
     stats.retries           = ex.retries;
     stats.retries_build     = ex.retries_build;
+    stats.replans           = ex.replans;
     uint64_t in_pages       = 0;
     for (uint32_t ri = 0; ri < q.rels.size(); ++ri) {
         const QRel& r = q.rels[ri];

@@ -191,6 +191,9 @@ struct Options {
     uint64_t scan_insert_max_bytes = uint64_t(8) << 20; // hash tables on a plain scan up to this size are
                                                         // inserted straight from the pages (0: never)
     uint32_t scan_workgroups       = 512; // workgroups of the scan-driven partition (16..4096)
+    // Re-plan after every hash-table build with the tables built so far as
+    // fixed build sides (the reference's incremental loop); 0: plan once.
+    bool     incremental           = true;
 };
 
 // Tunables of one batched join.
@@ -235,6 +238,8 @@ public:
     std::unique_ptr<DevTable> upload(const cgj_host_table& t, const std::vector<bool>* used = nullptr);
     std::unique_ptr<DevTable> generate(uint64_t seed, uint64_t nrows, const cgj_gen_column* cols, uint32_t ncols);
     void prepare_column(DevColumn& c);
+    // Prepares several columns with one host round trip.
+    void prepare_columns(const std::vector<DevColumn*>& cols);
 
     std::unique_ptr<DevResult> execute(const cgj_plan& plan, const std::vector<const DevTable*>& inputs);
     // Runs independent plans concurrently: `workers` host threads, each with
@@ -297,6 +302,9 @@ public:
     PlannedQuery plan_query(const cgj_plan& plan, const std::vector<const DevTable*>& inputs, Estimator& est);
     std::unique_ptr<DevResult> run_planned(const PlannedQuery& pq, const std::vector<const DevTable*>& inputs,
                                            bool count_only);
+    // Up to 64 deterministic sample rows of every relation's join columns,
+    // gathered on the device in one launch; NOT NULL applied.
+    std::shared_ptr<Samples> take_samples(const Query& q, const std::vector<const DevTable*>& inputs);
 
 private:
     void check_types(const Query& q, const std::vector<const DevTable*>& inputs) const;

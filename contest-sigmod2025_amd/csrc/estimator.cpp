@@ -456,9 +456,21 @@ QueryStats Estimator::stats(Context& ctx, const Query& q, const std::vector<cons
     return st;
 }
 
-CardFn Estimator::model(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep) {
+CardFn Estimator::model(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep,
+                        std::shared_ptr<Samples> samples) {
     std::vector<double> base;
     for (const auto& r : q.rels) base.push_back(double(r.rows));
+    // the sampling estimate of a set: the largest sampled cardinality in it,
+    // read when asked (re-filtering after builds updates the samples)
+    auto sampled = [samples, base](uint64_t set) {
+        double c = 0;
+        for (uint64_t t = set; t; t &= t - 1) {
+            const uint32_t r = uint32_t(__builtin_ctzll(t));
+            c                = std::max(c, samples ? samples->card(r) : base[r]);
+        }
+        return c;
+    };
+    if (mode == EST_SAMPLE) return sampled;
     if (mode == EST_MAX || q.rels.size() < 2) return max_card_model(q, base);
     auto st = std::make_shared<QueryStats>(stats(ctx, q, in, rep));
     if (mode == EST_INDEPENDENCE) {
@@ -486,18 +498,19 @@ CardFn Estimator::model(Context& ctx, const Query& q, const std::vector<const De
             const double base_l = std::max(fz.log10_independence(multi[i]), 0.0);
             double       l      = base_l + y[2 * i];
             if (!(y[2 * i + 1] <= max_s) || !std::isfinite(l)) {
-                l = base_l;
+                (*est)[multi[i]] = -1.0; // low confidence: the sampling estimate
                 ++rep.fallbacks;
+                continue;
             }
             (*est)[multi[i]] = std::pow(10.0, std::min(l, 300.0));
         }
     }
     rep.ms_model += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     auto fz = std::make_shared<SetFeaturizer>(q, *st);
-    return [st, est, base, fz](uint64_t set) {
+    return [st, est, base, fz, sampled](uint64_t set) {
         if (!(set & (set - 1))) return base[__builtin_ctzll(set)];
         auto it = est->find(set);
-        if (it != est->end()) return it->second;
+        if (it != est->end()) return it->second >= 0 ? it->second : sampled(set);
         return std::pow(10.0, fz->log10_independence(set)); // more sets than the cap
     };
 }

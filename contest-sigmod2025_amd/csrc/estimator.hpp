@@ -125,27 +125,33 @@ double independence_base(const Query& q, const QueryStats& st, uint64_t relset);
 // more than `limit`.
 bool connected_subsets(const Query& q, uint32_t limit, std::vector<uint64_t>& out);
 
-enum : uint32_t { EST_MAX = 0, EST_INDEPENDENCE = 1, EST_LEARNED = 2 };
+enum : uint32_t { EST_MAX = 0, EST_INDEPENDENCE = 1, EST_LEARNED = 2, EST_SAMPLE = 3 };
 
 // Cardinality model of the join ordering.
-//   EST_MAX          the reference's rule: a join is as large as its largest input
+//   EST_SAMPLE       the reference's planner (the default): the max rule over
+//                    sampled relation cardinalities, re-sampled after every
+//                    hash-table build (Samples)
+//   EST_MAX          the max rule over exact relation sizes
 //   EST_INDEPENDENCE relation sizes x exact pairwise selectivities
 //   EST_LEARNED      the independence estimate corrected by the MLP's log10
 //                    residual for every connected set, batched in one BF16 MFMA
 //                    inference; sets whose predicted spread (sigma, log10
-//                    units) exceeds `max_sigma` keep the independence estimate
+//                    units) exceeds `max_sigma` fall back to the sampling
+//                    estimate (the max rule over sampled cardinalities)
 // Pairwise selectivities come from exact 2-way counts on the device, cached
 // per (table, column) pair for the context's lifetime: tables of a shared
 // pool are joined by many plans. model() may run on several threads at once,
 // each with its own context (counts and inference run on that context).
 class Estimator {
 public:
-    uint32_t             mode      = EST_MAX;
+    uint32_t             mode      = EST_SAMPLE;
     std::shared_ptr<Mlp> mlp;
     float                max_sigma = 0.5f;
     uint32_t             max_sets  = 1u << 15;
 
-    CardFn     model(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep);
+    // `samples`: the query's sampled relations (EST_SAMPLE, and EST_LEARNED's fallback).
+    CardFn     model(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep,
+                     std::shared_ptr<Samples> samples = nullptr);
     QueryStats stats(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep);
     size_t     cached_pairs() const {
         std::lock_guard<std::mutex> lock(pairs_mu_);

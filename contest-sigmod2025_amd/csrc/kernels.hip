@@ -1262,7 +1262,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4
     for (uint64_t m = m_first; m < m_end && !ws.stop; m += m_step) {
         uint64_t r0  = m * a.morsel_rows;
         uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
-        for (uint64_t cb = r0; cb < end && !ws.stop; cb += 64 * 64) { // runs of 64 blocks (a morsel: <= 4096 rows)
+        // runs of 64 blocks (a morsel: <= 4096 rows); past the abort bound a
+        // block ends after one round and the morsel loop stops (checks here
+        // cost registers)
+        for (uint64_t cb = r0; cb < end; cb += 64 * 64) {
         const uint64_t cend = cb + 64 * 64 < end ? cb + 64 * 64 : end;
         const uint32_t nblk = uint32_t((cend - cb + 63) / 64);
         uint64_t       live = nblk >= 64 ? ~0ull : ((1ull << nblk) - 1);
@@ -1271,7 +1274,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4
             pf_seen += nblk;
             pf_live += uint32_t(__popcll(live));
         }
-        for (; live && !ws.stop; live &= live - 1) {
+        for (; live; live &= live - 1) {
             const uint64_t rb = cb + 64 * uint64_t(__builtin_ctzll(live));
             uint64_t row    = rb + lane;
             bool     active = row < end;
@@ -2161,6 +2164,68 @@ hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_wave
         k_pipeline<24><<<dim3(blocks), dim3(256), lds, s>>>(a);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// join sampling: single-row decode of sample rows, and their membership in a
+// built hash table
+// ---------------------------------------------------------------------------
+// Row `row` of a column, decoded by one thread: the block index gives the
+// page, the validity bit and the non-NULL offset (dense columns need none).
+__device__ inline uint64_t decode_one(const ScanCol& c, uint64_t row) {
+    if (c.dense_rows) {
+        const uint32_t d = c.dense_rows;
+        return load_value(c, uint32_t(row / d), uint32_t(row % d), false);
+    }
+    const BlockIdx bi   = c.blocks[row / 64];
+    const uint32_t lane = uint32_t(row % 64);
+    if (bi.complex) return decode_row_slow(c, bi.page, uint32_t(row));
+    if (!((bi.valid >> lane) & 1ull)) return kNull;
+    const uint64_t before = bi.valid & ((1ull << lane) - 1ull);
+    if (lane < bi.split) return load_value(c, bi.page, bi.nn + uint32_t(__popcll(before)), false);
+    const uint64_t second = before & ~((bi.split >= 64 ? ~0ull : (1ull << bi.split)) - 1ull);
+    return load_value(c, bi.page + 1, uint32_t(__popcll(second)), false);
+}
+
+__global__ __launch_bounds__(64) void k_sample_gather(const SampleCol* cols, const uint64_t* rowids, uint64_t* out) {
+    const SampleCol& sc = cols[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < sc.n; i += blockDim.x) out[sc.first + i] = decode_one(sc.col, rowids[sc.first + i]);
+}
+
+hipError_t launch_sample_gather(const SampleCol* cols, uint32_t ncols, const uint64_t* rowids, uint64_t* out,
+                                hipStream_t s) {
+    if (ncols == 0) return hipSuccess;
+    k_sample_gather<<<dim3(ncols), dim3(64), 0, s>>>(cols, rowids, out);
+    return hipGetLastError();
+}
+
+// Linear probing from the key's home slot to a match or an empty slot, at
+// most one region's worth of slots.
+__global__ __launch_bounds__(256) void k_sample_probe(SampleTable t, const uint64_t* keys, uint32_t n,
+                                                      uint32_t* found) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t key = keys[i];
+    uint32_t       hit = 0;
+    if (key != kNull) {
+        uint64_t p = hash_key(key) >> t.shift;
+        for (uint64_t step = 0; step <= t.mask; ++step) {
+            const uint64_t k = t.ht[p * t.words];
+            if (k == key) {
+                hit = 1;
+                break;
+            }
+            if (k == kEmpty) break;
+            p = (p & ~t.mask) | ((p + 1) & t.mask);
+        }
+    }
+    found[i] = hit;
+}
+
+hipError_t launch_sample_probe(SampleTable t, const uint64_t* keys, uint32_t n, uint32_t* found, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_sample_probe<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(t, keys, n, found);
     return hipGetLastError();
 }
 

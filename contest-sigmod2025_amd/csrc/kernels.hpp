@@ -294,6 +294,29 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
 bool       probe1_applicable(const PipeArgs& a, uint32_t slots);
 uint32_t   pipeline_waves_per_block();
 
+// ---- join sampling (the planner's cardinality estimates) ----
+// One scanned column of the sample gather: rows rowids[first .. first + n) of
+// `col` are decoded into out[first ..]. The descriptors and row ids are in
+// device memory.
+struct SampleCol {
+    ScanCol  col;
+    uint32_t first;
+    uint32_t n;
+};
+hipError_t launch_sample_gather(const SampleCol* cols, uint32_t ncols, const uint64_t* rowids, uint64_t* out,
+                                hipStream_t s);
+// A built hash table as the sample re-filter sees it: plain entries of
+// `words` words (key first) or grouped two-word slots; linear probing wraps
+// inside regions of mask + 1 slots.
+struct SampleTable {
+    const uint64_t* ht;
+    uint64_t        mask;
+    uint32_t        shift;
+    uint32_t        words; // 2 for a grouped table
+};
+// found[i] = 1 when keys[i] is a key of the table (NULL never is), else 0.
+hipError_t launch_sample_probe(SampleTable t, const uint64_t* keys, uint32_t n, uint32_t* found, hipStream_t s);
+
 // Synthetic generators.
 struct GenSpec {
     int32_t  type;

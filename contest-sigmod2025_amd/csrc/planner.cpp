@@ -205,8 +205,9 @@ inline uint64_t low_mask(uint32_t v) { return v >= 63 ? ~0ull : ((1ull << (v + 1
 
 struct Dp {
     const Query&                      q;
-    const CardFn&                     card;
-    std::vector<uint64_t>             adj;
+    const std::vector<PlanVertex>&    vx;
+    const CardFn&                     card; // over vertex masks
+    std::vector<uint64_t>             adj;  // vertex adjacency
     PlanTree                          tree;
     std::unordered_map<uint64_t, int> best;
     std::unordered_map<uint64_t, double> cards;
@@ -214,8 +215,20 @@ struct Dp {
     size_t                            budget = 4u << 20;
     bool                              overflow = false;
 
-    Dp(const Query& q_, const CardFn& c) : q(q_), card(c), adj(q_.adjacency()) {}
+    Dp(const Query& q_, const std::vector<PlanVertex>& v, const CardFn& c) : q(q_), vx(v), card(c), adj(v.size(), 0) {
+        for (size_t i = 0; i < vx.size(); ++i)
+            for (size_t j = i + 1; j < vx.size(); ++j)
+                if (!q.shared(vx[i].set, vx[j].set).empty()) {
+                    adj[i] |= 1ull << j;
+                    adj[j] |= 1ull << i;
+                }
+    }
 
+    uint64_t base(uint64_t vmask) const {
+        uint64_t s = 0;
+        for (uint64_t t = vmask; t; t &= t - 1) s |= vx[__builtin_ctzll(t)].set;
+        return s;
+    }
     uint64_t neigh(uint64_t s) const {
         uint64_t n = 0;
         for (uint64_t t = s; t; t &= t - 1) n |= adj[__builtin_ctzll(t)];
@@ -227,6 +240,16 @@ struct Dp {
         double c = std::max(card(s), 0.0);
         cards.emplace(s, c);
         return c;
+    }
+    // A built table (a single vertex with a key) may only be a build side,
+    // joined on its key class.
+    bool can_join(uint64_t b, uint64_t p) const {
+        if (!(p & (p - 1)) && vx[__builtin_ctzll(p)].key >= 0) return false;
+        if (!(b & (b - 1)) && vx[__builtin_ctzll(b)].key >= 0) {
+            const auto sh = q.shared(base(b), base(p));
+            return std::find(sh.begin(), sh.end(), uint32_t(vx[__builtin_ctzll(b)].key)) != sh.end();
+        }
+        return true;
     }
     void emit_pair(uint64_t a, uint64_t b) {
         if (pairs.size() >= budget) {
@@ -263,6 +286,7 @@ struct Dp {
     }
 
     void consider(uint64_t b, uint64_t p) {
+        if (!can_join(b, p)) return;
         auto ib = best.find(b), ip = best.find(p);
         if (ib == best.end() || ip == best.end()) return;
         uint64_t        t  = b | p;
@@ -274,7 +298,7 @@ struct Dp {
         TreeNode n;
         n.build = ib->second;
         n.probe = ip->second;
-        n.set   = t;
+        n.set   = nb.set | np.set;
         n.card  = card_of(t);
         n.cost  = c;
         tree.nodes.push_back(n);
@@ -282,14 +306,15 @@ struct Dp {
     }
 
     void leaves() {
-        for (uint32_t r = 0; r < q.rels.size(); ++r) {
+        for (uint32_t v = 0; v < vx.size(); ++v) {
             TreeNode n;
-            n.rel  = int(r);
-            n.set  = 1ull << r;
-            n.card = card_of(n.set);
-            n.cost = 0;
+            n.rel   = vx[v].rel;
+            n.group = vx[v].group;
+            n.set   = vx[v].set;
+            n.card  = card_of(1ull << v);
+            n.cost  = 0;
             tree.nodes.push_back(n);
-            best[n.set] = int(tree.nodes.size() - 1);
+            best[1ull << v] = int(tree.nodes.size() - 1);
         }
     }
 
@@ -297,13 +322,14 @@ struct Dp {
     // with the smallest joint cardinality (the fallback for very large graphs).
     void greedy() {
         std::vector<uint64_t> comps;
-        for (uint32_t r = 0; r < q.rels.size(); ++r) comps.push_back(1ull << r);
+        for (uint32_t v = 0; v < vx.size(); ++v) comps.push_back(1ull << v);
         while (comps.size() > 1) {
             double bc = 1e300;
             size_t bi = 0, bj = 0;
             for (size_t i = 0; i < comps.size(); ++i)
                 for (size_t j = i + 1; j < comps.size(); ++j) {
                     if (!(neigh(comps[i]) & comps[j])) continue;
+                    if (!can_join(comps[i], comps[j]) && !can_join(comps[j], comps[i])) continue;
                     double c = card_of(comps[i] | comps[j]);
                     if (c < bc) {
                         bc = c;
@@ -312,8 +338,9 @@ struct Dp {
                     }
                 }
             if (bc == 1e300) throw std::runtime_error("join graph is not connected");
-            uint64_t a = comps[bi], b = comps[bj];
-            if (card_of(a) <= card_of(b))
+            uint64_t   a = comps[bi], b = comps[bj];
+            const bool ab = can_join(a, b), ba = can_join(b, a);
+            if (ab && (!ba || card_of(a) <= card_of(b)))
                 consider(a, b);
             else
                 consider(b, a);
@@ -324,7 +351,7 @@ struct Dp {
 
     void run() {
         leaves();
-        uint32_t n = uint32_t(q.rels.size());
+        uint32_t n = uint32_t(vx.size());
         for (int v = int(n) - 1; v >= 0; --v) {
             uint64_t s = 1ull << v;
             emit_csg(s);
@@ -346,14 +373,25 @@ struct Dp {
 
 } // namespace
 
-PlanTree plan_joins(const Query& q, const CardFn& card) {
-    if (q.rels.empty()) throw std::runtime_error("plan has no scans");
-    Dp dp(q, card);
+PlanTree plan_joins_vertices(const Query& q, const std::vector<PlanVertex>& vertices, const CardFn& card) {
+    if (vertices.empty()) throw std::runtime_error("plan has no scans");
+    if (vertices.size() > 64) throw std::runtime_error("more than 64 join-graph vertices");
+    Dp dp(q, vertices, card);
     dp.run();
-    auto it = dp.best.find(q.all_rels());
+    const uint64_t all = vertices.size() >= 64 ? ~0ull : ((1ull << vertices.size()) - 1);
+    auto           it  = dp.best.find(all);
     if (it == dp.best.end()) throw std::runtime_error("join graph is not connected");
     dp.tree.root = it->second;
     return std::move(dp.tree);
+}
+
+PlanTree plan_joins(const Query& q, const CardFn& card) {
+    std::vector<PlanVertex> v(q.rels.size());
+    for (uint32_t r = 0; r < q.rels.size(); ++r) {
+        v[r].set = 1ull << r;
+        v[r].rel = int(r);
+    }
+    return plan_joins_vertices(q, v, card); // vertex masks are relation masks here
 }
 
 JoinTuning autotune_join(uint64_t build_rows, uint64_t probe_rows, uint32_t key_bytes) {

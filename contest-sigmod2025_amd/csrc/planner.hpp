@@ -3,6 +3,8 @@
 
 #include <cstdint>
 #include <functional>
+#include <map>
+#include <memory>
 #include <set>
 #include <vector>
 
@@ -60,17 +62,30 @@ Query restrict_query(const Query& q, uint64_t relset);
 // Join tree: leaves are relations; an inner node joins `build` (hashed) with
 // `probe` (streamed through the pipeline).
 struct TreeNode {
-    int      rel   = -1;
+    int      rel   = -1; // leaf: a base relation
+    int      group = -1; // leaf: a hash table built earlier (index into the caller's vertex list)
     int      build = -1;
     int      probe = -1;
-    uint64_t set   = 0;
+    uint64_t set   = 0;  // base relations covered
     double   card  = 0;
     double   cost  = 0;
 };
 struct PlanTree {
     std::vector<TreeNode> nodes;
     int                   root = -1;
-    bool                  leaf(int n) const { return nodes[n].rel >= 0; }
+    bool                  leaf(int n) const { return nodes[n].rel >= 0 || nodes[n].group >= 0; }
+};
+
+// A vertex of the join graph being planned: a base relation, or a hash table
+// built by an earlier pipeline over a set of relations. A built table may only
+// be the build side of a join, on its key class (reference
+// engine/query/QueryGraph.cpp canJoin: a hash table cannot be probed as a
+// stream, and it is keyed on one class).
+struct PlanVertex {
+    uint64_t set   = 0;  // base relations covered
+    int      rel   = -1; // base relation, or -1
+    int      group = -1; // built table id, or -1
+    int32_t  key   = -1; // built table: its key class
 };
 
 // Cardinality of a connected relation set.
@@ -85,6 +100,9 @@ CardFn max_card_model(const Query& q, const std::vector<double>& base_card);
 // card(T) + 2 card(build): every hash-table tuple is written and read once
 // more than a streamed one.
 PlanTree plan_joins(const Query& q, const CardFn& card);
+// The same DP over arbitrary vertices; `card` receives vertex masks (bit i =
+// vertices[i]). Tree node sets are base-relation masks.
+PlanTree plan_joins_vertices(const Query& q, const std::vector<PlanVertex>& vertices, const CardFn& card);
 
 // What the cardinality model did while a query was planned.
 struct EstimateReport {
@@ -95,12 +113,37 @@ struct EstimateReport {
     double   ms_model   = 0; // featurisation + inference
 };
 
+// Row samples of each relation's join columns, the reference's sampling
+// estimate (engine/query/QueryPlan.cpp:719-746 computeSamples, :66-72
+// recomputeCardinality): up to 64 rows per relation; a relation's estimated
+// cardinality is rows x (sample rows passing every restriction so far) /
+// (sample rows), or rows / (2 x sample rows) when none passes. The
+// restrictions are NOT NULL on every join column, then membership in each
+// hash table built on one of the relation's join classes.
+struct Samples {
+    std::vector<uint64_t>                                  rows;    // relation rows
+    std::vector<uint32_t>                                  size;    // sample rows (<= 64)
+    std::vector<std::map<uint32_t, std::vector<uint64_t>>> keys;    // join class -> sample values (slots)
+    std::vector<uint64_t>                                  matches; // bit i: sample row i passes so far
+    // per relation: the sample rows found in each hash table probed by the
+    // re-filter (the table's selectivity on this relation, for probe order)
+    std::vector<std::map<const void*, uint64_t>>           pass;
+    double card(uint32_t r) const {
+        if (size[r] == 0) return 0;
+        const int m = __builtin_popcountll(matches[r]);
+        return double(rows[r]) * (m ? double(m) / size[r] : 1.0 / (2.0 * size[r]));
+    }
+};
+
 // A query and the join tree chosen for it.
 struct PlannedQuery {
-    Query          q;
-    PlanTree       tree;
-    EstimateReport rep;
-    double         ms_plan = 0; // estimation + join ordering, statistics counts excluded
+    Query                    q;
+    PlanTree                 tree;
+    EstimateReport           rep;
+    double                   ms_plan = 0; // estimation + join ordering, statistics counts excluded
+    uint32_t                 mode    = 0; // estimator mode that planned it
+    CardFn                   card;        // that model's cardinality of base-relation sets
+    std::shared_ptr<Samples> samples;     // sampled relations (modes that use them)
 };
 
 // Per-join tuning chosen from (build rows, probe rows, key width).

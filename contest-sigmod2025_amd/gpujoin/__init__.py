@@ -99,6 +99,8 @@ class Stats(C.Structure):
         ("retries_build", C.c_uint32),
         ("ms_host_upload", C.c_double),
         ("ms_host_download", C.c_double),
+        ("replans", C.c_uint32),
+        ("pad0", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -163,6 +165,8 @@ SIGNATURES = {
     "cgj_dtable_generate": (C.c_void_p, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(GenColumn), C.c_uint32]),
     "cgj_host_generate": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(GenColumn),
                                     C.POINTER(ResultColumn)]),
+    "cgj_sample_cardinalities": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
+                                           C.POINTER(C.c_double), C.c_uint32, C.POINTER(C.c_uint32)]),
     "cgj_count_device_noalloc": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
                                            C.POINTER(C.c_uint64)]),
     "cgj_count_device": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
@@ -201,7 +205,7 @@ SIGNATURES = {
                                  C.POINTER(C.c_uint32)]),
 }
 
-EST_MAX, EST_INDEPENDENCE, EST_LEARNED = 0, 1, 2
+EST_MAX, EST_INDEPENDENCE, EST_LEARNED, EST_SAMPLE = 0, 1, 2, 3
 
 _lib = None
 
@@ -727,8 +731,10 @@ class Context:
 
     # -- cardinality estimation for join ordering --------------------------
     def set_estimator(self, mode: int, mlp: Optional["Mlp"] = None, max_sigma: float = 0.5):
-        """EST_MAX (the reference's max-of-inputs rule, default),
-        EST_INDEPENDENCE, or EST_LEARNED with a network of num_features() inputs."""
+        """EST_SAMPLE (the reference's planner: the max-of-inputs rule over
+        sampled cardinalities, the default), EST_MAX (the rule over exact
+        sizes), EST_INDEPENDENCE, or EST_LEARNED with a network of
+        num_features() inputs (low-confidence sets take the sampling estimate)."""
         h = mlp.handle if mlp is not None else None
         if lib().cgj_set_estimator(self.handle, mode, h, max_sigma) != 0:
             raise RuntimeError(self.error())
@@ -741,6 +747,18 @@ class Context:
         m = _Marshal(plan, [])
         arr = (C.c_void_p * max(len(tables), 1))(*[t.handle for t in tables])
         return m, arr
+
+    def sample_cardinalities(self, plan: Plan, tables: Sequence[DeviceTable]):
+        """The sampling estimate of each relation of the plan (numbered as for
+        plan_features); a set's estimate is the largest of its relations'."""
+        import numpy as np
+        m, arr = self._plan_args(plan, tables)
+        cards = np.zeros(64, np.float64)
+        n = C.c_uint32()
+        if lib().cgj_sample_cardinalities(self.handle, C.byref(m.plan), arr, len(tables),
+                                          cards.ctypes.data_as(C.POINTER(C.c_double)), 64, C.byref(n)) != 0:
+            raise RuntimeError(self.error())
+        return cards[: n.value]
 
     def plan_features(self, plan: Plan, tables: Sequence[DeviceTable]):
         """(sets, features) of every connected relation set of two or more

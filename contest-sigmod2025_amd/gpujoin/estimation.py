@@ -18,7 +18,7 @@ from typing import Dict, List, Sequence, Tuple
 
 import numpy as np
 
-from . import EST_INDEPENDENCE, EST_LEARNED, EST_MAX, Context, DeviceTable, Mlp
+from . import EST_INDEPENDENCE, EST_LEARNED, EST_MAX, EST_SAMPLE, Context, DeviceTable, Mlp
 
 
 @dataclass
@@ -27,6 +27,7 @@ class SetSamples:
     sets: np.ndarray       # uint64 relation sets
     features: np.ndarray   # n x num_features float32
     rows: np.ndarray       # exact cardinalities (float64)
+    sampled: np.ndarray = None  # the sampling estimate of each set (largest sampled relation)
 
 
 def plan_inputs(pool_tables: Sequence[DeviceTable], pp) -> List[DeviceTable]:
@@ -47,6 +48,7 @@ def collect(ctx: Context, planset, pool_tables: Sequence[DeviceTable], plan_ids:
         pp = planset.plans[pid]
         tabs = plan_inputs(pool_tables, pp)
         sets, feats = ctx.plan_features(pp.plan, tabs)
+        rel_cards = ctx.sample_cardinalities(pp.plan, tabs)
         full = (1 << len(pp.tables)) - 1
         idx = list(range(len(sets)))
         if len(idx) > max_sets_per_plan:
@@ -64,15 +66,17 @@ def collect(ctx: Context, planset, pool_tables: Sequence[DeviceTable], plan_ids:
                 if counts is not None:
                     counts[key] = c
             rows.append(float(c))
-        out.append(SetSamples(pid, sets[idx], feats[idx], np.array(rows)))
+        sampled = np.array([max(rel_cards[r] for r in range(len(rel_cards)) if (int(s) >> r) & 1)
+                            for s in sets[idx]])
+        out.append(SetSamples(pid, sets[idx], feats[idx], np.array(rows), sampled))
     return out
 
 
 def save(path: str, samples: Sequence[SetSamples]):
-    """Writes samples as arrays (plan, set, features, rows) to an .npz file."""
+    """Writes samples as arrays (plan, set, features, rows, sampled) to an .npz file."""
     np.savez(path, plan=np.concatenate([np.full(len(s.rows), s.plan) for s in samples]),
              sets=np.concatenate([s.sets for s in samples]), features=np.concatenate([s.features for s in samples]),
-             rows=np.concatenate([s.rows for s in samples]))
+             rows=np.concatenate([s.rows for s in samples]), sampled=np.concatenate([s.sampled for s in samples]))
 
 
 def base_log10(features: np.ndarray) -> np.ndarray:
@@ -110,12 +114,16 @@ def estimates(samples: Sequence[SetSamples], planset, mode: int, mlp: Mlp = None
               max_sigma: float = 0.5) -> np.ndarray:
     """Estimated cardinalities of the samples' sets under one model, computed
     on the host from the features (they hold the same inputs the engine uses):
+    EST_SAMPLE = largest sampled relation (gathered on the device by collect),
     EST_MAX = largest relation, EST_INDEPENDENCE = feature 10 (x 8 = log10 of
     the independence estimate), EST_LEARNED = that estimate corrected by the
-    network's residual (BF16 inference), kept uncorrected where the network's
-    sigma exceeds `max_sigma`."""
+    network's residual (BF16 inference), the sampling estimate where the
+    network's sigma exceeds `max_sigma`."""
     x = np.concatenate([s.features for s in samples])
     indep = 10.0 ** (x[:, 10].astype(np.float64) * 8.0)
+    sampled = np.concatenate([s.sampled for s in samples])
+    if mode == EST_SAMPLE:
+        return sampled
     if mode == EST_MAX:
         return 10.0 ** (x[:, 11].astype(np.float64) * 8.0)
     if mode == EST_INDEPENDENCE:
@@ -124,7 +132,7 @@ def estimates(samples: Sequence[SetSamples], planset, mode: int, mlp: Mlp = None
     mu, s = out[:, 0].astype(np.float64), out[:, 1].astype(np.float64)
     base = base_log10(x)
     learned = 10.0 ** np.minimum(base + mu, 300.0)
-    return np.where(np.exp(s) <= max_sigma, learned, 10.0 ** base)
+    return np.where(np.exp(s) <= max_sigma, learned, sampled)
 
 
 def true_cost(ctx: Context, planset, pool_tables, pid: int, counts: Dict) -> float:
@@ -160,11 +168,12 @@ def evaluate(ctx: Context, planset, pool_tables, samples: Sequence[SetSamples], 
              max_sigma: float = 0.5) -> Evaluation:
     ev = Evaluation()
     true = np.concatenate([s.rows for s in samples])
-    for name, mode in (("max", EST_MAX), ("independence", EST_INDEPENDENCE), ("learned", EST_LEARNED)):
+    for name, mode in (("sample", EST_SAMPLE), ("max", EST_MAX), ("independence", EST_INDEPENDENCE),
+                       ("learned", EST_LEARNED)):
         q = qerror(estimates(samples, planset, mode, mlp, max_sigma), true)
         ev.qerror_median[name] = float(np.median(q))
         ev.qerror_p90[name] = float(np.percentile(q, 90))
         ctx.set_estimator(mode, mlp if mode == EST_LEARNED else None, max_sigma)
         ev.cost[name] = sum(true_cost(ctx, planset, pool_tables, s.plan, counts) for s in samples)
-    ctx.set_estimator(EST_MAX)
+    ctx.set_estimator(EST_SAMPLE)
     return ev

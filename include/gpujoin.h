@@ -121,12 +121,15 @@ typedef struct {
                                    statistics counts excluded */
     double   ms_stats;          /* time of the 2-way counts run for the statistics catalog */
     uint32_t est_sets;          /* connected relation sets estimated by the network */
-    uint32_t est_fallbacks;     /* ... of them, by the independence rule (low confidence) */
+    uint32_t est_fallbacks;     /* ... of them, by the sampling estimate (low confidence) */
     uint32_t est_stat_joins;    /* 2-way counts run (pairs not yet in the catalog) */
     uint32_t retries_build;     /* ... of them, builds that changed layout (walk cap, region overflow) */
     double   ms_host_upload;    /* cgj_execute only: wall time of copying the scanned host pages in */
     double   ms_host_download;  /* cgj_execute only: wall time of copying the result pages out into
                                    one operator-new block per page */
+    uint32_t replans;           /* join orderings computed (one per hash-table build of the
+                                   incremental loop, plus the output pipeline's) */
+    uint32_t pad0;
 } cgj_stats;
 
 /* Execution options of a context (0 = success, -1 = unknown name):
@@ -150,7 +153,10 @@ typedef struct {
  *   "block_prefilter": a pipeline over pages tests its early bloom filters on eight 64-row
  *       blocks at a time before the full per-block work (1, default) or not (0);
  *   "scan_insert_max_bytes": a hash table on a plain scan of at most this many bytes is
- *       inserted straight from the pages by one kernel (default 8 MiB, 0 = never). */
+ *       inserted straight from the pages by one kernel (default 8 MiB, 0 = never);
+ *   "incremental": re-plan after every hash-table build, with the tables built so far as
+ *       fixed build sides and the samples re-filtered through them (1, default), or plan
+ *       once (0). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
@@ -308,19 +314,27 @@ int      cgj_mlp_predict_f32(cgj_context* ctx, cgj_mlp* m, const float* x, uint3
 void     cgj_mlp_free(cgj_context* ctx, cgj_mlp* m);
 
 /* ---- cardinality estimation for join ordering ----
- * The reference orders joins with DPccp under the rule "a join is as large as
- * its largest input" (engine/query/QueryGraph.cpp:53-68, fed by the sampled
- * base cardinalities of engine/query/QueryPlan.cpp:66-159, 719-746): mode
- * CGJ_EST_MAX, the default. CGJ_EST_INDEPENDENCE multiplies relation sizes by
- * exact pairwise join selectivities; CGJ_EST_LEARNED adds the network's output
- * mu (a log10 correction; the network must have cgj_num_features() inputs) to
- * the independence estimate (log10, at least 0) of every connected relation
- * set, all sets in one batched inference, and keeps the independence estimate
- * where the predicted spread sigma exceeds `max_sigma` (log10 units). Pairwise selectivities come from exact 2-way counts on the device,
- * cached per (table, column) pair for the context's lifetime. The estimator
- * only changes the join order: results are identical in every mode. The
- * context keeps a reference to the network, so its handle may be freed. */
-enum { CGJ_EST_MAX = 0, CGJ_EST_INDEPENDENCE = 1, CGJ_EST_LEARNED = 2 };
+ * The reference plans with DPccp under the rule "a join is as large as its
+ * largest input" (engine/query/QueryGraph.cpp:53-68) over sampled relation
+ * cardinalities (engine/query/QueryPlan.cpp:66-72, 719-746): up to 64 sample
+ * rows per relation, cardinality = rows x passing sample rows / sample rows
+ * (rows / (2 x sample rows) when none passes), re-filtered through every hash
+ * table it builds, with a re-plan after each build (QueryPlan.cpp:670-676,
+ * 748-782). That is CGJ_EST_SAMPLE, the default; samples are gathered on the
+ * device and re-filtered by probing each new hash table. CGJ_EST_MAX applies
+ * the max rule to exact relation sizes. CGJ_EST_INDEPENDENCE multiplies
+ * relation sizes by exact pairwise join selectivities; CGJ_EST_LEARNED adds
+ * the network's output mu (a log10 correction; the network must have
+ * cgj_num_features() inputs) to the independence estimate (log10, at least 0)
+ * of every connected relation set, all sets in one batched inference, and uses
+ * the sampling estimate where the predicted spread sigma exceeds `max_sigma`
+ * (log10 units). Pairwise selectivities come from exact 2-way counts on the
+ * device, cached per (table, column) pair for the context's lifetime. Every
+ * mode re-plans after each hash-table build (option "incremental"), pricing a
+ * built table at its exact size. The estimator only changes the join order:
+ * results are identical in every mode. The context keeps a reference to the
+ * network, so its handle may be freed. */
+enum { CGJ_EST_MAX = 0, CGJ_EST_INDEPENDENCE = 1, CGJ_EST_LEARNED = 2, CGJ_EST_SAMPLE = 3 };
 uint32_t cgj_num_features(void);
 int      cgj_set_estimator(cgj_context* ctx, uint32_t mode, cgj_mlp* mlp, float max_sigma);
 /* Relations of a plan are numbered in its depth-first scan order (left
@@ -329,6 +343,14 @@ int      cgj_set_estimator(cgj_context* ctx, uint32_t mode, cgj_mlp* mlp, float 
  * returns the number of such sets, or -1 on error. */
 int64_t  cgj_plan_features(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
                            uint64_t* sets, float* features, uint64_t cap);
+/* The sampling estimate of each relation of the plan (numbered as for
+ * cgj_plan_features): rows x the fraction of its (up to 64) sample rows whose
+ * join columns are all non-NULL, rows / (2 x sample rows) when none is
+ * (reference engine/query/QueryPlan.cpp:66-72, 719-746). The estimate of a
+ * set is the largest of its relations' (QueryGraph.cpp:53-68). `cards` holds
+ * one double per relation; *num_rels receives their number. */
+int      cgj_sample_cardinalities(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs,
+                                  uint32_t num_inputs, double* cards, uint32_t cap, uint32_t* num_rels);
 /* Exact result rows of the join of a connected relation set of the plan. */
 int      cgj_count_subset(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
                           uint64_t relset, uint64_t* rows);

@@ -124,7 +124,7 @@ def test_statistics_catalog_counts_pairs_once(ctx, small_pool):
         assert second["est_stat_joins"] == 0  # every pair already in the catalog
         assert first["est_stat_joins"] <= len(pp.edges)
     finally:
-        ctx.set_estimator(gj.EST_MAX)
+        ctx.set_estimator(gj.EST_SAMPLE)
 
 
 def test_results_identical_under_every_estimator(ctx, small_pool):
@@ -135,21 +135,26 @@ def test_results_identical_under_every_estimator(ctx, small_pool):
         ins = est.plan_inputs(tabs, pp)
         types = [t for _, t in pp.plan.nodes[pp.plan.root].output_attrs]
         rows = []
-        for mode in (gj.EST_MAX, gj.EST_INDEPENDENCE, gj.EST_LEARNED):
+        for mode in (gj.EST_SAMPLE, gj.EST_MAX, gj.EST_INDEPENDENCE, gj.EST_LEARNED):
             ctx.set_estimator(mode, mlp if mode == gj.EST_LEARNED else None, max_sigma=100.0)
-            say(f"plan {pid} estimator {mode}")
-            rows.append(check_table(ctx.execute_device(pp.plan, ins).download(), types))
-        ctx.set_estimator(gj.EST_MAX)
-        assert rows[0] == rows[1] == rows[2]
+            for inc in (1, 0):
+                ctx.set_option("incremental", inc)
+                say(f"plan {pid} estimator {mode} incremental {inc}")
+                rows.append(check_table(ctx.execute_device(pp.plan, ins).download(), types))
+        ctx.set_option("incremental", 1)
+        ctx.set_estimator(gj.EST_SAMPLE)
+        assert all(r == rows[0] for r in rows)
         plan = pp.plan
         plan.inputs = _host_inputs(ps, pp)
         assert rows[0] == check_table(oracle.execute(plan), types)
 
 
-def test_learned_estimator_beats_max_rule_on_held_out_plans(ctx):
+def test_learned_estimator_on_held_out_plans(ctx):
     """Train on 90 plans, evaluate on 40 others of the same pool: median
-    q-error of the learned estimate at most 4, and join orders priced with
-    exact cardinalities cheaper than under the max-of-inputs rule."""
+    q-error of the learned estimate at most 4 and below the sampling
+    estimate's (the reference's planner), join orders priced with exact
+    cardinalities cheaper than the sampling planner's, and the learned
+    estimate compared with the independence rule it corrects."""
     ps = wl.c5(scale=0.005, nplans=130)
     tabs = [ctx.generate(ps.seed, t.rows, t.columns) for t in ps.pool]
     counts = {}
@@ -161,5 +166,7 @@ def test_learned_estimator_beats_max_rule_on_held_out_plans(ctx):
     ev = est.evaluate(ctx, ps, tabs, test_s, mlp, counts)
     say(f"q-error median {ev.qerror_median} p90 {ev.qerror_p90} cost {ev.cost} loss {loss}")
     assert ev.qerror_median["learned"] <= 4.0
-    assert ev.qerror_median["learned"] < ev.qerror_median["max"]
-    assert ev.cost["learned"] < ev.cost["max"]
+    assert ev.qerror_median["learned"] < ev.qerror_median["sample"]
+    assert ev.cost["learned"] < ev.cost["sample"]
+    # the network corrects the independence rule; on these plans it may match it
+    assert ev.qerror_p90["learned"] <= 1.5 * ev.qerror_p90["independence"]

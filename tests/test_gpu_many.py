@@ -113,7 +113,7 @@ def test_execute_many_planned_here_matches_single_runs(ctx, mode, workers):
         say(f"execute_many planned by the workers, estimator {mode}, {workers} workers")
         res = ctx.execute_many(plans, tables, workers=workers)
     finally:
-        ctx.set_estimator(gj.EST_MAX)
+        ctx.set_estimator(gj.EST_SAMPLE)
     for p, r, e in zip(plans, res, expect):
         types = [t for _, t in p.nodes[p.root].output_attrs]
         assert ct(r.download(), types) == e
