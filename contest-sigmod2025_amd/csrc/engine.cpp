@@ -118,6 +118,7 @@ Context::~Context() {
         if (pin_[b]) (void)hipHostFree(pin_[b]);
     }
     if (spin_) (void)hipHostFree(spin_);
+    if (sample_pin_) (void)hipHostFree(sample_pin_);
     workers_.clear();
     est_.reset(); // its network's buffers go back to the heap
     heap_->trim();
@@ -188,6 +189,11 @@ uint8_t* Context::small_pinned(size_t bytes) {
         spin_bytes_ = want;
     }
     return static_cast<uint8_t*>(spin_);
+}
+
+uint8_t* Context::sample_pinned() {
+    if (!sample_pin_) CGJ_HIP(hipHostMalloc(&sample_pin_, kSamplePinnedBytes, hipHostMallocDefault));
+    return static_cast<uint8_t*>(sample_pin_);
 }
 
 namespace {
@@ -477,7 +483,23 @@ private:
     // Runs the probe chain of `node` into target t (builds first); false when a
     // build side is empty (the inner join is empty).
     bool     run_chain(int node, Target& t);
-    void     refilter(uint32_t key, const HashTable* ht);
+    // Sample re-filter through a table the incremental loop builds: the
+    // loop arms it before the build, the build queues the probe of the
+    // samples ahead of its last synchronisation (queue_refilter), and the
+    // loop applies the hits afterwards (finish_refilter), so the re-filter
+    // adds no round trip of its own.
+    struct PendingRefilter {
+        bool                                       armed   = false;
+        uint32_t                                   key     = 0;
+        uint64_t                                   grouped = 0;       // relations covered by tables after the build
+        const HashTable*                           table   = nullptr; // table the queued probe reads (null: none)
+        std::vector<std::pair<uint32_t, uint32_t>> todo;              // relation, sample keys queued for it
+        DevBuf                                     dev;
+    };
+    PendingRefilter rf_;
+    void            queue_refilter(const HashTable* ht);
+    void            drop_refilter() { rf_.table = nullptr; }
+    void            finish_refilter(const HashTable* ht);
     std::vector<std::unique_ptr<HashTable>> tables_;
     std::vector<std::unique_ptr<Tuples>>    tuples_;
     std::vector<const DevColumn*>           colreg_;
@@ -809,6 +831,7 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
         ht->grouped = true;
         ht->unique  = false;
     }
+    queue_refilter(ht.get());
     CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
     CGJ_HIP(hipEventSynchronize(ev1_));
     float fms = 0;
@@ -946,6 +969,7 @@ bool Executor::scan_insert(uint32_t rel, const std::vector<uint32_t>& eqs, uint6
                                ctx_.options.group_walk, count, s));
     uint32_t back[6];
     CGJ_HIP(hipMemcpyAsync(back, fc.get(), sizeof(back), hipMemcpyDeviceToHost, s));
+    queue_refilter(ht.get());
     CGJ_HIP(hipEventRecord(ev1_, s));
     CGJ_HIP(hipEventSynchronize(ev1_));
     float fms = 0;
@@ -953,12 +977,16 @@ bool Executor::scan_insert(uint32_t rel, const std::vector<uint32_t>& eqs, uint6
     ms_build += fms;
     const uint64_t n = uint64_t(back[4]) | (uint64_t(back[5]) << 32);
     if (back[2]) { // an insert passed the walk cap: the table is incomplete
+        drop_refilter();
         ++retries;
         ++retries_build;
         walked = true;
         return false;
     }
-    if (n == 0) return true; // empty build side
+    if (n == 0) {
+        drop_refilter();
+        return true; // empty build side
+    }
     ht->tuples = n;
     ht->unique = back[0] == 0;
     build_tuples += n;
@@ -1024,11 +1052,13 @@ bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64
     uint32_t f[4];
     CGJ_HIP(hipMemcpyAsync(&n, total.get(), sizeof(n), hipMemcpyDeviceToHost, s));
     CGJ_HIP(hipMemcpyAsync(f, flags.get(), sizeof(f), hipMemcpyDeviceToHost, s));
+    queue_refilter(ht.get());
     CGJ_HIP(hipEventRecord(ev1_, s));
     CGJ_HIP(hipEventSynchronize(ev1_));
     float fms = 0;
     CGJ_HIP(hipEventElapsedTime(&fms, ev0_, ev1_));
     ms_build += fms;
+    if (n == 0 || f[1] || f[2]) drop_refilter();
     if (n == 0) return true; // empty build side
     if (f[1] || f[2]) { // a region overflowed or a key repeats heavily: the staged path handles it
         ++retries;
@@ -1382,47 +1412,62 @@ bool Executor::next_build(int node, int& x, uint32_t& key, uint64_t& probe_rows)
     return false;
 }
 
-void Executor::refilter(uint32_t key, const HashTable* ht) {
-    if (!samples_) return;
+void Executor::queue_refilter(const HashTable* ht) {
+    rf_.table = nullptr;
+    rf_.todo.clear();
+    if (!rf_.armed || !samples_) return;
     // relations still planned on their own that produce the key class
-    uint64_t grouped = 0;
-    for (const Group& g : groups_) grouped |= g.set;
-    std::vector<std::pair<uint32_t, const std::vector<uint64_t>*>> todo;
-    size_t                                                         n = 0;
+    size_t n = 0;
     for (uint32_t r = 0; r < q_.rels.size(); ++r) {
-        if (((grouped >> r) & 1) || samples_->size[r] == 0) continue;
-        auto it = samples_->keys[r].find(key);
+        if (((rf_.grouped >> r) & 1) || samples_->size[r] == 0) continue;
+        auto it = samples_->keys[r].find(rf_.key);
         if (it == samples_->keys[r].end()) continue;
-        todo.emplace_back(r, &it->second);
+        rf_.todo.emplace_back(r, uint32_t(it->second.size()));
         n += it->second.size();
     }
+    rf_.table = ht;
     if (!n) return;
-    uint8_t*  pin   = ctx_.small_pinned(n * sizeof(uint64_t) + n * sizeof(uint32_t));
-    uint64_t* keys  = reinterpret_cast<uint64_t*>(pin);
-    uint32_t* found = reinterpret_cast<uint32_t*>(pin + n * sizeof(uint64_t));
-    size_t    k     = 0;
-    for (auto& [r, v] : todo)
-        for (uint64_t x : *v) keys[k++] = x;
-    DevBuf      dev(&ctx_.heap(), n * (sizeof(uint64_t) + sizeof(uint32_t)));
+    if (n * (sizeof(uint64_t) + sizeof(uint32_t)) > Context::kSamplePinnedBytes)
+        throw std::runtime_error("sample re-filter: more sample keys than its buffer holds");
+    uint8_t*  pin  = ctx_.sample_pinned();
+    uint64_t* keys = reinterpret_cast<uint64_t*>(pin);
+    size_t    k    = 0;
+    for (auto& [r, cnt] : rf_.todo)
+        for (uint64_t x : samples_->keys[r].find(rf_.key)->second) keys[k++] = x;
+    rf_.dev       = DevBuf(&ctx_.heap(), n * (sizeof(uint64_t) + sizeof(uint32_t)));
     hipStream_t s = ctx_.stream();
-    CGJ_HIP(hipMemcpyAsync(dev.get(), keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    CGJ_HIP(hipMemcpyAsync(rf_.dev.get(), keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     SampleTable st;
     st.ht    = ht->slots.get<uint64_t>();
     st.mask  = ht->rmask;
     st.shift = ht->shift;
     st.words = ht->grouped ? 2 : ht->words;
-    uint32_t* dfound = reinterpret_cast<uint32_t*>(dev.get<uint8_t>() + n * sizeof(uint64_t));
-    CGJ_HIP(launch_sample_probe(st, dev.get<uint64_t>(), uint32_t(n), dfound, s));
-    CGJ_HIP(hipMemcpyAsync(found, dfound, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    CGJ_HIP(hipStreamSynchronize(s));
-    k = 0;
-    for (auto& [r, v] : todo) {
+    uint32_t* dfound = reinterpret_cast<uint32_t*>(rf_.dev.get<uint8_t>() + n * sizeof(uint64_t));
+    CGJ_HIP(launch_sample_probe(st, rf_.dev.get<uint64_t>(), uint32_t(n), dfound, s));
+    CGJ_HIP(hipMemcpyAsync(pin + n * sizeof(uint64_t), dfound, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+}
+
+void Executor::finish_refilter(const HashTable* ht) {
+    if (!samples_) return;
+    if (rf_.table != ht) { // the build's path did not queue it: probe now
+        queue_refilter(ht);
+        CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
+    }
+    // the build synchronised after the queued copies: the hits are here
+    size_t n = 0;
+    for (auto& [r, cnt] : rf_.todo) n += cnt;
+    const uint32_t* found = reinterpret_cast<const uint32_t*>(ctx_.sample_pinned() + n * sizeof(uint64_t));
+    size_t          k     = 0;
+    for (auto& [r, cnt] : rf_.todo) {
         uint64_t m = 0;
-        for (size_t i = 0; i < v->size(); ++i, ++k)
+        for (uint32_t i = 0; i < cnt; ++i, ++k)
             if (found[k]) m |= 1ull << i;
         samples_->matches[r] &= m;
         samples_->pass[r][ht] = m;
     }
+    rf_.table = nullptr;
+    rf_.todo.clear();
+    rf_.dev.reset();
 }
 
 std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const VertexCard& card,
@@ -1471,21 +1516,31 @@ std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const Vert
             if (count_only) res->nrows = t.count;
             return res;
         }
+        // The samples steer the next join order and the probe order of a
+        // pipeline with several probes: with one relation and one table left
+        // after this build there is nothing for them to decide.
+        const uint64_t set     = tree.nodes[x].set;
+        uint64_t       covered = set;
+        size_t         ntables = 1;
+        for (const Group& g : groups_)
+            if ((g.set & set) != g.set) { // tables the new one consumes drop out
+                covered |= g.set;
+                ++ntables;
+            }
+        const uint32_t left = uint32_t(q_.rels.size()) - uint32_t(__builtin_popcountll(covered));
+        rf_.armed   = samples_ && (left >= 2 || ntables >= 2);
+        rf_.key     = key;
+        rf_.grouped = covered;
+        rf_.table   = nullptr;
         const HashTable* ht = build(x, key, prows);
         if (!ht) return res; // an empty build side empties the inner join
-        const uint64_t set = tree.nodes[x].set;
         std::vector<Group> keep;
         for (const Group& g : groups_)
-            if ((g.set & set) != g.set) keep.push_back(g); // tables the new one consumed drop out
+            if ((g.set & set) != g.set) keep.push_back(g);
         keep.push_back(Group{set, key, ht});
         groups_ = std::move(keep);
-        // the samples steer the next join order and the probe order of a
-        // pipeline with several probes: with one relation and one table left
-        // there is nothing for them to decide
-        uint64_t covered = 0;
-        for (const Group& g : groups_) covered |= g.set;
-        const uint32_t left = uint32_t(q_.rels.size()) - uint32_t(__builtin_popcountll(covered));
-        if (left >= 2 || groups_.size() >= 2) refilter(key, ht);
+        if (rf_.armed) finish_refilter(ht);
+        rf_.armed = false;
     }
 }
 

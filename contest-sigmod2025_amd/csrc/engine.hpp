@@ -265,6 +265,12 @@ public:
     // estimator's features and outputs), grown on demand and kept for the
     // context's lifetime.
     uint8_t* small_pinned(size_t bytes);
+    // Pinned host buffer of kSamplePinnedBytes for the sample re-filter of
+    // the incremental loop (keys out, hit flags back), apart from
+    // small_pinned so a re-filter queued ahead of a build's last
+    // synchronisation is not overwritten by the build's own transfers.
+    static constexpr size_t kSamplePinnedBytes = size_t(64) << 10; // 64 relations x 64 keys x 12 B
+    uint8_t*                sample_pinned();
     void                    staging_done(int b); // records the copy just issued through buffer b
     double   copy_probe(uint64_t bytes, uint32_t reps);
 
@@ -322,6 +328,7 @@ private:
     void*                       pin_[2]    = {nullptr, nullptr};
     hipEvent_t                  pin_ev_[2] = {nullptr, nullptr};
     void*                       spin_       = nullptr;
+    void*                       sample_pin_ = nullptr;
     size_t                      spin_bytes_ = 0;
     std::vector<std::unique_ptr<Context>> workers_;
 };
