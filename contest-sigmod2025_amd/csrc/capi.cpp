@@ -358,6 +358,10 @@ int cgj_set_estimator(cgj_context* ctx, uint32_t mode, cgj_mlp* mlp, float max_s
     });
 }
 
+int cgj_clear_statistics(cgj_context* ctx) {
+    return guarded(ctx, [&] { ctx->impl->estimator().clear_pairs(); });
+}
+
 int64_t cgj_plan_features(cgj_context* ctx, const cgj_plan* plan, cgj_dtable* const* inputs, uint32_t num_inputs,
                           uint64_t* sets, float* features, uint64_t cap) {
     int64_t n  = -1;

@@ -42,39 +42,124 @@ size_t DeviceHeap::round(size_t b) {
     return (b + q - 1) & ~(q - 1);
 }
 
+// Cached (freed, not yet released) blocks of all heaps together stay below a
+// quarter of the device's memory: the contexts of execute_many each keep a
+// heap, and caches that grew without bound on every context would leave the
+// runtime itself no memory (kernel launches allocate too). A heap whose free
+// takes the total over the bound releases its own oldest blocks until the
+// total is 1/8 below it, in one go, since a release waits for the whole
+// device (hipFree synchronises it). While worker contexts run, the bound is
+// doubled and the heaps settle under it when the workers are done.
+namespace {
+std::mutex               g_heaps_mu;
+std::vector<DeviceHeap*> g_heaps;
+std::atomic<uint64_t>    g_cached{0};
+std::atomic<uint32_t>    g_parallel{0};
+std::atomic<uint64_t>    g_cache_cap{0};
+
+uint64_t cache_cap() {
+    uint64_t c = g_cache_cap.load();
+    if (!c) {
+        size_t freeb = 0, total = 0;
+        c = hipMemGetInfo(&freeb, &total) == hipSuccess && total ? total / 4 : (uint64_t(16) << 30);
+        g_cache_cap.store(c);
+    }
+    return c;
+}
+} // namespace
+
+DeviceHeap::DeviceHeap() {
+    if (hipGetDevice(&device_) != hipSuccess) device_ = 0;
+    std::lock_guard<std::mutex> lock(g_heaps_mu);
+    g_heaps.push_back(this);
+}
+
+DeviceHeap::Parallel::Parallel() { ++g_parallel; }
+DeviceHeap::Parallel::~Parallel() { --g_parallel; }
+
+void DeviceHeap::settle(int device) {
+    const uint64_t              cap = cache_cap();
+    std::lock_guard<std::mutex> lock(g_heaps_mu);
+    for (DeviceHeap* h : g_heaps) {
+        if (g_cached <= cap) break;
+        if (h->device_ != device) continue;
+        std::lock_guard<std::mutex> hl(h->mu_);
+        const uint64_t              over = g_cached - std::min<uint64_t>(g_cached, cap - cap / 8);
+        h->evict_locked(h->cached_ - std::min<uint64_t>(h->cached_, over));
+    }
+}
+
+void DeviceHeap::trim_device(int device) {
+    std::lock_guard<std::mutex> lock(g_heaps_mu);
+    for (DeviceHeap* h : g_heaps)
+        if (h->device_ == device) h->trim();
+}
+
 void* DeviceHeap::alloc(size_t bytes) {
-    size_t r  = round(bytes);
-    auto   it = free_.lower_bound(r);
-    if (it != free_.end() && it->first <= r + r / 2) {
-        void* p = it->second;
-        live_[p] = it->first;
-        free_.erase(it);
-        return p;
+    const size_t r = round(bytes);
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto                        it = free_.lower_bound(r);
+        if (it != free_.end() && it->first <= r + r / 2) {
+            const auto blk = it->second;
+            void*      p   = blk->ptr;
+            live_[p]       = blk->bytes;
+            cached_ -= blk->bytes;
+            g_cached -= blk->bytes;
+            free_.erase(it);
+            order_.erase(blk);
+            return p;
+        }
     }
     void*      p = nullptr;
     hipError_t e = hipMalloc(&p, r);
     if (e != hipSuccess) {
-        trim();
+        (void)hipGetLastError();
+        trim_device(device_);
         e = hipMalloc(&p, r);
     }
     if (e != hipSuccess) throw HipError("hipMalloc(" + std::to_string(r) + "): " + hipGetErrorString(e));
+    std::lock_guard<std::mutex> lock(mu_);
     live_[p] = r;
     return p;
 }
 
 void DeviceHeap::free(void* p) {
-    auto it = live_.find(p);
+    std::lock_guard<std::mutex> lock(mu_);
+    auto                        it = live_.find(p);
     if (it == live_.end()) return;
-    free_.emplace(it->second, p);
+    const size_t bytes = it->second;
     live_.erase(it);
+    order_.push_back(Cached{bytes, p, {}});
+    auto blk     = std::prev(order_.end());
+    blk->by_size = free_.emplace(bytes, blk);
+    cached_ += bytes;
+    const uint64_t cap   = g_parallel ? 2 * cache_cap() : cache_cap();
+    const uint64_t total = g_cached += bytes;
+    if (total > cap) evict_locked(cached_ - std::min<uint64_t>(cached_, total - (cap - cap / 8)));
+}
+
+void DeviceHeap::evict_locked(size_t keep) {
+    while (cached_ > keep && !order_.empty()) {
+        Cached& c = order_.front();
+        (void)hipFree(c.ptr); // waits for work still using the block
+        cached_ -= c.bytes;
+        g_cached -= c.bytes;
+        free_.erase(c.by_size);
+        order_.pop_front();
+    }
 }
 
 void DeviceHeap::trim() {
-    for (auto& kv : free_) (void)hipFree(kv.second);
-    free_.clear();
+    std::lock_guard<std::mutex> lock(mu_);
+    evict_locked(0);
 }
 
 DeviceHeap::~DeviceHeap() {
+    {
+        std::lock_guard<std::mutex> lock(g_heaps_mu);
+        g_heaps.erase(std::remove(g_heaps.begin(), g_heaps.end(), this), g_heaps.end());
+    }
     trim();
     for (auto& kv : live_) (void)hipFree(kv.first);
     live_.clear();
@@ -1561,6 +1646,7 @@ std::vector<std::unique_ptr<DevResult>> Context::execute_many(const std::vector<
         workers_.back()->hints_  = hints_;
     }
     for (uint32_t w = 0; w < workers; ++w) workers_[w]->options = options;
+    heap_->trim(); // this context's cached blocks would sit idle while the workers run
     // Each worker plans its query with this context's estimator (statistics
     // catalog shared under a lock, the network's inference on the worker's own
     // stream) and executes it, so planning runs on as many threads as
@@ -1581,9 +1667,13 @@ std::vector<std::unique_ptr<DevResult>> Context::execute_many(const std::vector<
             next      = plans.size(); // stop the other workers early
         }
     };
-    std::vector<std::thread> threads;
-    for (uint32_t w = 0; w < workers; ++w) threads.emplace_back(run, w);
-    for (auto& t : threads) t.join();
+    {
+        DeviceHeap::Parallel     parallel;
+        std::vector<std::thread> threads;
+        for (uint32_t w = 0; w < workers; ++w) threads.emplace_back(run, w);
+        for (auto& t : threads) t.join();
+    }
+    DeviceHeap::settle(device_);
     for (const auto& e : errors)
         if (!e.empty()) throw std::runtime_error("execute_many: " + e);
     return out;

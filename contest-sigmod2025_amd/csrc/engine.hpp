@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -38,15 +39,39 @@ struct HipError : std::runtime_error {
 // share ownership of the heap, so tables and results may outlive their context.
 class DeviceHeap : public std::enable_shared_from_this<DeviceHeap> {
 public:
+    DeviceHeap();
     ~DeviceHeap();
     void* alloc(size_t bytes);
     void  free(void* p);
     void  trim();
+    // While worker contexts run queries on one device, releases (each waits
+    // for the whole device) are put off until the cached total passes twice
+    // the bound; settle() brings the device's heaps back under the bound
+    // once the workers are done.
+    struct Parallel {
+        Parallel();
+        ~Parallel();
+    };
+    static void settle(int device);
 
 private:
     static size_t round(size_t bytes);
-    std::multimap<size_t, void*>      free_;
-    std::unordered_map<void*, size_t> live_;
+    // Releases the cached blocks of every heap of `device` (this one
+    // included): an allocation that failed retries after this.
+    static void trim_device(int device);
+    // Releases the oldest cached blocks until at most `keep` bytes stay cached.
+    void        evict_locked(size_t keep);
+    struct Cached {
+        size_t                                       bytes;
+        void*                                        ptr;
+        std::multimap<size_t, std::list<Cached>::iterator>::iterator by_size;
+    };
+    int                                                 device_ = 0;
+    std::mutex                                          mu_; // worker contexts trim each other's caches
+    std::list<Cached>                                   order_; // cached blocks, oldest first
+    std::multimap<size_t, std::list<Cached>::iterator> free_;  // the same blocks by size
+    size_t                                              cached_ = 0;
+    std::unordered_map<void*, size_t>                   live_;
 };
 
 // RAII device buffer from a DeviceHeap.

@@ -153,6 +153,10 @@ public:
     CardFn     model(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep,
                      std::shared_ptr<Samples> samples = nullptr);
     QueryStats stats(Context& ctx, const Query& q, const std::vector<const DevTable*>& in, EstimateReport& rep);
+    void       clear_pairs() {
+        std::lock_guard<std::mutex> lock(pairs_mu_);
+        pair_rows_.clear();
+    }
     size_t     cached_pairs() const {
         std::lock_guard<std::mutex> lock(pairs_mu_);
         return pair_rows_.size();

@@ -196,6 +196,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_void_p)]),
     "cgj_num_features": (C.c_uint32, []),
     "cgj_set_estimator": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_float]),
+    "cgj_clear_statistics": (C.c_int, [C.c_void_p]),
     "cgj_plan_features": (C.c_int64, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32,
                                       C.POINTER(C.c_uint64), _F32P, C.c_uint64]),
     "cgj_count_subset": (C.c_int, [C.c_void_p, C.POINTER(CPlan), C.POINTER(C.c_void_p), C.c_uint32, C.c_uint64,
@@ -737,6 +738,12 @@ class Context:
         num_features() inputs (low-confidence sets take the sampling estimate)."""
         h = mlp.handle if mlp is not None else None
         if lib().cgj_set_estimator(self.handle, mode, h, max_sigma) != 0:
+            raise RuntimeError(self.error())
+
+    def clear_statistics(self):
+        """Empties the catalog of pairwise join counts the independence and
+        learned estimators read (they are counted again on demand)."""
+        if lib().cgj_clear_statistics(self.handle) != 0:
             raise RuntimeError(self.error())
 
     @staticmethod

@@ -337,6 +337,11 @@ void     cgj_mlp_free(cgj_context* ctx, cgj_mlp* m);
 enum { CGJ_EST_MAX = 0, CGJ_EST_INDEPENDENCE = 1, CGJ_EST_LEARNED = 2, CGJ_EST_SAMPLE = 3 };
 uint32_t cgj_num_features(void);
 int      cgj_set_estimator(cgj_context* ctx, uint32_t mode, cgj_mlp* mlp, float max_sigma);
+/* Empties the context's catalog of pairwise join counts (the next plans under
+ * CGJ_EST_INDEPENDENCE or CGJ_EST_LEARNED count their pairs again). No
+ * counterpart in the reference, which keeps no catalog: a caller that times
+ * planning from cold clears it first. */
+int      cgj_clear_statistics(cgj_context* ctx);
 /* Relations of a plan are numbered in its depth-first scan order (left
  * input first). Writes, for up to `cap` connected sets of two or more
  * relations, the set (bit r = relation r) and its cgj_num_features() features;
