@@ -451,6 +451,8 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "scan_insert_max_bytes") options.scan_insert_max_bytes = v > 0 ? uint64_t(v) : 0;
     else if (name == "scan_workgroups") options.scan_workgroups = v >= 16 && v <= 4096 ? uint32_t(v) : 512u;
     else if (name == "incremental") options.incremental = v != 0;
+    else if (name == "restrictions") options.restrictions = v != 0;
+    else if (name == "singletons") options.singletons = v != 0;
     else return false;
     return true;
 }
@@ -523,6 +525,11 @@ public:
         counters_ = DevBuf(&ctx.heap(), sizeof(PipeCounters));
         CGJ_HIP(hipEventCreate(&ev0_));
         CGJ_HIP(hipEventCreate(&ev1_));
+        rfilt_.resize(q.rels.size());
+        rtables_.resize(q.rels.size());
+        sel_.resize(q.rels.size());
+        sel_done_.assign(q.rels.size(), false);
+        for (const QFilter& f : q.filters) add_filter(q.attrs[f.attr].rel, host_filter(f));
     }
     ~Executor() {
         (void)hipEventDestroy(ev0_);
@@ -542,6 +549,7 @@ public:
 
     double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0, ms_partition = 0, ms_replan = 0;
     uint32_t launches = 0, retries = 0, retries_build = 0, probe_lean = 0, probe_waves = 0, replans = 0;
+    uint32_t restrictions = 0, semijoins = 0;
     uint64_t build_tuples = 0, probe_rows = 0, build_bytes = 0, out_ref_pages = 0, probe_input_pages = 0;
 
 private:
@@ -582,6 +590,28 @@ private:
         DevBuf                                     dev;
     };
     PendingRefilter rf_;
+    // Restrictions of each relation's scans (reference Restriction.hpp,
+    // RestrictionLogic.cpp): the query's own (singleton constants) and those
+    // the incremental loop derives from built tables. Before a relation's
+    // first scan its selection words are computed once (launch_filter_select)
+    // and every kernel scanning it reads them.
+    std::vector<std::vector<FilterSpec>> rfilt_;
+    std::vector<DevBuf>                  sel_;
+    std::vector<bool>                    sel_done_;
+    // Semi-join tables restricting a relation (exact membership of its key):
+    // a pipeline driven by the relation probes them as early, payload-free
+    // probes (with their bloom filters); a scan building a hash table turns
+    // them into selection words.
+    std::vector<std::vector<std::pair<const HashTable*, uint32_t>>> rtables_;
+    uint64_t                             done_ = 0; // relations a semi-join consumed (no longer planned)
+    FilterSpec      host_filter(const QFilter& f) const;
+    FilterSpec      table_filter(uint32_t attr, const HashTable* ht) const;
+    void            add_filter(uint32_t rel, const FilterSpec& f);
+    const uint64_t* sel_for(uint32_t rel);
+    // After building `ht` on class `key` over relations `set`: restricts the
+    // relations still unjoined that produce the key; returns true when the
+    // table is consumed as a semi-join (no longer a join input).
+    bool            restrict_after_build(const HashTable* ht, uint32_t key, uint64_t set);
     void            queue_refilter(const HashTable* ht);
     void            drop_refilter() { rf_.table = nullptr; }
     void            finish_refilter(const HashTable* ht);
@@ -1014,7 +1044,134 @@ ScanBuildArgs Executor::scan_args(uint32_t rel, const std::vector<uint32_t>& eqs
     }
     sa.notnull |= 1u; // the key
     sa.nrows = q_.rels[rel].rows;
+    for (auto [ht, key] : rtables_[rel]) add_filter(rel, table_filter(q_.attr_of(rel, key), ht));
+    rtables_[rel].clear();
+    sa.sel = sel_for(rel);
     return sa;
+}
+
+FilterSpec Executor::host_filter(const QFilter& f) const {
+    const QAttr& at = q_.attrs[f.attr];
+    FilterSpec   s{};
+    s.col       = column(at.rel, at.col).view(0);
+    s.kind      = f.kind;
+    s.a         = f.a;
+    s.b         = f.b;
+    s.tiny_bits = f.tiny_bits;
+    s.tiny_hash = f.tiny_hash;
+    for (int i = 0; i < kTinySlots; ++i) s.tiny[i] = size_t(i) < f.tiny.size() ? f.tiny[size_t(i)] : kNull;
+    return s;
+}
+
+FilterSpec Executor::table_filter(uint32_t attr, const HashTable* ht) const {
+    const QAttr& at = q_.attrs[attr];
+    FilterSpec   s{};
+    s.col   = column(at.rel, at.col).view(0);
+    s.kind  = FLT_TABLE;
+    s.table = ht->slots.get<uint64_t>();
+    s.mask  = ht->rmask;
+    s.shift = ht->shift;
+    s.words = ht->grouped ? 2 : ht->words;
+    return s;
+}
+
+void Executor::add_filter(uint32_t rel, const FilterSpec& f) {
+    if (rel >= rfilt_.size() || q_.rels[rel].rows == 0) return;
+    ++restrictions;
+    if (!sel_done_[rel]) {
+        rfilt_[rel].push_back(f);
+        return;
+    }
+    // the selection words exist already: narrow them
+    FilterArgs fa{};
+    fa.nrows      = q_.rels[rel].rows;
+    fa.n          = 1;
+    fa.accumulate = 1;
+    fa.f[0]       = f;
+    CGJ_HIP(launch_filter_select(fa, sel_[rel].get<uint64_t>(), ctx_.stream()));
+}
+
+const uint64_t* Executor::sel_for(uint32_t rel) {
+    if (rel >= rfilt_.size()) return nullptr;
+    if (sel_done_[rel]) return sel_[rel].get<uint64_t>();
+    if (rfilt_[rel].empty()) return nullptr;
+    const uint64_t nrows = q_.rels[rel].rows;
+    sel_[rel]            = DevBuf(&ctx_.heap(), std::max<uint64_t>(1, (nrows + 63) / 64) * sizeof(uint64_t));
+    const auto& fs       = rfilt_[rel];
+    for (size_t i = 0; i < fs.size(); i += kMaxFilters) {
+        FilterArgs fa{};
+        fa.nrows      = nrows;
+        fa.accumulate = i > 0 ? 1 : 0;
+        fa.n          = uint32_t(std::min<size_t>(kMaxFilters, fs.size() - i));
+        for (uint32_t k = 0; k < fa.n; ++k) fa.f[k] = fs[i + k];
+        CGJ_HIP(launch_filter_select(fa, sel_[rel].get<uint64_t>(), ctx_.stream()));
+    }
+    sel_done_[rel] = true;
+    return sel_[rel].get<uint64_t>();
+}
+
+bool Executor::restrict_after_build(const HashTable* ht, uint32_t key, uint64_t set) {
+    if (!ctx_.options.restrictions) return false;
+    // relations still planned on their own that produce the key class
+    uint64_t grouped = done_ | set;
+    bool     other   = false; // another built table produces the key
+    for (const Group& g : groups_) {
+        if ((g.set & set) == g.set) continue; // consumed by the new table
+        grouped |= g.set;
+        if (q_.produced(g.set).count(key)) other = true;
+    }
+    std::vector<uint32_t> targets;
+    for (uint32_t r = 0; r < q_.rels.size(); ++r)
+        if (!((grouped >> r) & 1) && std::binary_search(q_.rels[r].eqs.begin(), q_.rels[r].eqs.end(), key))
+            targets.push_back(r);
+    if (targets.empty()) return false;
+    // The join is a semi-join when the table is duplicate-free and carries
+    // only its key, and no other built table produces the key (whose rows
+    // could not be restricted any more): the table then only restricts the
+    // key's producers exactly (reference QueryPlan.cpp:602-625, JoinPrecise).
+    const bool semijoin = ht->unique && ht->words == 1 && !other && q_.eq_attrs[key].size() > 1;
+    // A table of at most 32 tuples restricts by its keys (Eq, Eq2, a range or
+    // a tiny table, RestrictionLogic.cpp:397-447); else a semi-join keeps the
+    // exact table membership, and a join leaves its probe to filter.
+    QFilter small;
+    bool    have_small = false;
+    if (ht->tuples <= 32) {
+        DevBuf         buf(&ctx_.heap(), 32 * sizeof(uint64_t) + sizeof(uint32_t));
+        auto*          cnt = reinterpret_cast<unsigned int*>(buf.get<uint8_t>() + 32 * sizeof(uint64_t));
+        hipStream_t    st  = ctx_.stream();
+        uint8_t*       pin = ctx_.sample_pinned() + Context::kSamplePinnedBytes - 512;
+        CGJ_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), st));
+        CGJ_HIP(launch_table_keys(ht->slots.get<uint64_t>(), ht->cap, ht->grouped ? 2 : ht->words, buf.get<uint64_t>(),
+                                  32, cnt, st));
+        CGJ_HIP(hipMemcpyAsync(pin, buf.get(), 32 * sizeof(uint64_t) + sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        CGJ_HIP(hipStreamSynchronize(st));
+        const uint32_t n = std::min<uint32_t>(*reinterpret_cast<const uint32_t*>(pin + 32 * sizeof(uint64_t)), 32);
+        std::vector<uint64_t> keys(reinterpret_cast<const uint64_t*>(pin), reinterpret_cast<const uint64_t*>(pin) + n);
+        have_small = small_restriction(keys, 0, small);
+    }
+    if (!have_small && !semijoin) return false;
+    for (uint32_t r : targets) {
+        const uint32_t attr = q_.attr_of(r, key);
+        if (have_small) {
+            QFilter f = small;
+            f.attr    = attr;
+            add_filter(r, host_filter(f));
+            if (samples_ && samples_->size[r]) { // the sample rows that pass
+                auto it = samples_->keys[r].find(key);
+                if (it != samples_->keys[r].end())
+                    for (size_t i = 0; i < it->second.size(); ++i)
+                        if (!f.pass(it->second[i])) samples_->matches[r] &= ~(1ull << i);
+            }
+        } else {
+            rtables_[r].emplace_back(ht, key);
+            ++restrictions;
+        }
+    }
+    if (semijoin) {
+        done_ |= set;
+        ++semijoins;
+    }
+    return semijoin;
 }
 
 bool Executor::scan_insert(uint32_t rel, const std::vector<uint32_t>& eqs, uint64_t probe_est, const HashTable*& out,
@@ -1160,6 +1317,13 @@ bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64
 }
 
 void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
+    // semi-join tables restricting the driving relation: early probes, first
+    if (d.rel >= 0 && !rtables_[uint32_t(d.rel)].empty()) {
+        std::vector<ProbeStep> semi;
+        for (auto [ht, key] : rtables_[uint32_t(d.rel)]) semi.push_back(ProbeStep{ht, key, 0});
+        rtables_[uint32_t(d.rel)].clear();
+        probes.insert(probes.begin(), semi.begin(), semi.end());
+    }
     // Probe order: where the driving relation's samples were probed into the
     // tables, the most selective table goes first (fewest sample rows found),
     // among those whose key the pipeline already carries; the reference
@@ -1223,7 +1387,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
     // radix-partitioned by the probe key's hash; the pipeline then walks the
     // partitions XCD by XCD so each probes a cache-resident range of the table.
     bool xcd_order = false;
-    if (d.rel >= 0 && !probes.empty()) {
+    if (d.rel >= 0 && !probes.empty() && rfilt_[uint32_t(d.rel)].empty()) {
         const HashTable* h0 = probes[0].ht;
         uint64_t         hb = h0->cap * h0->words * sizeof(uint64_t);
         if (hb >= ctx_.options.radix_min_table_bytes && q_.rels[d.rel].rows >= ctx_.options.radix_min_rows) {
@@ -1245,6 +1409,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
     std::vector<uint64_t>        scan_cols;
     if (d.rel >= 0) {
         a.driver = DRIVER_PAGES;
+        a.sel    = sel_for(uint32_t(d.rel));
         scan_eqs = q_.needed(1ull << d.rel);
         nrows    = q_.rels[d.rel].rows;
         if (scan_eqs.size() > size_t(kMaxScan)) throw std::runtime_error("too many scanned columns in a pipeline");
@@ -1578,7 +1743,7 @@ std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const Vert
             grouped |= groups_[g].set;
         }
         for (uint32_t r = 0; r < q_.rels.size(); ++r)
-            if (!((grouped >> r) & 1)) {
+            if (!(((grouped | done_) >> r) & 1)) {
                 PlanVertex v;
                 v.set = 1ull << r;
                 v.rel = int(r);
@@ -1605,7 +1770,7 @@ std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const Vert
         // pipeline with several probes: with one relation and one table left
         // after this build there is nothing for them to decide.
         const uint64_t set     = tree.nodes[x].set;
-        uint64_t       covered = set;
+        uint64_t       covered = set | done_;
         size_t         ntables = 1;
         for (const Group& g : groups_)
             if ((g.set & set) != g.set) { // tables the new one consumes drop out
@@ -1619,10 +1784,11 @@ std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const Vert
         rf_.table   = nullptr;
         const HashTable* ht = build(x, key, prows);
         if (!ht) return res; // an empty build side empties the inner join
+        const bool         semi = restrict_after_build(ht, key, set);
         std::vector<Group> keep;
         for (const Group& g : groups_)
             if ((g.set & set) != g.set) keep.push_back(g);
-        keep.push_back(Group{set, key, ht});
+        if (!semi) keep.push_back(Group{set, key, ht});
         groups_ = std::move(keep);
         if (rf_.armed) finish_refilter(ht);
         rf_.armed = false;
@@ -1762,16 +1928,102 @@ PlannedQuery Context::plan_query(const cgj_plan& plan, const std::vector<const D
     PlannedQuery pq;
     pq.q = import_inputs(plan, inputs);
     check_types(pq.q, inputs);
+    for (uint32_t a : pq.q.out) pq.types.push_back(pq.q.attrs[a].type);
     const auto t0 = std::chrono::steady_clock::now();
     pq.mode       = est.mode;
     bool empty    = false;
     for (const auto& r : pq.q.rels) empty = empty || r.rows == 0;
+    bool single = false;
+    for (const auto& r : pq.q.rels) single = single || r.rows == 1;
+    if (options.singletons && single && !empty && pq.q.rels.size() > 1) {
+        SingletonReduction red = eliminate_singletons(pq.q, singleton_values(pq.q, inputs));
+        if (red.eliminated) {
+            for (ConstOut& c : red.consts)
+                if (c.type == T_VARCHAR && c.value != kNull) {
+                    const QAttr& at = pq.q.attrs[pq.q.rels[c.rel].attrs[c.slot]];
+                    c.str           = read_string(inputs[pq.q.rels[c.rel].table]->cols[at.col], c.value);
+                }
+            pq.q          = std::move(red.q);
+            pq.consts     = std::move(red.consts);
+            pq.out_pos    = std::move(red.out_pos);
+            pq.empty      = red.empty;
+            pq.singletons = red.eliminated;
+        }
+    }
+    if (pq.empty) { // no row can result: nothing to plan
+        pq.ms_plan = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return pq;
+    }
     if ((est.mode == EST_SAMPLE || est.mode == EST_LEARNED) && !empty) pq.samples = take_samples(pq.q, inputs);
     pq.card       = est.model(*this, pq.q, inputs, pq.rep, pq.samples);
     pq.tree       = plan_joins(pq.q, pq.card);
     pq.ms_plan =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() - pq.rep.ms_stats;
     return pq;
+}
+
+std::vector<std::vector<uint64_t>> Context::singleton_values(const Query& q,
+                                                             const std::vector<const DevTable*>& inputs) {
+    std::vector<std::vector<uint64_t>>          vals(q.rels.size());
+    std::vector<SampleCol>                      cols;
+    std::vector<std::pair<uint32_t, uint32_t>> where; // (relation, attribute index)
+    for (uint32_t r = 0; r < q.rels.size(); ++r) {
+        if (q.rels[r].rows != 1) continue;
+        for (uint32_t i = 0; i < q.rels[r].attrs.size(); ++i) {
+            const DevColumn& c = inputs[q.rels[r].table]->cols[q.attrs[q.rels[r].attrs[i]].col];
+            // column k reads row id k of the list and writes output k
+            cols.push_back(SampleCol{c.view(0), uint32_t(cols.size()), 1});
+            where.emplace_back(r, i);
+        }
+        vals[r].assign(q.rels[r].attrs.size(), kNull);
+    }
+    if (cols.empty()) return vals;
+    const size_t cb = cols.size() * sizeof(SampleCol), ob = cols.size() * sizeof(uint64_t);
+    uint8_t*     pin = small_pinned(cb + 2 * ob);
+    std::memcpy(pin, cols.data(), cb);
+    std::memset(pin + cb, 0, ob); // every row id is 0: the relation's one row
+    DevBuf dev(heap_.get(), cb + 2 * ob);
+    CGJ_HIP(hipMemcpyAsync(dev.get(), pin, cb + ob, hipMemcpyHostToDevice, stream_));
+    uint8_t* d = dev.get<uint8_t>();
+    CGJ_HIP(launch_sample_gather(reinterpret_cast<const SampleCol*>(d), uint32_t(cols.size()),
+                                 reinterpret_cast<const uint64_t*>(d + cb), reinterpret_cast<uint64_t*>(d + cb + ob),
+                                 stream_));
+    CGJ_HIP(hipMemcpyAsync(pin + cb + ob, d + cb + ob, ob, hipMemcpyDeviceToHost, stream_));
+    CGJ_HIP(hipStreamSynchronize(stream_));
+    const uint64_t* out = reinterpret_cast<const uint64_t*>(pin + cb + ob);
+    for (size_t k = 0; k < where.size(); ++k) vals[where[k].first][where[k].second] = out[k];
+    return vals;
+}
+
+std::string Context::read_string(const DevColumn& c, uint64_t handle) {
+    uint64_t       p   = handle_page(handle);
+    const uint32_t idx = handle_idx(handle);
+    if (p >= c.npages) throw std::runtime_error("string handle past the column's pages");
+    uint8_t* pin  = small_pinned(kPageSize);
+    auto     read = [&](uint64_t page) {
+        CGJ_HIP(hipMemcpyAsync(pin, c.pages.get<uint8_t>() + page * kPageSize, kPageSize, hipMemcpyDeviceToHost,
+                               stream_));
+        CGJ_HIP(hipStreamSynchronize(stream_));
+    };
+    auto u16 = [&](uint32_t off) { return uint32_t(pin[off]) | (uint32_t(pin[off + 1]) << 8); };
+    read(p);
+    std::string s;
+    if (idx == 0xffff) { // a long string: the first page, then its continuations
+        for (;;) {
+            const uint32_t len = std::min<uint32_t>(u16(2), kPageSize - 4);
+            s.append(reinterpret_cast<const char*>(pin + 4), len);
+            if (++p >= c.npages) break;
+            read(p);
+            if (u16(0) != 0xfffe) break;
+        }
+        return s;
+    }
+    const uint32_t nv = u16(2);
+    if (idx >= nv) throw std::runtime_error("string handle past the page's values");
+    const uint32_t base = 4 + 2 * nv;
+    const uint32_t st = idx ? u16(4 + 2 * (idx - 1)) : 0, en = u16(4 + 2 * idx);
+    if (en < st || base + en > kPageSize) throw std::runtime_error("corrupt string page");
+    return std::string(reinterpret_cast<const char*>(pin + base + st), en - st);
 }
 
 std::shared_ptr<Samples> Context::take_samples(const Query& q, const std::vector<const DevTable*>& inputs) {
@@ -1799,7 +2051,9 @@ std::shared_ptr<Samples> Context::take_samples(const Query& q, const std::vector
         for (uint32_t i = 0; i < size; ++i) ids[i] = sm_mix(seed + (i + 1) * kGolden) % rows;
         std::sort(ids.begin(), ids.end());
         for (uint32_t e : q.rels[r].eqs) {
-            if (!q.is_join_eq(e)) continue;
+            bool restricted = false;
+            for (const QFilter& f : q.filters) restricted = restricted || (q.attrs[f.attr].rel == r && q.attrs[f.attr].eq == e);
+            if (!q.is_join_eq(e) && !restricted) continue;
             const DevColumn& c = inputs[q.rels[r].table]->cols[q.attrs[q.attr_of(r, e)].col];
             cols.push_back(SampleCol{c.view(0), uint32_t(rowids.size()), size});
             rowids.insert(rowids.end(), ids.begin(), ids.end());
@@ -1824,7 +2078,11 @@ std::shared_ptr<Samples> Context::take_samples(const Query& q, const std::vector
             const auto [r, e] = where[k];
             std::vector<uint64_t> v(out + cols[k].first, out + cols[k].first + cols[k].n);
             for (uint32_t i = 0; i < v.size(); ++i)
-                if (v[i] == kNull) sm->matches[r] &= ~(1ull << i); // NULL never joins
+                if (v[i] == kNull) sm->matches[r] &= ~(1ull << i); // NULL never joins or passes a restriction
+            for (const QFilter& f : q.filters) // the scan's restrictions
+                if (q.attrs[f.attr].rel == r && q.attrs[f.attr].eq == e)
+                    for (uint32_t i = 0; i < v.size(); ++i)
+                        if (!f.pass(v[i])) sm->matches[r] &= ~(1ull << i);
             sm->keys[r][e] = std::move(v);
         }
     }
@@ -1843,7 +2101,10 @@ std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const st
     const EstimateReport& rep = pq.rep;
     Executor              ex(*this, q, pq.tree, inputs);
     std::unique_ptr<DevResult> res;
-    if (!options.incremental) {
+    if (pq.empty) {
+        res = std::make_unique<DevResult>();
+        for (uint32_t a : q.out) res->types.push_back(q.attrs[a].type);
+    } else if (!options.incremental) {
         res = ex.run(count_only);
     } else {
         // Vertex cardinalities for re-planning. The max-of-inputs rule (sampled
@@ -1880,7 +2141,16 @@ std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const st
         };
         res = ex.run_incremental(count_only, vcard, smp);
     }
+    if (pq.singletons) { // constant columns and the output order
+        res->pool_types = res->types;
+        res->types      = pq.types;
+        res->out_pos    = pq.out_pos;
+        res->consts     = pq.consts;
+    }
     stats                   = cgj_stats{};
+    stats.singletons        = pq.singletons;
+    stats.restrictions      = ex.restrictions;
+    stats.semijoins         = ex.semijoins;
     stats.ms_plan           = pq.ms_plan + ex.ms_replan;
     stats.ms_stats          = rep.ms_stats;
     stats.est_sets          = rep.sets;
@@ -1914,15 +2184,69 @@ std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const st
     return res;
 }
 
+namespace {
+// `n` rows of one value as contest pages (the library's page writer, in
+// chunks of bounded size).
+void encode_constant(const ConstOut& c, uint64_t n, cgj_result_column& out) {
+    out.type      = c.type;
+    out.num_pages = 0;
+    std::vector<void*> pages;
+    const bool         null = c.value == kNull;
+    const uint64_t     step = c.type == T_VARCHAR ? std::max<uint64_t>(1, (uint64_t(32) << 20) / (c.str.size() + 1))
+                                                  : (uint64_t(1) << 20);
+    for (uint64_t done = 0; done < n; done += step) {
+        const uint64_t       m = std::min(step, n - done);
+        std::vector<uint8_t> valid(m, null ? 0 : 1);
+        cgj_result_column    part{};
+        int                  rc;
+        if (c.type == T_VARCHAR) {
+            const uint64_t        len = null ? 0 : c.str.size();
+            std::vector<uint64_t> offs(m + 1);
+            std::string           chars;
+            chars.reserve(m * len);
+            for (uint64_t i = 0; i < m; ++i) {
+                offs[i] = chars.size();
+                if (!null) chars += c.str;
+            }
+            offs[m] = chars.size();
+            rc      = cgj_encode_column(c.type, offs.data(), chars.data(), valid.data(), m, &part);
+        } else if (c.type == T_INT32) {
+            std::vector<int32_t> v(m, int32_t(uint32_t(c.value)));
+            rc = cgj_encode_column(c.type, v.data(), nullptr, valid.data(), m, &part);
+        } else {
+            std::vector<uint64_t> v(m, null ? 0 : c.value);
+            rc = cgj_encode_column(c.type, v.data(), nullptr, valid.data(), m, &part);
+        }
+        if (rc != 0) throw std::runtime_error("constant column: page encoding failed");
+        for (uint64_t k = 0; k < part.num_pages; ++k) pages.push_back(part.pages[k]);
+        delete[] part.pages;
+    }
+    out.num_pages = pages.size();
+    out.pages     = new void*[pages.empty() ? 1 : pages.size()];
+    for (size_t k = 0; k < pages.size(); ++k) out.pages[k] = pages[k];
+}
+} // namespace
+
 void Context::download(const DevResult& r, cgj_result& out) {
     PoolView v;
     v.nrows          = r.nrows;
-    v.types          = r.types;
+    v.types          = r.out_pos.empty() && r.consts.empty() ? r.types : r.pool_types;
     v.pool           = r.pool.get<uint8_t>();
     v.pool_pages     = r.pool_pages;
     v.reserved_pages = r.reserved_pages;
     v.total_pages    = r.total_pages;
-    download_pool(v, out);
+    if (r.out_pos.empty() && r.consts.empty()) {
+        download_pool(v, out);
+        return;
+    }
+    cgj_result pool{};
+    download_pool(v, pool);
+    out.num_rows    = r.nrows;
+    out.num_columns = uint32_t(r.types.size());
+    out.columns     = new cgj_result_column[r.types.size()]();
+    for (uint32_t j = 0; j < pool.num_columns; ++j) out.columns[r.out_pos[j]] = pool.columns[j];
+    delete[] pool.columns;
+    for (const ConstOut& c : r.consts) encode_constant(c, r.nrows, out.columns[c.pos]);
 }
 
 void Context::download_column(const DevColumn& c, cgj_result_column& out) {

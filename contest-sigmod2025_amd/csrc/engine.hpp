@@ -157,6 +157,12 @@ struct DevTable {
 struct DevResult {
     uint64_t             nrows       = 0;
     std::vector<int32_t> types;
+    // Output columns whose rows all hold one value (removed one-row
+    // relations) are written on download: the pool holds the other columns,
+    // pool column j being output column out_pos[j] of type pool_types[j].
+    std::vector<ConstOut> consts;
+    std::vector<uint32_t> out_pos;    // empty: every column is in the pool, in order
+    std::vector<int32_t>  pool_types;
     uint64_t             total_pages    = 0;
     uint64_t             pool_pages     = 0;
     uint64_t             reserved_pages = 0; // directory entries to scan (valid flag in word 3)
@@ -219,6 +225,8 @@ struct Options {
     // Re-plan after every hash-table build with the tables built so far as
     // fixed build sides (the reference's incremental loop); 0: plan once.
     bool     incremental           = true;
+    bool     restrictions          = true; // incremental loop: restrict key producers after each build
+    bool     singletons            = true; // remove one-row relations before planning
 };
 
 // Tunables of one batched join.
@@ -336,6 +344,10 @@ public:
     // Up to 64 deterministic sample rows of every relation's join columns,
     // gathered on the device in one launch; NOT NULL applied.
     std::shared_ptr<Samples> take_samples(const Query& q, const std::vector<const DevTable*>& inputs);
+    // Slot values of every attribute of the one-row relations (others: empty).
+    std::vector<std::vector<uint64_t>> singleton_values(const Query& q, const std::vector<const DevTable*>& inputs);
+    // The bytes of the string a VARCHAR slot (string handle) of column c names.
+    std::string read_string(const DevColumn& c, uint64_t handle);
 
 private:
     void check_types(const Query& q, const std::vector<const DevTable*>& inputs) const;

@@ -483,7 +483,7 @@ hipError_t launch_part_build(const uint64_t* stage, uint64_t n, uint64_t n_real,
 // Decodes the tuple words of a 64-row block; returns active and no NULL in a
 // `notnull` word (the key, word 0, is always one).
 __device__ inline bool scan_build_row(const ScanBuildArgs& a, uint64_t blk, bool active, uint64_t (&v)[kMaxWords]) {
-    bool valid = active;
+    bool valid = active && (!a.sel || ((load_invariant_u64(a.sel + blk) >> lane_id()) & 1));
 #pragma unroll
     for (int w = 0; w < kMaxWords; ++w) {
         v[w] = kNull;
@@ -506,6 +506,7 @@ __global__ __launch_bounds__(1024) void k_scan_hist(ScanBuildArgs a, uint32_t* b
         const bool     active = b * 64 + lane_id() < a.nrows;
         const uint64_t key    = decode_block(a.col[0], b, active);
         bool           valid  = active && key != kNull;
+        if (a.sel) valid = valid && ((load_invariant_u64(a.sel + b) >> lane_id()) & 1);
         for (uint32_t w = 1; w < a.words; ++w) // NULL checks of the other join columns
             if ((a.notnull >> w) & 1u) valid = valid && decode_block(a.col[w], b, active) != kNull;
         if (valid) atomicAdd(&hist[a.rshift >= 64 ? 0u : uint32_t(hash_key(key) >> a.rshift)], 1u);
@@ -1282,6 +1283,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4
 #pragma unroll
             for (int k = 0; k < NA; ++k) s[k] = kNull;
             bool alive = active;
+            if (a.driver == DRIVER_PAGES && a.sel) { // the scan's restrictions
+                const uint64_t w = load_invariant_u64(a.sel + rb / 64);
+                if (!w) continue;
+                alive = alive && ((w >> lane) & 1);
+            }
             if (a.driver == DRIVER_PAGES) {
                 // Bloom cascade on keys that come straight from the scan, each
                 // key column decoded just before its filter and only while a
@@ -1736,6 +1742,7 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                 for (int c = 0; c < NO; ++c) v[c] = (a.out[c].src == SRC_SCAN && active) ? t[a.out[c].word] : kNull;
             }
             bool           alive = active && !hole && key != kNull;
+            if (a.driver == DRIVER_PAGES && a.sel) alive = alive && ((load_invariant_u64(a.sel + blk) >> lane) & 1);
             const uint64_t h     = hash_key(key);
             if (alive && a.bloom) {
                 uint64_t h2 = hash_bloom(h);
@@ -2113,6 +2120,7 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
     l.page_chunk = a.page_chunk ? a.page_chunk : kPageChunk;
     l.aligned_pairs = aligned_pairs && P.words <= 2 ? 1u : 0u;
     l.abort_pages   = a.abort_pages;
+    l.sel           = a.sel;
     // With the dynamic morsel schedule a grid larger than one resident wave
     // set only adds a tail: cap it at what the kernel's occupancy fits.
     static const void* const kernels[kLeanOut + 1] = {
@@ -2197,6 +2205,76 @@ hipError_t launch_sample_gather(const SampleCol* cols, uint32_t ncols, const uin
                                 hipStream_t s) {
     if (ncols == 0) return hipSuccess;
     k_sample_gather<<<dim3(ncols), dim3(64), 0, s>>>(cols, rowids, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// scan restrictions: one wave per 64-row block, the ballot of the rows that
+// pass is the block's selection word
+// ---------------------------------------------------------------------------
+__device__ inline bool filter_pass(const FilterSpec& f, uint64_t v) {
+    if (v == kNull) return false;
+    switch (f.kind) {
+    case FLT_EQ: return v == f.a;
+    case FLT_EQ2: return v == f.a || v == f.b;
+    case FLT_RANGE: return v >= f.a && v <= f.b;
+    case FLT_TINY: return f.tiny[tiny_slot(v, f.tiny_hash, f.tiny_bits)] == v;
+    case FLT_BLOOM: {
+        const uint64_t h2 = hash_bloom(hash_key(v));
+        const uint64_t bb = bloom_bits(h2);
+        return (f.table[bloom_word(h2, f.shift)] & bb) == bb;
+    }
+    default: { // FLT_TABLE: walk to the key or an empty slot (tables hold an empty slot per region)
+        uint64_t p = hash_key(v) >> f.shift;
+        for (uint64_t step = 0; step <= f.mask; ++step) {
+            const uint64_t k = f.table[p * f.words];
+            if (k == v) return true;
+            if (k == kEmpty) return false;
+            p = (p & ~f.mask) | ((p + 1) & f.mask);
+        }
+        return false;
+    }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_filter_select(FilterArgs a, uint64_t* sel) {
+    const uint64_t nb     = (a.nrows + 63) / 64;
+    const uint64_t nwaves = uint64_t(gridDim.x) * (blockDim.x / 64);
+    for (uint64_t b = uint64_t(blockIdx.x) * (blockDim.x / 64) + wave_in_block(); b < nb; b += nwaves) {
+        const bool active = b * 64 + lane_id() < a.nrows;
+        bool       ok     = active;
+        for (uint32_t i = 0; i < a.n; ++i) {
+            if (!__ballot(ok)) break;
+            const uint64_t v = decode_block(a.f[i].col, b, active);
+            ok               = ok && filter_pass(a.f[i], v);
+        }
+        const uint64_t m = __ballot(ok);
+        if (lead_lane()) sel[b] = a.accumulate ? (sel[b] & m) : m;
+    }
+}
+
+hipError_t launch_filter_select(const FilterArgs& a, uint64_t* sel, hipStream_t s) {
+    const uint64_t nb = (a.nrows + 63) / 64;
+    if (nb == 0) return hipSuccess;
+    const uint64_t blocks = (nb + 3) / 4 < 8192 ? (nb + 3) / 4 : 8192;
+    k_filter_select<<<dim3(uint32_t(blocks)), dim3(256), 0, s>>>(a, sel);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_table_keys(const uint64_t* ht, uint64_t cap, uint32_t stride, uint64_t* out,
+                                                    uint32_t max, unsigned int* count) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < cap; i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t k = ht[i * stride];
+        if (k == kEmpty) continue;
+        const unsigned int at = atomicAdd(count, 1u);
+        if (at < max) out[at] = k;
+    }
+}
+
+hipError_t launch_table_keys(const uint64_t* ht, uint64_t cap, uint32_t stride, uint64_t* out, uint32_t max,
+                             unsigned int* count, hipStream_t s) {
+    const uint64_t blocks = (cap + 255) / 256 < 4096 ? (cap + 255) / 256 : 4096;
+    k_table_keys<<<dim3(uint32_t(blocks ? blocks : 1)), dim3(256), 0, s>>>(ht, cap, stride, out, max, count);
     return hipGetLastError();
 }
 

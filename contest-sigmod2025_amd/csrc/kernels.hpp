@@ -96,6 +96,8 @@ struct PipeArgs {
     uint32_t        page_chunk;  // pool pages reserved per atomic (1..kPageChunk)
     uint32_t        grid_per_cu; // host only: waves per CU of the launch (0 = default)
     uint32_t        prefilter;   // pages driver: test early bloom filters four blocks at a time first
+    const uint64_t* sel;         // pages driver: per 64-row block, the rows that pass the scan's
+                                 // restrictions (launch_filter_select), nullptr = all
     uint64_t        abort_pages; // a wave whose reservation starts at or past this many pool pages (or
                                  // staging slots) stops: the result cannot fit device memory
 };
@@ -144,6 +146,7 @@ struct LeanArgs {
     // entries a step then never spans two 64-byte lines. Entries of <= 2 words.
     uint32_t        aligned_pairs;
     uint64_t        abort_pages; // as PipeArgs::abort_pages
+    const uint64_t* sel;         // as PipeArgs::sel
 };
 
 // ---- batched joins: one persistent launch over a device work queue ----
@@ -271,6 +274,7 @@ struct ScanBuildArgs {
     uint64_t blocks_per_wg;
     uint32_t rshift;   // region = hash >> rshift (64: one region)
     uint32_t nregions;
+    const uint64_t* sel; // as PipeArgs::sel
 };
 hipError_t launch_scan_partition(const ScanBuildArgs& a, uint32_t blocks, uint32_t* bh, uint32_t* scan_scratch,
                                  uint64_t* total, uint64_t* parts, uint32_t threads, hipStream_t s);
@@ -308,6 +312,55 @@ hipError_t launch_sample_gather(const SampleCol* cols, uint32_t ncols, const uin
 // A built hash table as the sample re-filter sees it: plain entries of
 // `words` words (key first) or grouped two-word slots; linear probing wraps
 // inside regions of mask + 1 slots.
+// ---- scan restrictions (reference engine/storage/RestrictionLogic.cpp:40-447) ----
+// A restriction keeps the rows whose value in one column passes it; NULL never
+// passes. Kinds: equal to a, equal to a or b, a <= v <= b (unsigned 64-bit
+// slots: the reference's Lt/Gt/Between over contiguous keys), membership in a
+// direct-mapped table of up to 32 values (TinyTable; slot = value mod 2^bits,
+// Fibonacci hash, or folded Fibonacci hash), a table's bloom filter (Join), or
+// exact membership in a hash table (JoinPrecise).
+constexpr int kMaxFilters = 4;
+constexpr int kTinySlots  = 32;
+enum : uint32_t { FLT_EQ = 0, FLT_EQ2 = 1, FLT_RANGE = 2, FLT_TINY = 3, FLT_BLOOM = 4, FLT_TABLE = 5 };
+enum : uint32_t { TINY_MOD = 0, TINY_FIBO = 1, TINY_FIBO2 = 2 };
+struct FilterSpec {
+    ScanCol         col;
+    uint32_t        kind;
+    uint32_t        tiny_bits;  // FLT_TINY: 2^tiny_bits slots
+    uint32_t        tiny_hash;  // FLT_TINY: TINY_*
+    uint32_t        words;      // FLT_TABLE: words per slot
+    uint64_t        a, b;       // FLT_EQ / FLT_EQ2 / FLT_RANGE
+    const uint64_t* table;      // FLT_TABLE: slots; FLT_BLOOM: bloom words
+    uint64_t        mask;       // FLT_TABLE: region slots - 1
+    uint32_t        shift;      // FLT_TABLE: slot = hash_key(v) >> shift; FLT_BLOOM: log2(words)
+    uint32_t        pad;
+    uint64_t        tiny[kTinySlots]; // FLT_TINY: values, kNull in empty slots
+};
+struct FilterArgs {
+    uint64_t   nrows;
+    uint32_t   n;          // restrictions (<= kMaxFilters)
+    uint32_t   accumulate; // AND into the existing selection words
+    FilterSpec f[kMaxFilters];
+};
+// sel[b] = bit i set when row 64b + i passes every restriction (and, with
+// `accumulate`, passed before).
+hipError_t launch_filter_select(const FilterArgs& a, uint64_t* sel, hipStream_t s);
+// Keys of the non-empty slots of a hash table (slot stride `stride` words):
+// out[i] for i < min(*count, max), *count = keys found (zeroed by the caller).
+hipError_t launch_table_keys(const uint64_t* ht, uint64_t cap, uint32_t stride, uint64_t* out, uint32_t max,
+                             unsigned int* count, hipStream_t s);
+// Slot of value v in a tiny table (reference RestrictionLogic.cpp:355-382).
+__host__ __device__ inline uint32_t tiny_slot(uint64_t v, uint32_t hash, uint32_t bits) {
+    switch (hash) {
+    case TINY_FIBO: return uint32_t((v * 11400714819323198485ull) >> (64 - bits));
+    case TINY_FIBO2: {
+        const uint64_t m = v * 11400714819323198485ull;
+        return uint32_t(m ^ (m >> 32)) >> (32 - bits);
+    }
+    default: return uint32_t(v & ((1ull << bits) - 1));
+    }
+}
+
 struct SampleTable {
     const uint64_t* ht;
     uint64_t        mask;

@@ -2,7 +2,9 @@
 #include "planner.hpp"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -149,6 +151,178 @@ Query import_plan(const cgj_plan& plan, const std::vector<uint64_t>& table_rows,
     return q;
 }
 
+bool QFilter::pass(uint64_t v) const {
+    if (v == ~0ull) return false; // NULL
+    switch (kind) {
+    case FLT_EQ: return v == a;
+    case FLT_EQ2: return v == a || v == b;
+    case FLT_RANGE: return v >= a && v <= b;
+    case FLT_TINY: return tiny[tiny_slot(v, tiny_hash, tiny_bits)] == v;
+    default: throw std::runtime_error("restriction kind not evaluated on the host");
+    }
+}
+
+bool small_restriction(std::vector<uint64_t> keys, uint32_t attr, QFilter& out) {
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    if (keys.empty() || keys.size() > 32) return false;
+    out      = QFilter{};
+    out.attr = attr;
+    if (keys.size() == 1) {
+        out.kind = FLT_EQ;
+        out.a    = keys[0];
+        return true;
+    }
+    if (keys.size() == 2) {
+        out.kind = FLT_EQ2;
+        out.a    = keys[0];
+        out.b    = keys[1];
+        return true;
+    }
+    if (keys.back() - keys.front() == keys.size() - 1) { // a contiguous run
+        out.kind = FLT_RANGE;
+        out.a    = keys.front();
+        out.b    = keys.back();
+        return true;
+    }
+    if (keys.size() > 16) return false;
+    static const std::pair<uint32_t, uint32_t> kTry[] = {{3, TINY_MOD},  {4, TINY_MOD},   {5, TINY_MOD},  {4, TINY_FIBO},
+                                                         {5, TINY_FIBO}, {4, TINY_FIBO2}, {5, TINY_FIBO2}};
+    for (auto [bits, hash] : kTry) {
+        std::vector<uint64_t> slots(size_t(1) << bits, ~0ull);
+        bool                  ok = true;
+        for (uint64_t k : keys) {
+            uint64_t& sl = slots[tiny_slot(k, hash, bits)];
+            if (sl != ~0ull) {
+                ok = false;
+                break;
+            }
+            sl = k;
+        }
+        if (!ok) continue;
+        out.kind      = FLT_TINY;
+        out.tiny_bits = bits;
+        out.tiny_hash = hash;
+        out.tiny      = std::move(slots);
+        return true;
+    }
+    return false;
+}
+
+SingletonReduction eliminate_singletons(const Query& q, const std::vector<std::vector<uint64_t>>& values) {
+    SingletonReduction red;
+    const uint32_t     n = uint32_t(q.rels.size());
+    for (uint32_t a : q.out) red.types.push_back(q.attrs[a].type);
+    const auto adj       = q.adjacency();
+    auto       connected = [&](uint64_t set) {
+        if (!set) return false;
+        uint64_t seen = set & (~set + 1), frontier = seen;
+        while (frontier) {
+            uint64_t next = 0;
+            for (uint64_t t = frontier; t; t &= t - 1) next |= adj[__builtin_ctzll(t)];
+            frontier = next & set & ~seen;
+            seen |= frontier;
+        }
+        return seen == set;
+    };
+    // remove one-row relations in order while the rest stays connected
+    uint64_t keep = q.all_rels();
+    for (uint32_t r = 0; r < n; ++r) {
+        if (q.rels[r].rows != 1 || r >= values.size() || values[r].size() != q.rels[r].attrs.size()) continue;
+        if (__builtin_popcountll(keep) <= 1) break;
+        const uint64_t rest = keep & ~(1ull << r);
+        if (connected(rest)) keep = rest;
+    }
+    red.removed.assign(n, -1);
+    if (keep == q.all_rels()) {
+        red.q = q;
+        for (uint32_t r = 0; r < n; ++r) red.removed[r] = int32_t(r);
+        for (uint32_t i = 0; i < q.out.size(); ++i) red.out_pos.push_back(i);
+        return red;
+    }
+    // the constants of the removed relations' classes
+    std::map<uint32_t, uint64_t> cst;
+    for (uint32_t r = 0; r < n; ++r) {
+        if ((keep >> r) & 1) continue;
+        ++red.eliminated;
+        for (size_t i = 0; i < q.rels[r].attrs.size(); ++i) {
+            const uint32_t e = q.attrs[q.rels[r].attrs[i]].eq;
+            const uint64_t v = values[r][i];
+            if (q.is_join_eq(e) && v == ~0ull) red.empty = true; // NULL never joins
+            auto it = cst.find(e);
+            if (it != cst.end() && it->second != v) red.empty = true;
+            cst[e] = v;
+        }
+    }
+    // the remaining relations, their attributes and classes
+    Query&                s = red.q;
+    std::vector<int32_t>  eqmap(q.eq_attrs.size(), -1), attrmap(q.attrs.size(), -1);
+    for (uint32_t r = 0; r < n; ++r)
+        if ((keep >> r) & 1) {
+            red.removed[r] = int32_t(s.rels.size());
+            QRel nr;
+            nr.table = q.rels[r].table;
+            nr.rows  = q.rels[r].rows;
+            s.rels.push_back(nr);
+        }
+    for (uint32_t a = 0; a < q.attrs.size(); ++a) {
+        const QAttr& at = q.attrs[a];
+        if (red.removed[at.rel] < 0) continue;
+        if (eqmap[at.eq] < 0) {
+            eqmap[at.eq] = int32_t(s.eq_attrs.size());
+            s.eq_attrs.emplace_back();
+        }
+        attrmap[a] = int32_t(s.attrs.size());
+        s.attrs.push_back(QAttr{uint32_t(red.removed[at.rel]), at.col, at.type, uint32_t(eqmap[at.eq])});
+        s.eq_attrs[size_t(eqmap[at.eq])].push_back(uint32_t(attrmap[a]));
+        s.rels[size_t(red.removed[at.rel])].attrs.push_back(uint32_t(attrmap[a]));
+        s.rels[size_t(red.removed[at.rel])].eqs.push_back(uint32_t(eqmap[at.eq]));
+    }
+    for (auto& r : s.rels) {
+        std::sort(r.eqs.begin(), r.eqs.end());
+        r.eqs.erase(std::unique(r.eqs.begin(), r.eqs.end()), r.eqs.end());
+    }
+    // outputs: from a remaining member of the class, else a constant column
+    for (uint32_t i = 0; i < q.out.size(); ++i) {
+        const uint32_t a = q.out[i];
+        int32_t        m = attrmap[a];
+        for (size_t k = 0; m < 0 && k < q.eq_attrs[q.attrs[a].eq].size(); ++k)
+            m = attrmap[q.eq_attrs[q.attrs[a].eq][k]];
+        if (m >= 0) {
+            s.out.push_back(uint32_t(m));
+            red.out_pos.push_back(i);
+            continue;
+        }
+        ConstOut c;
+        c.pos   = i;
+        c.type  = q.attrs[a].type;
+        c.rel   = q.attrs[a].rel;
+        c.slot  = uint32_t(std::find(q.rels[c.rel].attrs.begin(), q.rels[c.rel].attrs.end(), a) -
+                          q.rels[c.rel].attrs.begin());
+        c.value = values[c.rel][c.slot];
+        red.consts.push_back(c);
+    }
+    // restrictions: the existing ones of remaining relations, and each class
+    // constant on every remaining member
+    for (const QFilter& f : q.filters)
+        if (attrmap[f.attr] >= 0) {
+            QFilter g = f;
+            g.attr    = uint32_t(attrmap[f.attr]);
+            s.filters.push_back(g);
+        }
+    for (uint32_t a = 0; a < q.attrs.size(); ++a) {
+        auto it = cst.find(q.attrs[a].eq);
+        if (attrmap[a] < 0 || it == cst.end()) continue;
+        QFilter f;
+        f.attr = uint32_t(attrmap[a]);
+        f.kind = FLT_EQ;
+        f.a    = it->second;
+        if (it->second == ~0ull) red.empty = true; // a remaining member equal to NULL: no row
+        s.filters.push_back(f);
+    }
+    return red;
+}
+
 Query restrict_query(const Query& q, uint64_t relset) {
     const uint32_t n = uint32_t(q.rels.size());
     if (relset == 0 || (n < 64 && (relset >> n) != 0)) throw std::runtime_error("relation set out of range");
@@ -204,24 +378,49 @@ namespace {
 inline uint64_t low_mask(uint32_t v) { return v >= 63 ? ~0ull : ((1ull << (v + 1)) - 1); } // bits 0..v
 
 struct Dp {
-    const Query&                      q;
-    const std::vector<PlanVertex>&    vx;
-    const CardFn&                     card; // over vertex masks
-    std::vector<uint64_t>             adj;  // vertex adjacency
-    PlanTree                          tree;
-    std::unordered_map<uint64_t, int> best;
-    std::unordered_map<uint64_t, double> cards;
+    const Query&                   q;
+    const std::vector<PlanVertex>& vx;
+    const CardFn&                  card; // over vertex masks
+    std::vector<uint64_t>          adj;  // vertex adjacency
+    PlanTree                       tree;
+    // Per vertex set: the cheapest plan's cost and build side (the probe side
+    // is the rest), and the set's cardinality. A dense array indexed by the
+    // set up to kDenseVertices vertices, a hash map above. The tree is built
+    // from the root's entry once the search is done.
+    struct Entry {
+        double   cost  = std::numeric_limits<double>::infinity();
+        double   card  = -1.0;
+        uint64_t build = 0;
+    };
+    static constexpr uint32_t                 kDenseVertices = 18;
+    bool                                      dense;
+    std::vector<Entry>                        dense_;
+    std::unordered_map<uint64_t, Entry>       sparse_;
     std::vector<std::pair<uint64_t, uint64_t>> pairs;
-    size_t                            budget = 4u << 20;
-    bool                              overflow = false;
+    size_t                                    budget   = 4u << 20;
+    bool                                      overflow = false;
 
-    Dp(const Query& q_, const std::vector<PlanVertex>& v, const CardFn& c) : q(q_), vx(v), card(c), adj(v.size(), 0) {
+    Dp(const Query& q_, const std::vector<PlanVertex>& v, const CardFn& c)
+        : q(q_), vx(v), card(c), adj(v.size(), 0), dense(v.size() <= kDenseVertices) {
         for (size_t i = 0; i < vx.size(); ++i)
             for (size_t j = i + 1; j < vx.size(); ++j)
                 if (!q.shared(vx[i].set, vx[j].set).empty()) {
                     adj[i] |= 1ull << j;
                     adj[j] |= 1ull << i;
                 }
+        if (dense) {
+            dense_.resize(size_t(1) << vx.size());
+            // neighbourhood of every vertex set, from the set without its lowest vertex
+            nb_.resize(size_t(1) << vx.size());
+            for (uint64_t t = 1; t < nb_.size(); ++t) nb_[t] = nb_[t & (t - 1)] | adj[__builtin_ctzll(t)];
+        }
+    }
+    std::vector<uint64_t> nb_; // dense only: union of the members' adjacency
+    Entry& entry(uint64_t s) { return dense ? dense_[s] : sparse_[s]; }
+    const Entry* find(uint64_t s) const {
+        if (dense) return &dense_[s];
+        auto it = sparse_.find(s);
+        return it == sparse_.end() ? nullptr : &it->second;
     }
 
     uint64_t base(uint64_t vmask) const {
@@ -230,16 +429,15 @@ struct Dp {
         return s;
     }
     uint64_t neigh(uint64_t s) const {
+        if (dense) return nb_[s] & ~s;
         uint64_t n = 0;
         for (uint64_t t = s; t; t &= t - 1) n |= adj[__builtin_ctzll(t)];
         return n & ~s;
     }
     double card_of(uint64_t s) {
-        auto it = cards.find(s);
-        if (it != cards.end()) return it->second;
-        double c = std::max(card(s), 0.0);
-        cards.emplace(s, c);
-        return c;
+        Entry& e = entry(s);
+        if (e.card < 0) e.card = std::max(card(s), 0.0);
+        return e.card;
     }
     // A built table (a single vertex with a key) may only be a build side,
     // joined on its key class.
@@ -285,36 +483,27 @@ struct Dp {
         for (uint64_t t = n; t; t = (t - 1) & n) csg_rec(s | t, x | n);
     }
 
+    // Cost C(T) = C(B) + C(P) + |T| + 2|B| of building b and probing it with p.
     void consider(uint64_t b, uint64_t p) {
         if (!can_join(b, p)) return;
-        auto ib = best.find(b), ip = best.find(p);
-        if (ib == best.end() || ip == best.end()) return;
-        uint64_t        t  = b | p;
-        const TreeNode& nb = tree.nodes[ib->second];
-        const TreeNode& np = tree.nodes[ip->second];
-        double          c  = nb.cost + np.cost + card_of(t) + 2.0 * nb.card;
-        auto            it = best.find(t);
-        if (it != best.end() && tree.nodes[it->second].cost <= c) return;
-        TreeNode n;
-        n.build = ib->second;
-        n.probe = ip->second;
-        n.set   = nb.set | np.set;
-        n.card  = card_of(t);
-        n.cost  = c;
-        tree.nodes.push_back(n);
-        best[t] = int(tree.nodes.size() - 1);
+        const Entry* eb = find(b);
+        const Entry* ep = find(p);
+        if (!eb || !ep || eb->cost == std::numeric_limits<double>::infinity() ||
+            ep->cost == std::numeric_limits<double>::infinity())
+            return;
+        const double cb = eb->cost, cp = ep->cost, nb = eb->card;
+        const uint64_t t = b | p;
+        const double   c = cb + cp + card_of(t) + 2.0 * nb;
+        Entry&         e = entry(t);
+        if (e.cost <= c) return;
+        e.cost  = c;
+        e.build = b;
     }
 
     void leaves() {
         for (uint32_t v = 0; v < vx.size(); ++v) {
-            TreeNode n;
-            n.rel   = vx[v].rel;
-            n.group = vx[v].group;
-            n.set   = vx[v].set;
-            n.card  = card_of(1ull << v);
-            n.cost  = 0;
-            tree.nodes.push_back(n);
-            best[1ull << v] = int(tree.nodes.size() - 1);
+            card_of(1ull << v);
+            entry(1ull << v).cost = 0;
         }
     }
 
@@ -361,13 +550,46 @@ struct Dp {
             greedy();
             return;
         }
-        std::stable_sort(pairs.begin(), pairs.end(), [](const auto& x, const auto& y) {
-            return __builtin_popcountll(x.first | x.second) < __builtin_popcountll(y.first | y.second);
-        });
-        for (auto& pr : pairs) {
+        // smaller sets first, so both halves of a pair are planned before it
+        // (a stable counting sort by the pair's vertex count)
+        std::vector<size_t> start(n + 2, 0);
+        for (auto& pr : pairs) ++start[size_t(__builtin_popcountll(pr.first | pr.second)) + 1];
+        for (uint32_t k = 1; k <= n + 1; ++k) start[k] += start[k - 1];
+        std::vector<std::pair<uint64_t, uint64_t>> sorted(pairs.size());
+        for (auto& pr : pairs) sorted[start[size_t(__builtin_popcountll(pr.first | pr.second))]++] = pr;
+        pairs.clear();
+        pairs.shrink_to_fit();
+        for (auto& pr : sorted) {
             consider(pr.first, pr.second);
             consider(pr.second, pr.first);
         }
+    }
+
+    // The plan of vertex set s as tree nodes (children first); -1 if none.
+    int emit_tree(uint64_t s) {
+        const Entry* e = find(s);
+        if (!e || e->cost == std::numeric_limits<double>::infinity()) return -1;
+        TreeNode n;
+        n.card = e->card;
+        n.cost = e->cost;
+        if (!(s & (s - 1))) {
+            const PlanVertex& v = vx[__builtin_ctzll(s)];
+            n.rel   = v.rel;
+            n.group = v.group;
+            n.set   = v.set;
+        } else {
+            const uint64_t b = e->build;
+            n.build          = emit_tree(b);
+            n.probe          = emit_tree(s ^ b);
+            if (n.build < 0 || n.probe < 0) return -1;
+            n.set = tree.nodes[size_t(n.build)].set | tree.nodes[size_t(n.probe)].set;
+            // emit_tree may have grown the node vector: the entry is re-read
+            const Entry* e2 = find(s);
+            n.card          = e2->card;
+            n.cost          = e2->cost;
+        }
+        tree.nodes.push_back(n);
+        return int(tree.nodes.size() - 1);
     }
 };
 
@@ -378,10 +600,10 @@ PlanTree plan_joins_vertices(const Query& q, const std::vector<PlanVertex>& vert
     if (vertices.size() > 64) throw std::runtime_error("more than 64 join-graph vertices");
     Dp dp(q, vertices, card);
     dp.run();
-    const uint64_t all = vertices.size() >= 64 ? ~0ull : ((1ull << vertices.size()) - 1);
-    auto           it  = dp.best.find(all);
-    if (it == dp.best.end()) throw std::runtime_error("join graph is not connected");
-    dp.tree.root = it->second;
+    const uint64_t all  = vertices.size() >= 64 ? ~0ull : ((1ull << vertices.size()) - 1);
+    const int      root = dp.emit_tree(all);
+    if (root < 0) throw std::runtime_error("join graph is not connected");
+    dp.tree.root = root;
     return std::move(dp.tree);
 }
 

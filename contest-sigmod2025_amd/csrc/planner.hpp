@@ -6,6 +6,7 @@
 #include <map>
 #include <memory>
 #include <set>
+#include <string>
 #include <vector>
 
 #include "../../include/gpujoin.h"
@@ -30,11 +31,35 @@ struct QRel {
     std::vector<uint32_t> eqs; // classes produced (sorted, unique)
 };
 
+// A restriction of one scanned attribute with values known on the host
+// (reference engine/query/Restriction.hpp; the kinds of
+// engine/storage/RestrictionLogic.cpp:40-447): FLT_EQ (v == a), FLT_EQ2
+// (v == a or b), FLT_RANGE (a <= v <= b), FLT_TINY (v in a direct-mapped table
+// of 2^tiny_bits slots). NULL never passes. The FLT_* and TINY_* codes are
+// those of the device restrictions (kernels.hpp).
+struct QFilter {
+    uint32_t              attr = 0;
+    uint32_t              kind = 0;
+    uint64_t              a = 0, b = 0;
+    uint32_t              tiny_bits = 0, tiny_hash = 0;
+    std::vector<uint64_t> tiny; // 2^tiny_bits slots, ~0 = empty
+    bool                  pass(uint64_t v) const;
+};
+
+// The restriction that keeps exactly the given keys (at most 32 distinct),
+// chosen as the reference's setupRestriction does (RestrictionLogic.cpp:397-447):
+// one key -> FLT_EQ, two -> FLT_EQ2, a contiguous run -> FLT_RANGE, up to 16
+// -> the first collision-free tiny table of 8/16/32 slots by value (then 16/32
+// slots by Fibonacci hash, then by folded Fibonacci hash). False when none of
+// these applies (more than 32 keys, or no tiny table fits).
+bool small_restriction(std::vector<uint64_t> keys, uint32_t attr, QFilter& out);
+
 struct Query {
     std::vector<QRel>                  rels;
     std::vector<QAttr>                 attrs;
     std::vector<std::vector<uint32_t>> eq_attrs; // class -> attributes
     std::vector<uint32_t>              out;      // root output attributes
+    std::vector<QFilter>               filters;  // restrictions of scanned attributes
 
     bool     is_join_eq(uint32_t eq) const { return eq_attrs[eq].size() > 1; }
     uint64_t all_rels() const { return rels.size() >= 64 ? ~0ull : ((1ull << rels.size()) - 1); }
@@ -135,9 +160,45 @@ struct Samples {
     }
 };
 
+// An output column whose every row holds one value: an attribute of a
+// relation removed as a singleton (one row) whose class no remaining
+// relation produces. VARCHAR values carry their bytes.
+struct ConstOut {
+    uint32_t    pos   = 0; // position among the query's output columns
+    uint32_t    rel   = 0; // the removed relation and its attribute (index in its attrs) that holds it
+    uint32_t    slot  = 0;
+    int32_t     type  = 0;
+    uint64_t    value = ~0ull; // the slot value (~0 = NULL)
+    std::string str;           // VARCHAR: the string
+};
+
+// Singleton elimination (reference engine/query/QueryPlan.cpp:680-717): every
+// relation of one row is removed while more than one relation remains and the
+// others stay connected; its values become constants of their classes. A
+// class with a constant restricts every remaining member to that value
+// (FLT_EQ); an output attribute whose class keeps no member becomes a constant
+// column. `values[r][i]` = the slot of attribute q.rels[r].attrs[i] for each
+// singleton r. The result is empty when a joined value is NULL or two
+// constants of one class differ.
+struct SingletonReduction {
+    Query                 q;          // the remaining relations (outputs without the constants)
+    std::vector<ConstOut> consts;
+    std::vector<uint32_t> out_pos;    // column j of q.out -> position among the original outputs
+    std::vector<int32_t>  types;      // the original output types
+    std::vector<int32_t>  removed;    // original relation -> -1 (removed) or its index in q
+    bool                  empty = false;
+    uint32_t              eliminated = 0;
+};
+SingletonReduction eliminate_singletons(const Query& q, const std::vector<std::vector<uint64_t>>& values);
+
 // A query and the join tree chosen for it.
 struct PlannedQuery {
     Query                    q;
+    std::vector<ConstOut>    consts;     // constant output columns (singleton elimination)
+    std::vector<uint32_t>    out_pos;    // q.out column -> output position (empty: identity)
+    std::vector<int32_t>     types;      // output types (all columns)
+    bool                     empty = false; // the result is known to be empty
+    uint32_t                 singletons = 0;
     PlanTree                 tree;
     EstimateReport           rep;
     double                   ms_plan = 0; // estimation + join ordering, statistics counts excluded

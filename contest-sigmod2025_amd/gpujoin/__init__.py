@@ -100,7 +100,9 @@ class Stats(C.Structure):
         ("ms_host_upload", C.c_double),
         ("ms_host_download", C.c_double),
         ("replans", C.c_uint32),
-        ("pad0", C.c_uint32),
+        ("singletons", C.c_uint32),
+        ("restrictions", C.c_uint32),
+        ("semijoins", C.c_uint32),
     ]
 
     def as_dict(self):

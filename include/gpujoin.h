@@ -129,7 +129,12 @@ typedef struct {
                                    one operator-new block per page */
     uint32_t replans;           /* join orderings computed (one per hash-table build of the
                                    incremental loop, plus the output pipeline's) */
-    uint32_t pad0;
+    uint32_t singletons;        /* one-row relations removed before planning (their values
+                                   restrict the other members of their classes) */
+    uint32_t restrictions;      /* scan restrictions applied (singleton constants; keys of
+                                   built tables of at most 32 tuples; semi-join tables) */
+    uint32_t semijoins;         /* built tables used only as an exact restriction of their
+                                   key's producers (duplicate-free, key only) */
 } cgj_stats;
 
 /* Execution options of a context (0 = success, -1 = unknown name):
@@ -156,7 +161,15 @@ typedef struct {
  *       inserted straight from the pages by one kernel (default 8 MiB, 0 = never);
  *   "incremental": re-plan after every hash-table build, with the tables built so far as
  *       fixed build sides and the samples re-filtered through them (1, default), or plan
- *       once (0). */
+ *       once (0);
+ *   "restrictions": in the incremental loop, a built table of at most 32 tuples restricts
+ *       the unjoined producers of its key by its keys (Eq, Eq2, range or tiny table), and a
+ *       duplicate-free table carrying only its key becomes an exact restriction of them
+ *       instead of a join input (reference engine/query/QueryPlan.cpp:602-625,
+ *       engine/storage/RestrictionLogic.cpp:397-447) (1, default) or neither (0);
+ *   "singletons": one-row relations are removed before planning and their values
+ *       restrict the other members of their classes (QueryPlan.cpp:680-717) (1, default)
+ *       or kept as relations (0). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
