@@ -17,6 +17,9 @@
 #include <cstring>
 #include <exception>
 #include <functional>
+#include <limits>
+#include <malloc.h>
+#include <mutex>
 #include <numeric>
 #include <thread>
 
@@ -184,7 +187,26 @@ uint64_t next_table_uid() {
     return next.fetch_add(1);
 }
 
+namespace {
+// Result pages are one operator-new block of 8 KiB each, freed by the caller
+// after use. glibc returns freed memory at the top of its heaps to the
+// kernel once it passes the trim threshold (128 KiB by default), so the next
+// result's pages are faulted in again, page by page: on c2 at 1/10 scale
+// (1 GB of result pages) that was 215 of 253 ms. With the threshold at its
+// maximum and the heap grown 256 MiB at a time, freed pages stay mapped and
+// the next result reuses them (45 ms). The reference keeps its pages for
+// reuse through a global operator new/delete (engine/infra/PageMemory.cpp:138-201).
+void keep_freed_pages() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)mallopt(M_TRIM_THRESHOLD, std::numeric_limits<int>::max());
+        (void)mallopt(M_TOP_PAD, 256 << 20);
+    });
+}
+} // namespace
+
 Context::Context(int device) : device_(device), est_(std::make_unique<Estimator>()) {
+    keep_freed_pages();
     CGJ_HIP(hipSetDevice(device));
     CGJ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     hipDeviceProp_t prop;
@@ -281,33 +303,79 @@ uint8_t* Context::sample_pinned() {
     return static_cast<uint8_t*>(sample_pin_);
 }
 
-namespace {
-// Runs f(i) for i in [0, n) on up to `max_threads` host threads (one below
-// 512 items): gathers and scatters of 8 KiB pages through the staging
-// buffers. The first exception of any thread is rethrown after all have
-// finished.
-void parallel_pages(uint64_t n, const std::function<void(uint64_t)>& f, uint32_t max_threads) {
-    const uint32_t nt =
-        n < 512 ? 1u : std::min<uint32_t>(max_threads, std::max(1u, std::thread::hardware_concurrency()));
-    if (nt == 1) {
+
+CopyPool::CopyPool(uint32_t threads) {
+    for (uint32_t t = 0; t + 1 < threads; ++t) th_.emplace_back([this] { work(); });
+}
+
+CopyPool::~CopyPool() {
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+}
+
+void CopyPool::drain() {
+    for (;;) {
+        const uint64_t i0 = next_.fetch_add(kGrain);
+        if (i0 >= n_) return;
+        const uint64_t i1 = std::min(n_, i0 + kGrain);
+        try {
+            for (uint64_t i = i0; i < i1; ++i) (*job_)(i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(mu_);
+            if (!err_) err_ = std::current_exception();
+            next_ = n_;
+        }
+    }
+}
+
+void CopyPool::work() {
+    uint64_t seen = 0;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lock(mu_);
+            cv_.wait(lock, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            ++active_;
+        }
+        drain();
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            if (--active_ == 0) done_.notify_all();
+        }
+    }
+}
+
+void CopyPool::run(uint64_t n, const std::function<void(uint64_t)>& f) {
+    if (n < 2 * kGrain || th_.empty()) {
         for (uint64_t i = 0; i < n; ++i) f(i);
         return;
     }
-    std::vector<std::thread>        th;
-    std::vector<std::exception_ptr> err(nt);
-    for (uint32_t t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-            try {
-                for (uint64_t i = n * t / nt; i < n * (t + 1) / nt; ++i) f(i);
-            } catch (...) {
-                err[t] = std::current_exception();
-            }
-        });
-    for (auto& x : th) x.join();
-    for (auto& e : err)
-        if (e) std::rethrow_exception(e);
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        job_  = &f;
+        n_    = n;
+        next_ = 0;
+        err_  = nullptr;
+        ++gen_;
+    }
+    cv_.notify_all();
+    drain(); // the caller works too
+    std::unique_lock<std::mutex> lock(mu_);
+    done_.wait(lock, [&] { return active_ == 0 && next_ >= n_; });
+    job_ = nullptr;
+    if (err_) std::rethrow_exception(err_);
 }
-} // namespace
+
+CopyPool& Context::copy_pool() {
+    if (!copy_pool_)
+        copy_pool_ = std::make_unique<CopyPool>(std::min<uint32_t>(16, std::max(1u, std::thread::hardware_concurrency())));
+    return *copy_pool_;
+}
 
 std::unique_ptr<DevTable> Context::upload(const cgj_host_table& t, const std::vector<bool>* used) {
     auto dt   = std::make_unique<DevTable>();
@@ -331,8 +399,7 @@ std::unique_ptr<DevTable> Context::upload(const cgj_host_table& t, const std::ve
             const uint64_t n   = std::min(per, c.npages - p0);
             const int      b   = chunk & 1;
             uint8_t*       buf = staging(b);
-            parallel_pages(
-                n, [&](uint64_t i) { std::memcpy(buf + i * kPageSize, hc.pages[p0 + i], kPageSize); }, 4);
+            copy_pool().run(n, [&](uint64_t i) { std::memcpy(buf + i * kPageSize, hc.pages[p0 + i], kPageSize); });
             CGJ_HIP(hipMemcpyAsync(c.pages.get<uint8_t>() + p0 * kPageSize, buf, n * kPageSize,
                                    hipMemcpyHostToDevice, stream_));
             staging_done(b);
@@ -2266,14 +2333,11 @@ void Context::download_column(const DevColumn& c, cgj_result_column& out) {
         if (k + 1 < nchunks) issue(k + 1);
         const uint8_t* buf = staging(int(k & 1)); // waits for chunk k's copy
         const uint64_t p0 = k * per, n = std::min(per, c.npages - p0);
-        parallel_pages(
-            n,
-            [&](uint64_t i) {
-                void* pg = ::operator new(kPageSize);
-                std::memcpy(pg, buf + i * kPageSize, kPageSize);
-                out.pages[p0 + i] = pg;
-            },
-            8);
+        copy_pool().run(n, [&](uint64_t i) {
+            void* pg = ::operator new(kPageSize);
+            std::memcpy(pg, buf + i * kPageSize, kPageSize);
+            out.pages[p0 + i] = pg;
+        });
     }
 }
 
@@ -2346,17 +2410,13 @@ void Context::download_pool(const PoolView& r, cgj_result& out) {
         if (k + 1 < nchunks) issue(k + 1);
         const uint8_t* buf = staging(int(k & 1)); // waits for chunk k's copy
         const uint64_t p0 = k * kChunkPages, n = std::min(kChunkPages, used - p0);
-        // fresh operator-new pages fault on first touch: more threads than the gather
-        parallel_pages(
-            n,
-            [&](uint64_t i) {
-                const auto w = where[p0 + i];
-                if (w.first == ~0u) return;
-                void* pg = ::operator new(kPageSize);
-                std::memcpy(pg, buf + i * kPageSize, kPageSize);
-                out.columns[w.first].pages[w.second] = pg;
-            },
-            8);
+        copy_pool().run(n, [&](uint64_t i) {
+            const auto w = where[p0 + i];
+            if (w.first == ~0u) return;
+            void* pg = ::operator new(kPageSize);
+            std::memcpy(pg, buf + i * kPageSize, kPageSize);
+            out.columns[w.first].pages[w.second] = pg;
+        });
     }
 }
 

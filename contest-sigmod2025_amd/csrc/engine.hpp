@@ -6,6 +6,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <thread>
+#include <functional>
+#include <exception>
+#include <condition_variable>
+#include <atomic>
 #include <list>
 #include <map>
 #include <memory>
@@ -182,6 +187,31 @@ struct PoolView {
     uint64_t             total_pages    = 0;
 };
 
+// Host threads copying pages between the pinned staging buffers and the
+// callers' pages: started once per context, each run split in grains of 64
+// pages taken in turn by the workers and the calling thread.
+class CopyPool {
+public:
+    explicit CopyPool(uint32_t threads);
+    ~CopyPool();
+    void run(uint64_t n, const std::function<void(uint64_t)>& f); // f(i) for i in [0, n)
+
+private:
+    static constexpr uint64_t                 kGrain = 64;
+    void                                      work();
+    void                                      drain();
+    std::vector<std::thread>                  th_;
+    std::mutex                                mu_;
+    std::condition_variable                   cv_, done_;
+    const std::function<void(uint64_t)>*      job_ = nullptr;
+    uint64_t                                  n_   = 0;
+    std::atomic<uint64_t>                     next_{0};
+    uint32_t                                  active_ = 0;
+    uint64_t                                  gen_    = 0;
+    bool                                      stop_   = false;
+    std::exception_ptr                        err_;
+};
+
 class Context;
 class Estimator;
 
@@ -298,6 +328,7 @@ public:
     // estimator's features and outputs), grown on demand and kept for the
     // context's lifetime.
     uint8_t* small_pinned(size_t bytes);
+    CopyPool& copy_pool();
     // Pinned host buffer of kSamplePinnedBytes for the sample re-filter of
     // the incremental loop (keys out, hit flags back), apart from
     // small_pinned so a re-filter queued ahead of a build's last
@@ -364,6 +395,7 @@ private:
     std::shared_ptr<BuildHints> hints_ = std::make_shared<BuildHints>();
     void*                       pin_[2]    = {nullptr, nullptr};
     hipEvent_t                  pin_ev_[2] = {nullptr, nullptr};
+    std::unique_ptr<CopyPool>   copy_pool_;
     void*                       spin_       = nullptr;
     void*                       sample_pin_ = nullptr;
     size_t                      spin_bytes_ = 0;
