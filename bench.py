@@ -609,6 +609,12 @@ def main():
         kernel_name = "k_lean (one-probe pipeline: page decode, hash probe, output pages)"
     else:
         kernel_name = "k_pipeline (fused probe chain: page decode, probes, output pages)"
+    notes = {}
+    if wl.name == "c2":
+        # the join tuner turns bloom filters on for builds of at most 16M keys
+        # probed by at least 8x as many rows (planner.cpp autotune_join)
+        notes["bloom"] = ("off: the build has 50M keys, above the 16M-key limit of the join tuner's bloom "
+                          "filters; the 10% missing probe keys walk the hash table to an empty slot")
     line = {
         "metric": METRIC,
         "value": round(tuples / (ms_step / 1000.0) / 1e9, 4),
@@ -628,6 +634,7 @@ def main():
             "output_rows": int(s["rows"]),
             "device_resident": True,
             "step": "index every scanned column (row prefix + block index), then run the query",
+            **notes,
         },
         "ms_prepare_per_step": round(1000.0 * sum(prep) / len(prep), 3),
         "hbm_frac_job": round(b_alg / (ms_step / 1000.0) / (HBM_PEAK_GBS * 1e9), 4),

@@ -1,7 +1,7 @@
 // Kernels of the cardinality estimator's MLP (x -> 256 -> 256 -> 2, ReLU):
 // FP32 training (tiled GEMM with fused bias/ReLU, backward helpers,
 // heteroscedastic loss, Adam) and batched inference with BF16 inputs and FP32
-// accumulation on the MFMA matrix cores (v_mfma_f32_32x32x8_bf16).
+// accumulation on the MFMA matrix cores (gfx950's v_mfma_f32_32x32x16_bf16).
 #include <hip/hip_runtime.h>
 
 #include "mlp.hpp"
@@ -160,31 +160,34 @@ __global__ void k_to_bf16(const float* src, uint16_t* dst, uint64_t n) {
     if (i < n) dst[i] = f32_to_bf16(src[i]);
 }
 
-typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
-typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float  f32x16 __attribute__((ext_vector_type(16)));
 
-// One 32-row block of activations (LDS, bf16, row stride `lds_stride`)
-// times a K x 256 bf16 weight matrix (global, row-major) for the 32-column
-// output tiles t of this wave; bias + ReLU, result to LDS as bf16.
-// MFMA 32x32x8 operand layout: lane l holds A[l % 32][4 (l / 32) + 0..3] and
-// B[4 (l / 32) + 0..3][l % 32]; accumulator item r is D[(r % 4) + 8 (r / 4) + 4 (l / 32)][l % 32].
-__device__ inline void mfma_layer(const uint16_t* act, uint32_t lds_stride, uint32_t K, const uint16_t* W,
-                                  const float* bias, uint16_t* out, uint32_t out_stride, uint32_t tile_first,
-                                  uint32_t tiles) {
-    const uint32_t lane = lane_id(), row = lane % 32, kq = 4 * (lane / 32);
+// One 32-row block of activations (LDS, bf16, row stride `lds_stride`, K
+// columns, zero past the real width) times a Kw x 256 bf16 weight matrix
+// (global, row-major; rows past Kw read as zero) for the 32-column output
+// tiles t of this wave; bias + ReLU, result to LDS as bf16.
+// gfx950's double-K MFMA 32x32x16: lane l holds A[l % 32][8 (l / 32) + 0..7]
+// and B[8 (l / 32) + 0..7][l % 32]; accumulator item r is
+// D[(r % 4) + 8 (r / 4) + 4 (l / 32)][l % 32].
+__device__ inline void mfma_layer(const uint16_t* act, uint32_t lds_stride, uint32_t K, uint32_t Kw,
+                                  const uint16_t* W, const float* bias, uint16_t* out, uint32_t out_stride,
+                                  uint32_t tile_first, uint32_t tiles) {
+    const uint32_t lane = lane_id(), row = lane % 32, kq = 8 * (lane / 32);
     for (uint32_t t = tile_first; t < tile_first + tiles; ++t) {
         const uint32_t col = t * 32 + row;
         f32x16         acc;
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-        for (uint32_t k0 = 0; k0 < K; k0 += 8) {
-            bf16x4 a, b;
+        for (uint32_t k0 = 0; k0 < K; k0 += 16) {
+            bf16x8 a, b;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                a[q] = short(act[row * lds_stride + k0 + kq + q]);
-                b[q] = short(W[uint64_t(k0 + kq + q) * kMlpHidden + col]);
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t k = k0 + kq + q;
+                a[q]             = __builtin_bit_cast(__bf16, act[row * lds_stride + k]);
+                b[q]             = __builtin_bit_cast(__bf16, uint16_t(k < Kw ? W[uint64_t(k) * kMlpHidden + col] : 0));
             }
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
@@ -195,7 +198,7 @@ __device__ inline void mfma_layer(const uint16_t* act, uint32_t lds_stride, uint
     }
 }
 
-// X: n x in (FP32, converted to BF16 on load), in_pad = in rounded up to 8.
+// X: n x in (FP32, converted to BF16 on load), in_pad = in rounded up to 16.
 // Layers 1-2 on MFMA, layer 3 (256 -> 2) in FP32 from the BF16 activations.
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* X, uint32_t n, uint32_t in, uint32_t in_pad,
                                                    const uint16_t* W1, const float* b1, const uint16_t* W2,
@@ -212,9 +215,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* X, uint32_t n, u
     }
     __syncthreads();
     const uint32_t wave = wave_in_block(); // 4 waves x 2 tiles of 32 columns = 256
-    mfma_layer(xs, in_pad, in_pad, W1, b1, h1, hs, wave * 2, 2);
+    mfma_layer(xs, in_pad, in_pad, in, W1, b1, h1, hs, wave * 2, 2);
     __syncthreads();
-    mfma_layer(h1, hs, kMlpHidden, W2, b2, h2, hs, wave * 2, 2);
+    mfma_layer(h1, hs, kMlpHidden, kMlpHidden, W2, b2, h2, hs, wave * 2, 2);
     __syncthreads();
     if (wave == 0) {
         const uint32_t lane = lane_id(), r = lane % 32, o = lane / 32;
@@ -261,7 +264,7 @@ hipError_t launch_mlp_infer(const float* X, uint32_t n, uint32_t in, const uint1
                             hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (in > kMlpMaxIn) return hipErrorInvalidValue;
-    const uint32_t in_pad = (in + 7) / 8 * 8;
+    const uint32_t in_pad = (in + 15) / 16 * 16;
     k_mlp_infer<<<blocks_for(n, 32), 256, 0, s>>>(X, n, in, in_pad, W1, b1, W2, b2, W3, b3, out);
     return hipGetLastError();
 }
