@@ -607,6 +607,8 @@ def main():
     traffic = load_traffic(wl.name)
     if s["probe_kernel"] == 1:
         kernel_name = "k_lean (one-probe pipeline: page decode, hash probe, output pages)"
+        if s["probe_packed"]:
+            kernel_name += "; hash table read packed, 8 bytes per slot"
     else:
         kernel_name = "k_pipeline (fused probe chain: page decode, probes, output pages)"
     notes = {}

@@ -131,6 +131,10 @@ __device__ inline uint4 load_invariant_u4(const void* p) {
     return make_uint4(q[0], q[1], q[2], q[3]);
 }
 
+__device__ inline uint32_t load_invariant_u32(const uint32_t* p) {
+    return *(const __attribute__((address_space(4))) uint32_t*)(uintptr_t)p;
+}
+
 __device__ inline uint64_t load_invariant_u64(const uint64_t* p) {
     const __attribute__((address_space(4))) uint32_t* q = (const __attribute__((address_space(4))) uint32_t*)(uintptr_t)p;
     return uint64_t(q[0]) | (uint64_t(q[1]) << 32);

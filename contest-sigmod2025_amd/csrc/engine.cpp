@@ -520,6 +520,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "incremental") options.incremental = v != 0;
     else if (name == "restrictions") options.restrictions = v != 0;
     else if (name == "singletons") options.singletons = v != 0;
+    else if (name == "packed_tables") options.packed_tables = v != 0;
     else return false;
     return true;
 }
@@ -554,6 +555,12 @@ struct HashTable {
     bool                  unique    = true;
     uint64_t              tuples    = 0;
     std::vector<uint32_t> eqs; // class of each word (word 0 = key)
+    // Packed copy for the one-probe kernel (two-word tables of INT32 classes):
+    // one word per slot, key | attr << 32, made on the first probe that can
+    // use it; `pack_flag` nonzero means an entry did not fit and the kernel
+    // reads `slots`.
+    mutable DevBuf        packed;
+    mutable DevBuf        pack_flag;
 };
 
 // Materialised tuples: the driver of a pipeline that continues a long chain.
@@ -616,6 +623,7 @@ public:
 
     double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0, ms_partition = 0, ms_replan = 0;
     uint32_t launches = 0, retries = 0, retries_build = 0, probe_lean = 0, probe_waves = 0, replans = 0;
+    uint32_t probe_packed = 0;
     uint32_t restrictions = 0, semijoins = 0;
     uint64_t build_tuples = 0, probe_rows = 0, build_bytes = 0, out_ref_pages = 0, probe_input_pages = 0;
 
@@ -687,6 +695,7 @@ private:
     std::vector<const DevColumn*>           colreg_;
     std::map<std::pair<uint32_t, uint64_t>, uint32_t> colid_;
     DevBuf                                  counters_;
+    DevBuf                                  wave_sums_; // one-probe kernel: per-wave closing sums
     DevBuf                                  col_base_, col_npages_;
     hipEvent_t                              ev0_ = nullptr, ev1_ = nullptr;
 
@@ -753,6 +762,21 @@ private:
     uint32_t per_cu_for(const PipeArgs& a, uint32_t nslots) const {
         if (a.grid_per_cu) return a.grid_per_cu;
         return use_lean(a, nslots) ? ctx_.options.lean_waves_per_cu : 16;
+    }
+    // Bytes of one build tuple's values: 4 per INT32 class, 8 per other class
+    // (the algorithmic size of a compact tuple).
+    uint64_t value_bytes(const std::vector<uint32_t>& eqs) const {
+        uint64_t b = 0;
+        for (uint32_t e : eqs) b += q_.attrs[q_.eq_attrs[e].front()].type == T_INT32 ? 4 : 8;
+        return b;
+    }
+    // A two-word table whose key and attribute classes are INT32: the
+    // one-probe kernel reads it packed, 8 bytes per slot instead of 16.
+    bool packable(const HashTable& ht) const {
+        if (!ctx_.options.packed_tables || !ctx_.options.lean_kernel || ht.grouped || ht.words != 2) return false;
+        for (uint32_t w = 0; w < 2; ++w)
+            if (q_.attrs[q_.eq_attrs[ht.eqs[w]].front()].type != T_INT32) return false;
+        return true;
     }
     // Whether the pipeline runs on the lean one-probe kernel, and its grid.
     bool     use_lean(const PipeArgs& a, uint32_t nslots) const {
@@ -865,13 +889,25 @@ uint64_t Executor::launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& 
     uint64_t waves = waves_for(a, nslots, nrows);
     CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
     if (a.nmorsels) {
-        if (use_lean(a, nslots))
+        if (use_lean(a, nslots)) {
+            // closing sums as per-wave stores, added up by a second small kernel
+            const uint64_t stride = (waves + 63) / 64 * 64;
+            if (wave_sums_.bytes() < stride * kSumFields * sizeof(unsigned long long))
+                wave_sums_ = DevBuf(&ctx_.heap(), stride * kSumFields * sizeof(unsigned long long));
+            a.wave_sums        = wave_sums_.get<unsigned long long>();
+            a.wave_sums_stride = uint32_t(stride);
             CGJ_HIP(launch_probe1(a, nslots, uint32_t(waves), ctx_.options.aligned_pairs, ctx_.stream()));
-        else
+            CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
+            const uint32_t nout = a.target == TARGET_PAGES ? a.nout : 0;
+            CGJ_HIP(launch_fold_sums(a.wave_sums, a.wave_sums_stride, nout, a.counters, ctx_.stream()));
+        } else {
             CGJ_HIP(launch_pipeline(a, nslots, uint32_t(waves), ctx_.stream()));
+            CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
+        }
+    } else {
+        CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
     }
-    CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
-    CGJ_HIP(hipEventSynchronize(ev1_));
+    CGJ_HIP(hipStreamSynchronize(ctx_.stream())); // the kernel and the sums' fold
     float f = 0;
     CGJ_HIP(hipEventElapsedTime(&f, ev0_, ev1_));
     ms += f;
@@ -1020,7 +1056,7 @@ const HashTable* Executor::make_table(Target& staged, const std::vector<uint32_t
     CGJ_HIP(hipEventElapsedTime(&fms, ev0_, ev1_));
     ms_build += fms;
     build_tuples += n;
-    build_bytes += n * ht->words * sizeof(uint64_t);
+    build_bytes += n * value_bytes(ht->eqs);
     staged.stage.reset();
     tables_.push_back(std::move(ht));
     return tables_.back().get();
@@ -1299,7 +1335,7 @@ bool Executor::scan_insert(uint32_t rel, const std::vector<uint32_t>& eqs, uint6
     ht->tuples = n;
     ht->unique = back[0] == 0;
     build_tuples += n;
-    build_bytes += n * words * sizeof(uint64_t);
+    build_bytes += n * value_bytes(ht->eqs);
     tables_.push_back(std::move(ht));
     out = tables_.back().get();
     return true;
@@ -1377,7 +1413,7 @@ bool Executor::scan_build(uint32_t rel, const std::vector<uint32_t>& eqs, uint64
     ht->tuples = n;
     ht->unique = f[0] == 0;
     build_tuples += n;
-    build_bytes += n * words * sizeof(uint64_t);
+    build_bytes += n * value_bytes(ht->eqs);
     tables_.push_back(std::move(ht));
     out = tables_.back().get();
     return true;
@@ -1517,6 +1553,17 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         P.bloom     = ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr;
         P.bloom_log = ht->bloom_log;
         P.early     = ks->second < scanned ? 1 : 0;
+        if (probes.size() == 1 && (t.kind == TARGET_PAGES || t.kind == TARGET_COUNT) && packable(*ht)) {
+            if (!ht->packed.get()) {
+                ht->packed    = DevBuf(&ctx_.heap(), ht->cap * sizeof(uint64_t));
+                ht->pack_flag = DevBuf(&ctx_.heap(), sizeof(uint32_t));
+                CGJ_HIP(hipMemsetAsync(ht->pack_flag.get(), 0, sizeof(uint32_t), ctx_.stream()));
+                CGJ_HIP(launch_pack_table(ht->slots.get<uint64_t>(), ht->cap, ht->packed.get<uint64_t>(),
+                                          ht->pack_flag.get<uint32_t>(), ctx_.stream()));
+            }
+            P.ht_packed = ht->packed.get<uint64_t>();
+            P.pack_flag = ht->pack_flag.get<uint32_t>();
+        }
         for (uint32_t w = 1; w < ht->words; ++w) {
             uint32_t e  = ht->eqs[w];
             auto     it = slot.find(e);
@@ -1637,9 +1684,10 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             res->total_pages    = 0;
             for (uint32_t i = 0; i < a.nout; ++i) res->total_pages += c.col_pages[i];
             res->pool_pages     = pages;
-            res->reserved_pages = std::min<uint64_t>(c.pages, pages);
+            res->reserved_pages = std::min<uint64_t>(c.reserved(), pages);
             ms_probe_kernel     = last_ms_;
             probe_lean          = use_lean(a, nslots) ? 1 : 0;
+            probe_packed        = c.packed ? 1 : 0;
             probe_waves         = uint32_t(waves_for(a, nslots, nrows));
             // output bytes at the reference's packing (TableTarget.cpp:111-182)
             for (uint32_t i = 0; i < a.nout; ++i) {
@@ -1654,13 +1702,13 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             }
             break;
         }
-        if (c.pages >= a.abort_pages)
+        if (c.reserved() >= a.abort_pages)
             throw std::runtime_error("query result too large: more than " + std::to_string(a.abort_pages) +
                                      " output pages (" + std::to_string(a.abort_pages * kPageSize >> 30) +
                                      " GiB), beyond device memory");
         if (attempt > 2) throw std::runtime_error("output page pool overflow persists");
         ++retries;
-        pages = c.pages + slack;
+        pages = c.reserved() + slack;
         if (pages * (kPageSize + 16) > ctx_.mem_bytes())
             throw std::runtime_error("query result too large: " + std::to_string(c.pages) +
                                      " output pages need more than the device memory");
@@ -2229,6 +2277,7 @@ std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const st
     stats.ms_probe_kernel   = ex.ms_probe_kernel;
     stats.ms_probe_partition = ex.ms_partition;
     stats.probe_kernel      = ex.probe_lean;
+    stats.probe_packed      = ex.probe_packed;
     stats.probe_waves       = ex.probe_waves;
     stats.build_tuples      = ex.build_tuples;
     stats.build_bytes       = ex.build_bytes;

@@ -257,6 +257,8 @@ struct Options {
     bool     incremental           = true;
     bool     restrictions          = true; // incremental loop: restrict key producers after each build
     bool     singletons            = true; // remove one-row relations before planning
+    // The one-probe kernel reads two-word INT32 tables packed (key | attr << 32).
+    bool     packed_tables         = true;
 };
 
 // Tunables of one batched join.

@@ -94,53 +94,66 @@ __global__ void k_scan_add(uint32_t* data, uint64_t n, const uint32_t* sums, con
 
 // Validity of rows [bit, bit + 64) of a page (nr rows, nv non-NULL values),
 // bit i = row bit + i; rows at or past nr read as 0. A page without NULLs is
-// all valid and its bitmap is not read; otherwise at most 9 bitmap bytes are,
-// none past the page end.
+// all valid and its bitmap is not read; otherwise the 9 bitmap bytes from
+// byte bit / 8 on come from two aligned 8-byte loads (the page end is
+// 8-aligned, so neither reaches past it; bytes past the page end read as 0).
 __device__ inline uint64_t page_bits64(const uint8_t* page, uint32_t nr, uint32_t nv, uint32_t bit) {
     if (bit >= nr) return 0;
     const uint32_t n = nr - bit < 64 ? nr - bit : 64;
     const uint64_t m = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
     if (nv == nr) return m;
-    const uint32_t nbytes = (nr + 7) / 8;
-    const uint8_t* bm     = page + kPageSize - nbytes;
-    const uint32_t b0 = bit >> 3, sh = bit & 7;
-    uint64_t       lo = 0, hi = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < 8; ++i)
-        if (b0 + i < nbytes) lo |= uint64_t(bm[b0 + i]) << (8 * i);
-    if (b0 + 8 < nbytes) hi = bm[b0 + 8];
-    const uint64_t w = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+    const uint32_t off = kPageSize - (nr + 7) / 8 + (bit >> 3); // first bitmap byte wanted
+    const uint32_t al  = off & ~7u, o = off & 7u, sh = bit & 7u;
+    const uint64_t w0  = *reinterpret_cast<const uint64_t*>(page + al);
+    const uint64_t w1  = al + 8 < kPageSize ? *reinterpret_cast<const uint64_t*>(page + al + 8) : 0ull;
+    const uint64_t lo  = o ? ((w0 >> (8 * o)) | (w1 << (64 - 8 * o))) : w0; // bytes off .. off + 7
+    const uint64_t hi  = (w1 >> (8 * o)) & 0xffull;                         // byte off + 8
+    const uint64_t w   = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
     return w & m;
 }
 
-// Block index, one wave per page: the blocks whose first row lies in page p
-// (64 at a time, one per lane) get their entries from the page's bitmap; a
-// block's non-NULL offset is the page's valid rows before it, from an
-// exclusive wave scan of the blocks' in-page valid counts. A block that runs
-// past the page continues on page p + 1 when that page holds all its remaining
-// rows (and is not a long-string page), else it is marked complex, as are
-// blocks on long-string pages. Reads only headers and bitmaps.
+// Block index, four pages per wave, 16 lanes per page: the blocks whose first
+// row lies in page p (16 at a time, one per lane) get their entries from the
+// page's bitmap; a block's non-NULL offset is the page's valid rows before it,
+// from an exclusive scan of the blocks' in-page valid counts over the page's
+// 16 lanes. A block that runs past the page continues on page p + 1 when that
+// page holds all its remaining rows (and is not a long-string page), else it
+// is marked complex, as are blocks on long-string pages. Reads only headers
+// and bitmaps.
 __global__ __launch_bounds__(256) void k_block_index_pages(const uint8_t* pages, uint64_t npages, int32_t type,
                                                           const uint32_t* row_start, uint64_t nrows,
                                                           BlockIdx* blocks) {
-    const uint64_t p = uint64_t(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
-    if (p >= npages) return; // the whole wave
-    const uint32_t s = row_start[p], e = row_start[p + 1];
-    if (e <= s) return; // an empty page starts no block
-    const uint64_t b_first = (uint64_t(s) + 63) / 64, b_last = (uint64_t(e) - 1) / 64;
-    if (b_first > b_last) return;
-    const uint8_t* page    = pages + p * kPageSize;
-    const uint32_t nr_raw  = *reinterpret_cast<const uint16_t*>(page);
-    const uint32_t nv      = *reinterpret_cast<const uint16_t*>(page + 2);
+    constexpr uint32_t kLanesPerPage = 16;
+    const uint32_t     lane          = lane_id();
+    const uint32_t     gl            = lane & (kLanesPerPage - 1);
+    const uint32_t     glast         = lane | (kLanesPerPage - 1); // last lane of this page's group
+    const uint64_t     wave          = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+    const uint64_t     p             = wave * (64 / kLanesPerPage) + lane / kLanesPerPage;
+    uint32_t           s = 0, e = 0;
+    if (p < npages) {
+        s = row_start[p];
+        e = row_start[p + 1];
+    }
+    // an empty page, or one whose rows all belong to a block started earlier, starts no block
+    const uint64_t b_first = (uint64_t(s) + 63) / 64;
+    const uint64_t b_last  = e > s ? (uint64_t(e) - 1) / 64 : 0;
+    const bool     has     = e > s && b_first <= b_last;
+    const uint8_t* page    = pages + (has ? p : 0) * kPageSize;
+    uint32_t       nr_raw = 0, nv = 0;
+    if (has) {
+        nr_raw = *reinterpret_cast<const uint16_t*>(page);
+        nv     = *reinterpret_cast<const uint16_t*>(page + 2);
+    }
     const bool     is_long = type == T_VARCHAR && nr_raw >= 0xfffe;
     const uint32_t nr      = e - s;
-    const uint32_t lane    = lane_id();
-    // valid rows of the page before the first block
-    const uint32_t j00   = uint32_t(b_first * 64 - s);
-    uint32_t       carry = is_long ? 0 : uint32_t(__popcll(page_bits64(page, nr, nv, 0) & (j00 >= 64 ? ~0ull : ((1ull << j00) - 1ull))));
-    for (uint64_t bb = b_first; bb <= b_last; bb += 64) {
-        const uint64_t b    = bb + lane;
-        const bool     have = b <= b_last;
+    // valid rows of the page before its first block
+    const uint32_t j00   = has ? uint32_t(b_first * 64 - s) : 0;
+    uint32_t       carry = (has && !is_long && j00)
+                               ? uint32_t(__popcll(page_bits64(page, nr, nv, 0) & ((1ull << j00) - 1ull)))
+                               : 0;
+    for (uint64_t k = 0;; k += kLanesPerPage) {
+        const uint64_t b    = b_first + k + gl;
+        const bool     have = has && b <= b_last;
         BlockIdx       x;
         x.page    = uint32_t(p);
         x.nn      = 0;
@@ -158,11 +171,11 @@ __global__ __launch_bounds__(256) void k_block_index_pages(const uint8_t* pages,
                 const uint32_t n1 = uint32_t(e - r0);
                 const uint64_t q  = p + 1;
                 bool           ok = q < npages && row_start[q + 1] >= r1;
-                const uint8_t* pq = pages + q * kPageSize;
                 if (ok) {
+                    const uint8_t* pq  = pages + q * kPageSize;
                     const uint32_t nrq = *reinterpret_cast<const uint16_t*>(pq);
                     const uint32_t nvq = *reinterpret_cast<const uint16_t*>(pq + 2);
-                    ok = !(type == T_VARCHAR && nrq >= 0xfffe);
+                    ok                 = !(type == T_VARCHAR && nrq >= 0xfffe);
                     if (ok) {
                         const uint32_t rows_q = row_start[q + 1] - row_start[q];
                         const uint32_t n2     = uint32_t(r1 - e);
@@ -173,10 +186,17 @@ __global__ __launch_bounds__(256) void k_block_index_pages(const uint8_t* pages,
                 if (!ok) x.complex = 1;
             }
         }
-        const uint32_t incl = wave_incl_scan(cnt);
-        x.nn                = uint16_t(carry + incl - cnt);
-        carry += __shfl(incl, 63, 64);
+        // inclusive scan of cnt over the page's lanes
+        uint32_t incl = cnt;
+#pragma unroll
+        for (uint32_t d = 1; d < kLanesPerPage; d <<= 1) {
+            const uint32_t t = __shfl_up(incl, d, kLanesPerPage);
+            if (gl >= d) incl += t;
+        }
+        x.nn = uint16_t(carry + incl - cnt);
+        carry += __shfl(incl, glast, 64);
         if (have) blocks[b] = x;
+        if (!__ballot(has && b_first + k + kLanesPerPage <= b_last)) break;
     }
 }
 
@@ -213,8 +233,8 @@ hipError_t launch_prepare_column(const uint8_t* pages, uint64_t npages, int32_t 
                                                                 npages);
     uint64_t nb = (nrows + 63) / 64;
     if (nb)
-        k_block_index_pages<<<dim3((npages + 3) / 4), dim3(256), 0, s>>>(pages, npages, type, row_start, nrows,
-                                                                         blocks);
+        k_block_index_pages<<<dim3((npages + 15) / 16), dim3(256), 0, s>>>(pages, npages, type, row_start, nrows,
+                                                                           blocks);
     return hipGetLastError();
 }
 
@@ -1572,7 +1592,8 @@ __device__ inline void lean_open(const LeanArgs& a, WaveState& ws, LeanPage& st,
     uint64_t next = ws.next;
     if (next == ws.end) {
         const uint32_t chunk = a.page_chunk;
-        next   = wave_fetch_add(&a.counters->pages, (unsigned long long)chunk);
+        const uint64_t base  = a.static_chunk ? uint64_t(gridDim.x) * kWavesPerBlock * chunk : 0;
+        next   = base + wave_fetch_add(&a.counters->pages, (unsigned long long)chunk);
         ws.end = next + chunk;
         if (next >= a.abort_pages) ws.stop = 1;
     }
@@ -1670,8 +1691,13 @@ __device__ inline void lean_finish(const LeanArgs& a, LeanPage& st, uint64_t bit
     else if (st.page != kNoPage && !st.bad && lead_lane())
         a.dir[4 * uint64_t(st.page) + 3] = 0; // opened, never filled
     if (lead_lane()) {
-        if (st.nnz) atomicAdd(&a.counters->nnz[c], (unsigned long long)st.nnz);
-        if (st.npages) atomicAdd(&a.counters->col_pages[c], (unsigned long long)st.npages);
+        if (a.wave_sums) {
+            a.wave_sums[uint64_t(1 + c) * a.wave_sums_stride + stream]            = st.nnz;
+            a.wave_sums[uint64_t(1 + kLeanOut + c) * a.wave_sums_stride + stream] = st.npages;
+        } else {
+            if (st.nnz) atomicAdd(&a.counters->nnz[c], (unsigned long long)st.nnz);
+            if (st.npages) atomicAdd(&a.counters->col_pages[c], (unsigned long long)st.npages);
+        }
     }
 }
 
@@ -1718,6 +1744,12 @@ template <int NO, int NS>
 __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end, uint32_t stream, LeanWave<NS>& W,
                                  int32_t unique) {
     const uint32_t lane = lane_id();
+    // the packed table is used unless its build flagged an entry that does not fit
+    bool packed;
+    {
+        const LeanArgs& a0 = args_at<LeanArgs>(args_addr);
+        packed             = a0.ht_packed && load_invariant_u32(a0.pack_flag) == 0;
+    }
     {
         for (uint64_t rb = r0; rb < end && !W.ws.stop; rb += 64) {
             const LeanArgs& a      = args_at<LeanArgs>(args_addr); // re-read per block (scalar cache)
@@ -1749,7 +1781,7 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                 uint64_t bb = bloom_bits(h2);
                 alive       = (a.bloom[bloom_word(h2, a.bloom_log)] & bb) == bb;
             }
-            const uint64_t* ht    = a.ht;
+            const uint64_t* ht    = packed ? a.ht_packed : a.ht;
             const uint32_t  words = a.words;
             const uint64_t  rmask = a.mask;
             uint64_t        q     = h >> a.shift; // next slot to inspect
@@ -1781,8 +1813,26 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                         const uint64_t qb   = (qa & ~rmask) | ((qa + 1) & rmask);
                         const uint64_t qn   = (qb & ~rmask) | ((qb + 1) & rmask);
                         uint64_t       ka, wa, kb, wb;
-                        ht_entry2(ht, words, qa, ka, wa);
-                        ht_entry2(ht, words, qb, kb, wb);
+                        if (packed) {
+                            // one 8-byte word per slot; with aligned pairs both
+                            // slots come from one 16-byte load
+                            uint64_t ea, eb;
+                            if (odd) {
+                                const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(ht + qa);
+                                ea                 = v.x;
+                                eb                 = v.y;
+                            } else {
+                                ea = ht[qa];
+                                eb = ht[qb];
+                            }
+                            ka = ea == kEmpty ? kEmpty : (ea & 0xffffffffull);
+                            wa = ea >> 32;
+                            kb = eb == kEmpty ? kEmpty : (eb & 0xffffffffull);
+                            wb = eb >> 32;
+                        } else {
+                            ht_entry2(ht, words, qa, ka, wa);
+                            ht_entry2(ht, words, qb, kb, wb);
+                        }
                         if (!skip && ka == key) {
                             have  = true;
                             hslot = qa;
@@ -1845,7 +1895,12 @@ __device__ inline void lean_flush(const LeanArgs& a, LeanWave<NS>& W, uint32_t s
     const uint32_t lane = lane_id();
 #pragma unroll
     for (int c = 0; c < NO; ++c) lean_finish(a, W.st[c], W.bits[c], uint32_t(c), stream);
-    if (lead_lane() && W.local) atomicAdd(&a.counters->tuples, (unsigned long long)W.local);
+    if (lead_lane()) {
+        if (a.wave_sums)
+            a.wave_sums[stream] = W.local;
+        else if (W.local)
+            atomicAdd(&a.counters->tuples, (unsigned long long)W.local);
+    }
     if (NO > 0)
         for (uint64_t i = W.ws.next + lane; i < W.ws.end && i < a.pool_pages; i += 64) a.dir[4 * i + 3] = 0;
     lean_wave_init(W);
@@ -1862,6 +1917,14 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
     LeanWave<NS>   W;
     lean_wave_init(W);
+    if (stream == 0 && a.ht_packed && load_invariant_u32(a.pack_flag) == 0 && lead_lane()) a.counters->packed = 1;
+    if (a.wave_sums && stream == 0 && lead_lane()) a.counters->waves = nwaves;
+    if (NO > 0 && a.static_chunk) {
+        // the wave's first pages need no atomic
+        W.ws.next = uint64_t(stream) * a.page_chunk;
+        W.ws.end  = W.ws.next + a.page_chunk;
+        if (stream == 0 && lead_lane()) a.counters->static_pages = uint64_t(nwaves) * a.page_chunk;
+    }
 
     // Morsel schedule: each wave starts on morsel `stream`, then takes the next
     // unclaimed one from a device counter (one atomic per 4096-row morsel),
@@ -1878,7 +1941,8 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
     for (uint64_t m = m_first; m < m_end && !W.ws.stop;) {
         uint64_t m_next = m + m_step;
         if (!a.xcd_order) {
-            m_next = nwaves + wave_fetch_add(&a.counters->morsel, 1ull);
+            // no atomic when every morsel was some wave's first
+            m_next = a.nmorsels <= nwaves ? m_end : nwaves + wave_fetch_add(&a.counters->morsel, 1ull);
         }
         const uint64_t r0  = m * a.morsel_rows;
         const uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
@@ -2074,6 +2138,59 @@ bool probe1_applicable(const PipeArgs& a, uint32_t slots) {
     return true;
 }
 
+// One thread per slot of a two-word table: key | attr << 32, or a flag when
+// the entry does not fit (see launch_pack_table).
+__global__ __launch_bounds__(256) void k_pack_table(const uint64_t* ht, uint64_t cap, uint64_t* packed,
+                                                    uint32_t* flag) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    const ulonglong2 e = *reinterpret_cast<const ulonglong2*>(ht + 2 * i);
+    uint64_t         w = kEmpty;
+    if (e.x != kEmpty) {
+        const bool fits = (e.x >> 32) == 0 && (e.y >> 32) == 0 && !(e.x == 0xffffffffull && e.y == 0xffffffffull);
+        if (!fits) atomicOr(flag, 1u);
+        w = e.x | (e.y << 32);
+    }
+    packed[i] = w;
+}
+
+// One workgroup per field: the sum over the grid's waves, added to the field.
+__global__ __launch_bounds__(256) void k_fold_sums(const unsigned long long* ws, uint32_t stride, uint32_t nout,
+                                                   PipeCounters* c) {
+    __shared__ unsigned long long part[4];
+    const uint32_t     f     = blockIdx.x; // 0: rows; 1..: nnz, then pages, of the used columns
+    const uint32_t     field = f == 0 ? 0u : f <= nout ? f : 1u + kLeanOut + (f - 1 - nout);
+    const uint64_t     n     = c->waves < stride ? c->waves : stride;
+    unsigned long long s     = 0;
+    for (uint64_t w = threadIdx.x; w < n; w += blockDim.x) s += ws[uint64_t(field) * stride + w];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane_id() == 0) part[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        if (f == 0)
+            c->tuples += t;
+        else if (f <= nout)
+            c->nnz[f - 1] += t;
+        else
+            c->col_pages[f - 1 - nout] += t;
+    }
+}
+
+hipError_t launch_fold_sums(const unsigned long long* wave_sums, uint32_t stride, uint32_t nout, PipeCounters* c,
+                            hipStream_t s) {
+    if (nout > uint32_t(kLeanOut)) return hipErrorInvalidValue;
+    k_fold_sums<<<dim3(1 + 2 * nout), dim3(256), 0, s>>>(wave_sums, stride, nout, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_table(const uint64_t* ht, uint64_t cap, uint64_t* packed, uint32_t* flag, hipStream_t s) {
+    if (cap == 0) return hipSuccess;
+    k_pack_table<<<dim3(uint32_t((cap + 255) / 256)), dim3(256), 0, s>>>(ht, cap, packed, flag);
+    return hipGetLastError();
+}
+
 hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, bool aligned_pairs, hipStream_t s) {
     uint32_t blocks = (grid_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks == 0) blocks = 1;
@@ -2121,6 +2238,13 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
     l.aligned_pairs = aligned_pairs && P.words <= 2 ? 1u : 0u;
     l.abort_pages   = a.abort_pages;
     l.sel           = a.sel;
+    l.static_chunk     = 1;
+    l.wave_sums        = a.wave_sums;
+    l.wave_sums_stride = a.wave_sums_stride;
+    if (P.ht_packed && P.pack_flag && P.words == 2) {
+        l.ht_packed = P.ht_packed;
+        l.pack_flag = P.pack_flag;
+    }
     // With the dynamic morsel schedule a grid larger than one resident wave
     // set only adds a tail: cap it at what the kernel's occupancy fits.
     static const void* const kernels[kLeanOut + 1] = {
@@ -2148,6 +2272,8 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
             if (resident && blocks > resident) blocks = resident;
         }
     }
+    // per-wave sums are stored at index `wave` < stride
+    if (l.wave_sums && uint64_t(blocks) * kWavesPerBlock > l.wave_sums_stride) return hipErrorInvalidValue;
     switch (no) {
     case 0: k_lean<0><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
     case 1: k_lean<1><<<dim3(blocks), dim3(256), 0, s>>>(l); break;

@@ -30,6 +30,11 @@ struct ProbeSpec {
     const uint64_t* bloom;     // blocked bloom filter (nullptr = none)
     uint32_t        bloom_log; // log2(words)
     uint32_t        early;     // key is a scanned column: filter before probing
+    // Packed copy of a two-word table (the one-probe kernel only; nullptr =
+    // none): one word per slot, key | attr << 32, same slots as `ht`; used
+    // unless *pack_flag is nonzero (an entry did not fit 32 + 32 bits).
+    const uint64_t* ht_packed;
+    const uint32_t* pack_flag;
     // Grouped table: ht holds 2-word slots [key, start << 24 | count] and the
     // matching tuples' words 1.. (attr_word / check_word - 1) are payload rows
     // start .. start + count - 1 of `pwords` words.
@@ -49,14 +54,28 @@ enum : uint32_t { TARGET_STAGE = 0, TARGET_PAGES = 1, TARGET_COUNT = 2 };
 enum : uint32_t { DRIVER_PAGES = 0, DRIVER_TUPLES = 1 };
 
 // Counters written by a pipeline launch (device memory).
+// The counters that waves update while a kernel runs (reservations, the
+// morsel queue, the closing sums) each sit on their own 256-byte line, so
+// their atomics do not queue behind each other's.
 struct PipeCounters {
     unsigned long long tuples;    // staged tuples / output rows / counted tuples
-    unsigned long long pages;     // pool pages (or staging slots) reserved
+    unsigned long long pad0[31];
+    unsigned long long pages;     // pool pages (or staging slots) reserved by atomics
+    unsigned long long pad1[31];
     unsigned long long morsel;    // morsels handed out beyond each wave's first (dynamic schedule)
+    unsigned long long pad2[31];
     unsigned long long overflow;  // nonzero: a buffer was too small, re-run
     unsigned long long fault;     // nonzero: a hash-table walk met no empty slot within its bound
+    unsigned long long packed;    // one-probe kernel: nonzero when it read the packed table
+    // one-probe kernel: pool pages 0 .. static_pages - 1 were given to the
+    // waves before any atomic (`page_chunk` each); atomics reserve after them
+    unsigned long long static_pages;
+    unsigned long long waves;     // one-probe kernel with per-wave sums: waves of the grid
+    unsigned long long pad3[27];
     unsigned long long nnz[kMaxOut];       // non-NULL values written per output column
     unsigned long long col_pages[kMaxOut]; // pages written per output column
+
+    unsigned long long reserved() const { return static_pages + pages; }
 };
 
 struct PipeArgs {
@@ -100,11 +119,16 @@ struct PipeArgs {
                                  // restrictions (launch_filter_select), nullptr = all
     uint64_t        abort_pages; // a wave whose reservation starts at or past this many pool pages (or
                                  // staging slots) stops: the result cannot fit device memory
+    // One-probe kernel: per-wave closing sums (see LeanArgs::wave_sums).
+    unsigned long long* wave_sums;
+    uint32_t            wave_sums_stride;
 };
 
 // ---- lean one-probe pipeline ----
 constexpr int      kLeanOut   = 4;  // fixed-width output columns
 constexpr uint32_t kPageChunk = 32; // pool pages reserved per atomic (upper bound)
+// Fields of a wave's closing sums: rows, then non-NULL values and pages per output column.
+constexpr uint32_t kSumFields = 1 + 2 * kLeanOut;
 
 // Where an output column's value comes from.
 enum : uint32_t { SRC_KEY = 0, SRC_SCAN = 1, SRC_HT = 2 };
@@ -147,6 +171,18 @@ struct LeanArgs {
     uint32_t        aligned_pairs;
     uint64_t        abort_pages; // as PipeArgs::abort_pages
     const uint64_t* sel;         // as PipeArgs::sel
+    const uint64_t* ht_packed;   // as ProbeSpec::ht_packed (nullptr = none)
+    const uint32_t* pack_flag;
+    // Nonzero: each wave of the launch owns pool pages [wave * page_chunk,
+    // (wave + 1) * page_chunk) from the start, and atomics on counters->pages
+    // reserve after the grid's static pages (k_lean; the batch kernel leaves it 0).
+    uint32_t        static_chunk;
+    // Nonzero: instead of atomics on the counters, each wave stores its closing
+    // sums as wave_sums[field * wave_sums_stride + wave] (fields as kSumFields)
+    // and launch_fold_sums adds them up after the kernel; wave 0 stores the
+    // grid's wave count in counters->waves.
+    unsigned long long* wave_sums;
+    uint32_t            wave_sums_stride;
 };
 
 // ---- batched joins: one persistent launch over a device work queue ----
@@ -297,6 +333,15 @@ hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_wave
 hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, bool aligned_pairs, hipStream_t s);
 bool       probe1_applicable(const PipeArgs& a, uint32_t slots);
 uint32_t   pipeline_waves_per_block();
+// Packs a two-word table of `cap` slots into one word per slot,
+// key | attr << 32 (empty slots stay all ones); sets *flag when an entry's key
+// or attribute needs more than 32 bits, the attribute is NULL, or the packed
+// word would read as empty. `flag` must be zero before the launch.
+hipError_t launch_pack_table(const uint64_t* ht, uint64_t cap, uint64_t* packed, uint32_t* flag, hipStream_t s);
+// Adds the per-wave closing sums of a one-probe launch (LeanArgs::wave_sums)
+// of `nout` output columns into the counters' tuples, nnz and col_pages.
+hipError_t launch_fold_sums(const unsigned long long* wave_sums, uint32_t stride, uint32_t nout, PipeCounters* c,
+                            hipStream_t s);
 
 // ---- join sampling (the planner's cardinality estimates) ----
 // One scanned column of the sample gather: rows rowids[first .. first + n) of

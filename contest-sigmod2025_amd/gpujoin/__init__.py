@@ -103,6 +103,7 @@ class Stats(C.Structure):
         ("singletons", C.c_uint32),
         ("restrictions", C.c_uint32),
         ("semijoins", C.c_uint32),
+        ("probe_packed", C.c_uint32),
     ]
 
     def as_dict(self):

@@ -112,7 +112,7 @@ typedef struct {
     uint64_t output_ref_pages;  /* output pages at the reference's packing:
                                    ceil((rows + width*nonnull)/bit budget) per
                                    fixed-width column, pages written otherwise */
-    uint64_t build_bytes;       /* compact hash-table tuples: tuples x words x 8 */
+    uint64_t build_bytes;       /* compact hash-table tuples: per tuple 4 bytes per INT32 word, 8 per other word */
     double   ms_probe_kernel;   /* HIP-event time of the output pipeline kernel */
     double   ms_probe_partition;/* HIP-event time of the probe side's radix partitioning */
     uint32_t probe_kernel;      /* output pipeline ran on: 0 the general kernel, 1 the lean one-probe kernel */
@@ -135,6 +135,8 @@ typedef struct {
                                    built tables of at most 32 tuples; semi-join tables) */
     uint32_t semijoins;         /* built tables used only as an exact restriction of their
                                    key's producers (duplicate-free, key only) */
+    uint32_t probe_packed;      /* the output pipeline's lean kernel read its hash table packed
+                                   (INT32 key | INT32 attribute << 32, 8 bytes per slot) */
 } cgj_stats;
 
 /* Execution options of a context (0 = success, -1 = unknown name):
@@ -169,7 +171,9 @@ typedef struct {
  *       engine/storage/RestrictionLogic.cpp:397-447) (1, default) or neither (0);
  *   "singletons": one-row relations are removed before planning and their values
  *       restrict the other members of their classes (QueryPlan.cpp:680-717) (1, default)
- *       or kept as relations (0). */
+ *       or kept as relations (0);
+ *   "packed_tables": the one-probe kernel reads a hash table of an INT32 key and one INT32
+ *       attribute as one 8-byte word per slot (1, default) or as two words (0). */
 int cgj_set_option(cgj_context* ctx, const char* name, double value);
 
 /* sizeof(cgj_stats) as compiled into the library (binding sanity check). */
