@@ -508,7 +508,7 @@ bool Context::set_option(const std::string& name, double v) {
     else if (name == "partitioned_build") options.partitioned_build = int32_t(v);
     else if (name == "lean_kernel") options.lean_kernel = v != 0;
     else if (name == "build_regions_log") options.build_regions_log = v >= 1 && v <= 14 ? uint32_t(v) : 14;
-    else if (name == "lean_waves_per_cu") options.lean_waves_per_cu = v >= 4 && v <= 64 ? uint32_t(v) : 32;
+    else if (name == "lean_waves_per_cu") options.lean_waves_per_cu = v >= 4 && v <= 64 ? uint32_t(v) : 24;
     else if (name == "aligned_pairs") options.aligned_pairs = v != 0;
     else if (name == "build_threads") options.build_threads = (v == 256 || v == 512) ? uint32_t(v) : 1024u;
     else if (name == "group_walk") options.group_walk = v >= 0 && v < 1e6 ? uint32_t(v) : 32u;
@@ -889,21 +889,22 @@ uint64_t Executor::launch(PipeArgs& a, uint32_t nslots, uint64_t nrows, double& 
     uint64_t waves = waves_for(a, nslots, nrows);
     CGJ_HIP(hipEventRecord(ev0_, ctx_.stream()));
     if (a.nmorsels) {
-        if (use_lean(a, nslots)) {
-            // closing sums as per-wave stores, added up by a second small kernel
-            const uint64_t stride = (waves + 63) / 64 * 64;
-            if (wave_sums_.bytes() < stride * kSumFields * sizeof(unsigned long long))
-                wave_sums_ = DevBuf(&ctx_.heap(), stride * kSumFields * sizeof(unsigned long long));
-            a.wave_sums        = wave_sums_.get<unsigned long long>();
-            a.wave_sums_stride = uint32_t(stride);
+        // Closing sums as per-wave stores, added up by a second small kernel,
+        // and each wave's first pages or staging slots given out statically:
+        // no wave queues an atomic behind the others' at its start or end.
+        const uint64_t stride = (waves + 63) / 64 * 64;
+        if (wave_sums_.bytes() < stride * kSumFields * sizeof(unsigned long long))
+            wave_sums_ = DevBuf(&ctx_.heap(), stride * kSumFields * sizeof(unsigned long long));
+        a.wave_sums        = wave_sums_.get<unsigned long long>();
+        a.wave_sums_stride = uint32_t(stride);
+        a.static_chunk     = 1;
+        if (use_lean(a, nslots))
             CGJ_HIP(launch_probe1(a, nslots, uint32_t(waves), ctx_.options.aligned_pairs, ctx_.stream()));
-            CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
-            const uint32_t nout = a.target == TARGET_PAGES ? a.nout : 0;
-            CGJ_HIP(launch_fold_sums(a.wave_sums, a.wave_sums_stride, nout, a.counters, ctx_.stream()));
-        } else {
+        else
             CGJ_HIP(launch_pipeline(a, nslots, uint32_t(waves), ctx_.stream()));
-            CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
-        }
+        CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
+        const uint32_t nout = a.target == TARGET_PAGES ? a.nout : 0;
+        CGJ_HIP(launch_fold_sums(a.wave_sums, a.wave_sums_stride, nout, a.counters, ctx_.stream()));
     } else {
         CGJ_HIP(hipEventRecord(ev1_, ctx_.stream()));
     }
@@ -1616,15 +1617,15 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             if (c.fault) throw std::runtime_error("hash-table probe met no empty slot in its region");
             if (!c.overflow) {
                 t.count = c.tuples;
-                t.slots = std::min<uint64_t>(c.pages, cap);
+                t.slots = std::min<uint64_t>(c.reserved(), cap);
                 break;
             }
-            if (c.pages >= a.abort_pages)
+            if (c.reserved() >= a.abort_pages)
                 throw std::runtime_error("join result too large: more than " + std::to_string(a.abort_pages) +
                                          " staged tuples, beyond device memory");
             if (attempt > 2) throw std::runtime_error("staging overflow persists");
             ++retries;
-            cap = c.pages + waves * chunk + 64;
+            cap = c.reserved() + waves * chunk + 64;
             if (cap * a.stage_words * sizeof(uint64_t) > ctx_.mem_bytes())
                 throw std::runtime_error("join result too large: " + std::to_string(c.pages) +
                                          " staged tuples need more than the device memory");

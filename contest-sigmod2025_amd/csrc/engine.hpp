@@ -233,7 +233,7 @@ struct Options {
     int32_t  bloom_bits            = -1;                 // bloom bits per key (-1 = automatic)
     int32_t  partitioned_build     = -1;                 // radix-partitioned build (-1 = automatic)
     bool     lean_kernel           = true;               // use the one-probe kernel where it applies
-    uint32_t lean_waves_per_cu     = 32;                 // lean kernel grid: waves per CU (8 per SIMD)
+    uint32_t lean_waves_per_cu     = 24;                 // lean kernel grid: waves per CU (6 per SIMD)
     uint32_t build_regions_log     = 14;                 // partitioned build: at most 2^this regions
     bool     aligned_pairs         = true;               // lean probe: 2-slot groups aligned to 2 slots
     // Partitioned build: threads per workgroup (256/512/1024). Its LDS tables

@@ -28,16 +28,18 @@ constexpr int kScanBlock = 1024;
 // many rows as page 0 (the last page at most as many), dense[1] = rows of page 0.
 __global__ void k_page_counts(const uint8_t* pages, uint64_t npages, int32_t type, uint32_t* counts,
                               uint32_t* dense) {
-    uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= npages) return;
+    const uint64_t p0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool     in = p0 < npages;
+    const uint64_t p  = in ? p0 : npages - 1; // lanes past the end repeat the last page and write nothing
     uint32_t nr = *reinterpret_cast<const uint16_t*>(pages + p * kPageSize);
     uint32_t c  = nr;
     if (type == T_VARCHAR && nr >= 0xfffe) c = nr - 0xfffe; // long string: 1 row on its first page
-    counts[p] = c;
+    if (in) counts[p] = c;
     const uint32_t nv = *reinterpret_cast<const uint16_t*>(pages + p * kPageSize + 2);
     const uint32_t n0 = *reinterpret_cast<const uint16_t*>(pages);
-    if (p == 0) dense[1] = n0;
-    if (type == T_VARCHAR || nv != nr || (p + 1 < npages ? nr != n0 : nr > n0)) atomicOr(dense, 1u);
+    if (p0 == 0) dense[1] = n0;
+    const bool mixed = type == T_VARCHAR || nv != nr || (p + 1 < npages ? nr != n0 : nr > n0);
+    if (__ballot(mixed) && lead_lane()) atomicOr(dense, 1u); // one atomic per wave, not per page
 }
 
 __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
@@ -929,7 +931,8 @@ __device__ inline int32_t fixed_budget(int32_t type) { return 8 * (int32_t(kPage
 __device__ inline void page_open(const PipeArgs& a, WaveState& ws, OutState& st, uint64_t* bits, int32_t budget) {
     uint64_t next = ws.next, end = ws.end;
     if (next == end) {
-        const unsigned long long base = wave_fetch_add(&a.counters->pages, (unsigned long long)a.page_chunk);
+        const uint64_t           sbase = a.static_chunk ? uint64_t(gridDim.x) * kWavesPerBlock * a.page_chunk : 0;
+        const unsigned long long base  = sbase + wave_fetch_add(&a.counters->pages, (unsigned long long)a.page_chunk);
         next   = base;
         ws.end = base + a.page_chunk;
         if (base >= a.abort_pages) ws.stop = 1;
@@ -1126,7 +1129,8 @@ __device__ inline void stage_put(const PipeArgs& a, WaveState& ws, bool have, ui
     uint64_t rem  = end - next;
     uint64_t base2 = 0;
     if (cnt > rem) {
-        base2 = wave_fetch_add(&a.counters->pages, (unsigned long long)a.stage_chunk);
+        const uint64_t sbase = a.static_chunk ? uint64_t(gridDim.x) * kWavesPerBlock * a.stage_chunk : 0;
+        base2                = sbase + wave_fetch_add(&a.counters->pages, (unsigned long long)a.stage_chunk);
         if (base2 >= a.abort_pages) ws.stop = 1;
         if (base2 + a.stage_chunk > a.stage_cap && lead_lane()) atomicExch(&a.counters->overflow, 1ull);
     }
@@ -1156,16 +1160,26 @@ __device__ inline void finish_column(const PipeArgs& a, OutState& st, uint64_t* 
     else if (st.page != kNoPage && !st.bad && lead_lane())
         a.dir[4 * uint64_t(st.page) + 3] = 0; // opened, never filled
     if (lead_lane()) {
-        if (st.nnz) atomicAdd(&a.counters->nnz[c], (unsigned long long)st.nnz);
-        if (st.npages) atomicAdd(&a.counters->col_pages[c], (unsigned long long)st.npages);
+        if (a.wave_sums) {
+            a.wave_sums[uint64_t(1 + c) * a.wave_sums_stride + stream]           = st.nnz;
+            a.wave_sums[uint64_t(1 + kMaxOut + c) * a.wave_sums_stride + stream] = st.npages;
+        } else {
+            if (st.nnz) atomicAdd(&a.counters->nnz[c], (unsigned long long)st.nnz);
+            if (st.npages) atomicAdd(&a.counters->col_pages[c], (unsigned long long)st.npages);
+        }
     }
 }
 
 // Flushes the wave's tuple count and retires the unused part of its last
 // reservation (staging slots become holes, pool pages invalid directory entries).
-__device__ inline void finish_wave(const PipeArgs& a, const WaveState& ws, uint64_t local) {
+__device__ inline void finish_wave(const PipeArgs& a, const WaveState& ws, uint64_t local, uint32_t stream) {
     uint32_t lane = lane_id();
-    if (lead_lane() && local) atomicAdd(&a.counters->tuples, (unsigned long long)local);
+    if (lead_lane()) {
+        if (a.wave_sums)
+            a.wave_sums[stream] = local;
+        else if (local)
+            atomicAdd(&a.counters->tuples, (unsigned long long)local);
+    }
     if (a.target == TARGET_STAGE) {
         for (uint64_t i = ws.next + lane; i < ws.end && i < a.stage_cap; i += 64) a.stage[i * a.stage_words] = kEmpty;
     } else if (a.target == TARGET_PAGES) {
@@ -1246,6 +1260,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4
     if (a.target == TARGET_PAGES && lane < a.nout) {
         OutState s{kNoPage, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         ost[lane] = s;
+    }
+    if (a.wave_sums && stream == 0 && lead_lane()) a.counters->waves = nwaves;
+    if (a.static_chunk && (a.target == TARGET_PAGES || a.target == TARGET_STAGE)) {
+        // the wave's first pages (or staging slots) need no atomic
+        const uint64_t chunk = a.target == TARGET_PAGES ? a.page_chunk : a.stage_chunk;
+        ws.next              = uint64_t(stream) * chunk;
+        ws.end               = ws.next + chunk;
+        if (stream == 0 && lead_lane()) a.counters->static_pages = uint64_t(nwaves) * chunk;
+        if (a.target == TARGET_STAGE && ws.end > a.stage_cap && lead_lane()) atomicExch(&a.counters->overflow, 1ull);
     }
 
     uint64_t local_count = 0;
@@ -1530,7 +1553,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4
             finish_column(a, st, obits + c * 64, c, stream, stg);
         }
     }
-    finish_wave(a, ws, local_count);
+    finish_wave(a, ws, local_count, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -1692,8 +1715,8 @@ __device__ inline void lean_finish(const LeanArgs& a, LeanPage& st, uint64_t bit
         a.dir[4 * uint64_t(st.page) + 3] = 0; // opened, never filled
     if (lead_lane()) {
         if (a.wave_sums) {
-            a.wave_sums[uint64_t(1 + c) * a.wave_sums_stride + stream]            = st.nnz;
-            a.wave_sums[uint64_t(1 + kLeanOut + c) * a.wave_sums_stride + stream] = st.npages;
+            a.wave_sums[uint64_t(1 + c) * a.wave_sums_stride + stream]           = st.nnz;
+            a.wave_sums[uint64_t(1 + kMaxOut + c) * a.wave_sums_stride + stream] = st.npages;
         } else {
             if (st.nnz) atomicAdd(&a.counters->nnz[c], (unsigned long long)st.nnz);
             if (st.npages) atomicAdd(&a.counters->col_pages[c], (unsigned long long)st.npages);
@@ -2143,15 +2166,17 @@ bool probe1_applicable(const PipeArgs& a, uint32_t slots) {
 __global__ __launch_bounds__(256) void k_pack_table(const uint64_t* ht, uint64_t cap, uint64_t* packed,
                                                     uint32_t* flag) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= cap) return;
-    const ulonglong2 e = *reinterpret_cast<const ulonglong2*>(ht + 2 * i);
-    uint64_t         w = kEmpty;
-    if (e.x != kEmpty) {
-        const bool fits = (e.x >> 32) == 0 && (e.y >> 32) == 0 && !(e.x == 0xffffffffull && e.y == 0xffffffffull);
-        if (!fits) atomicOr(flag, 1u);
-        w = e.x | (e.y << 32);
+    bool           bad = false;
+    if (i < cap) {
+        const ulonglong2 e = *reinterpret_cast<const ulonglong2*>(ht + 2 * i);
+        uint64_t         w = kEmpty;
+        if (e.x != kEmpty) {
+            bad = (e.x >> 32) != 0 || (e.y >> 32) != 0 || (e.x == 0xffffffffull && e.y == 0xffffffffull);
+            w   = e.x | (e.y << 32);
+        }
+        packed[i] = w;
     }
-    packed[i] = w;
+    if (__ballot(bad) && lead_lane()) atomicOr(flag, 1u); // one atomic per wave
 }
 
 // One workgroup per field: the sum over the grid's waves, added to the field.
@@ -2159,7 +2184,7 @@ __global__ __launch_bounds__(256) void k_fold_sums(const unsigned long long* ws,
                                                    PipeCounters* c) {
     __shared__ unsigned long long part[4];
     const uint32_t     f     = blockIdx.x; // 0: rows; 1..: nnz, then pages, of the used columns
-    const uint32_t     field = f == 0 ? 0u : f <= nout ? f : 1u + kLeanOut + (f - 1 - nout);
+    const uint32_t     field = f == 0 ? 0u : f <= nout ? f : 1u + kMaxOut + (f - 1 - nout);
     const uint64_t     n     = c->waves < stride ? c->waves : stride;
     unsigned long long s     = 0;
     for (uint64_t w = threadIdx.x; w < n; w += blockDim.x) s += ws[uint64_t(field) * stride + w];
@@ -2180,7 +2205,7 @@ __global__ __launch_bounds__(256) void k_fold_sums(const unsigned long long* ws,
 
 hipError_t launch_fold_sums(const unsigned long long* wave_sums, uint32_t stride, uint32_t nout, PipeCounters* c,
                             hipStream_t s) {
-    if (nout > uint32_t(kLeanOut)) return hipErrorInvalidValue;
+    if (nout > uint32_t(kMaxOut)) return hipErrorInvalidValue;
     k_fold_sums<<<dim3(1 + 2 * nout), dim3(256), 0, s>>>(wave_sums, stride, nout, c);
     return hipGetLastError();
 }
@@ -2287,6 +2312,8 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
 hipError_t launch_pipeline(const PipeArgs& a, uint32_t slots, uint32_t grid_waves, hipStream_t s) {
     uint32_t blocks = (grid_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks == 0) blocks = 1;
+    // per-wave sums are stored at index `wave` < stride
+    if (a.wave_sums && uint64_t(blocks) * kWavesPerBlock > a.wave_sums_stride) return hipErrorInvalidValue;
     size_t lds = size_t(kWavesPerBlock) * (kMaxOut * 6 + a.nout * 64) * sizeof(uint64_t);
     if (slots <= 4)
         k_pipeline<4><<<dim3(blocks), dim3(256), lds, s>>>(a);

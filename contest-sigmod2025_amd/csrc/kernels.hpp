@@ -119,16 +119,21 @@ struct PipeArgs {
                                  // restrictions (launch_filter_select), nullptr = all
     uint64_t        abort_pages; // a wave whose reservation starts at or past this many pool pages (or
                                  // staging slots) stops: the result cannot fit device memory
-    // One-probe kernel: per-wave closing sums (see LeanArgs::wave_sums).
+    // Per-wave closing sums instead of atomics (as LeanArgs::wave_sums).
     unsigned long long* wave_sums;
     uint32_t            wave_sums_stride;
+    // Nonzero: each wave owns pool pages (or staging slots) [wave * chunk,
+    // (wave + 1) * chunk) from the start, and atomics on counters->pages
+    // reserve after the grid's static part (as LeanArgs::static_chunk).
+    uint32_t            static_chunk;
 };
 
 // ---- lean one-probe pipeline ----
 constexpr int      kLeanOut   = 4;  // fixed-width output columns
 constexpr uint32_t kPageChunk = 32; // pool pages reserved per atomic (upper bound)
-// Fields of a wave's closing sums: rows, then non-NULL values and pages per output column.
-constexpr uint32_t kSumFields = 1 + 2 * kLeanOut;
+// Fields of a wave's closing sums: rows, non-NULL values of output column c
+// (field 1 + c), pages of output column c (field 1 + kMaxOut + c).
+constexpr uint32_t kSumFields = 1 + 2 * kMaxOut;
 
 // Where an output column's value comes from.
 enum : uint32_t { SRC_KEY = 0, SRC_SCAN = 1, SRC_HT = 2 };
@@ -338,8 +343,9 @@ uint32_t   pipeline_waves_per_block();
 // or attribute needs more than 32 bits, the attribute is NULL, or the packed
 // word would read as empty. `flag` must be zero before the launch.
 hipError_t launch_pack_table(const uint64_t* ht, uint64_t cap, uint64_t* packed, uint32_t* flag, hipStream_t s);
-// Adds the per-wave closing sums of a one-probe launch (LeanArgs::wave_sums)
-// of `nout` output columns into the counters' tuples, nnz and col_pages.
+// Adds the per-wave closing sums of a launch (LeanArgs::wave_sums,
+// PipeArgs::wave_sums) of `nout` output columns into the counters' tuples,
+// nnz and col_pages.
 hipError_t launch_fold_sums(const unsigned long long* wave_sums, uint32_t stride, uint32_t nout, PipeCounters* c,
                             hipStream_t s);
 

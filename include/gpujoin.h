@@ -146,7 +146,7 @@ typedef struct {
  *   "bloom_bits": bloom bits per key (-1 = automatic, 0 = none);
  *   "partitioned_build": radix-partitioned LDS build (-1 automatic, 0 off, 1 on);
  *   "lean_kernel": 1 (default) runs one-probe pipelines on the lean kernel, 0 on the general one;
- *   "lean_waves_per_cu": grid of the lean kernel in waves per compute unit (4..64, default 32);
+ *   "lean_waves_per_cu": grid of the lean kernel in waves per compute unit (4..64, default 24);
  *   "aligned_pairs": the lean kernel's probe steps read 2-slot groups aligned to two
  *       slots (1, default) or the next two slots (0);
  *   "build_threads": threads per workgroup of the partitioned build kernels (256, 512,
