@@ -1822,7 +1822,7 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                 // halve the dependent hash-table round trips.
                 bool     have = false, searching = live;
                 uint64_t hslot = 0, k1 = 0;
-                const uint64_t  odd   = a.aligned_pairs ? 1 : 0; // uniform
+                const uint64_t  odd   = (a.aligned_pairs || packed) ? 1 : 0; // uniform
                 while (__ballot(searching)) {
                     if (++walk > walk_limit) {
                         if (lead_lane()) atomicExch(&a.counters->fault, 1ull);
@@ -1837,17 +1837,11 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                         const uint64_t qn   = (qb & ~rmask) | ((qb + 1) & rmask);
                         uint64_t       ka, wa, kb, wb;
                         if (packed) {
-                            // one 8-byte word per slot; with aligned pairs both
-                            // slots come from one 16-byte load
-                            uint64_t ea, eb;
-                            if (odd) {
-                                const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(ht + qa);
-                                ea                 = v.x;
-                                eb                 = v.y;
-                            } else {
-                                ea = ht[qa];
-                                eb = ht[qb];
-                            }
+                            // one 8-byte word per slot, always aligned pairs: both
+                            // slots come from one 16-byte load (two loads of one
+                            // line can each miss the L2 and fetch it twice)
+                            const ulonglong2 v  = *reinterpret_cast<const ulonglong2*>(ht + qa);
+                            const uint64_t   ea = v.x, eb = v.y;
                             ka = ea == kEmpty ? kEmpty : (ea & 0xffffffffull);
                             wa = ea >> 32;
                             kb = eb == kEmpty ? kEmpty : (eb & 0xffffffffull);

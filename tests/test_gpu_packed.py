@@ -95,8 +95,8 @@ def test_packed_table_null_attribute_falls_back(ctx):
 
 
 def test_packed_table_consecutive_pairs(ctx):
-    """Probe steps over the next two slots (8-byte loads) instead of the
-    aligned pair (one 16-byte load)."""
+    """With probe steps over the next two slots selected for two-word tables,
+    a packed table still reads aligned pairs: same rows either way."""
     rng = np.random.default_rng(11)
     n = 50_000
     keys = rng.permutation(n)
