@@ -1850,24 +1850,20 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                             ht_entry2(ht, words, qa, ka, wa);
                             ht_entry2(ht, words, qb, kb, wb);
                         }
-                        if (!skip && ka == key) {
-                            have  = true;
-                            hslot = qa;
-                            k1    = wa;
-                            q     = qb;
-                        } else if (!skip && ka == kEmpty) {
-                            live = false;
-                        } else if (kb == key) {
-                            have  = true;
-                            hslot = qb;
-                            k1    = wb;
-                            q     = qn;
-                        } else if (kb == kEmpty) {
-                            live = false;
-                        } else {
-                            q = qn;
-                        }
-                        searching = live && !have;
+                        // the first deciding slot wins: a match, or an empty
+                        // slot that ends the walk. Selects, not an if-chain:
+                        // the chain's exec-mask bookkeeping cost more scalar
+                        // instructions than the whole step.
+                        const bool ma = !skip & (ka == key);
+                        const bool ea = !skip & (ka == kEmpty);
+                        const bool mb = !ma & !ea & (kb == key);
+                        const bool eb = !ma & !ea & !mb & (kb == kEmpty);
+                        have          = ma | mb;
+                        hslot         = ma ? qa : qb;
+                        k1            = ma ? wa : wb;
+                        q             = ma ? qb : qn;
+                        live          = live & !(ea | eb);
+                        searching     = live & !have;
                     }
                 }
                 if (unique > 0 || (unique < 0 && a.unique)) live = false; // a lane without a match has stopped
