@@ -45,20 +45,47 @@ size_t DeviceHeap::round(size_t b) {
     return (b + q - 1) & ~(q - 1);
 }
 
-// Cached (freed, not yet released) blocks of all heaps together stay below a
-// quarter of the device's memory: the contexts of execute_many each keep a
-// heap, and caches that grew without bound on every context would leave the
-// runtime itself no memory (kernel launches allocate too). A heap whose free
-// takes the total over the bound releases its own oldest blocks until the
-// total is 1/8 below it, in one go, since a release waits for the whole
-// device (hipFree synchronises it). While worker contexts run, the bound is
-// doubled and the heaps settle under it when the workers are done.
+// Released blocks of at least kSharedMin bytes go to one cache per device,
+// shared by the heaps of every context on it (execute_many's workers
+// included), so a table or page pool one worker released serves the next
+// query of any worker; smaller blocks stay in their heap's own cache. A
+// cached block remembers the stream of the heap that released it, and a heap
+// takes a block of its own stream when one fits (stream order makes its reuse
+// safe). For another stream: while worker contexts run, the release recorded
+// an event on the releasing stream and the new stream waits for it; outside
+// that, if the old stream still has work queued, an event recorded on it at
+// reuse (after the release, so after every kernel that could still use the
+// block) is waited for instead.
+//
+// The cache stays below a quarter of the device's memory: caches that grew
+// without bound would leave the runtime itself no memory (kernel launches
+// allocate too). A release that takes it over the bound frees the oldest
+// blocks until it is 1/8 below it, in one go, since a hipFree waits for the
+// whole device. While worker contexts run, the bound is doubled and the cache
+// settles under it when the workers are done.
 namespace {
-std::mutex               g_heaps_mu;
-std::vector<DeviceHeap*> g_heaps;
-std::atomic<uint64_t>    g_cached{0};
-std::atomic<uint32_t>    g_parallel{0};
-std::atomic<uint64_t>    g_cache_cap{0};
+constexpr size_t kSharedMin = size_t(4) << 20;
+struct CachedBlock {
+    size_t      bytes;
+    void*       ptr;
+    hipStream_t stream; // stream of the releasing heap (nullptr: idle)
+    hipEvent_t  ready;  // recorded on `stream` at release (nullptr: none)
+    std::multimap<size_t, std::list<CachedBlock>::iterator>::iterator by_size;
+};
+struct DeviceCache {
+    std::mutex                                              mu;
+    std::list<CachedBlock>                                  order; // oldest (coldest) first
+    std::multimap<size_t, std::list<CachedBlock>::iterator> by_size;
+    uint64_t                                                bytes = 0;
+    std::vector<hipEvent_t>                                 events; // idle events
+    uint32_t                                                heaps = 0;
+};
+constexpr int         kCacheDevices = 64;
+DeviceCache           g_cache[kCacheDevices];
+std::atomic<uint32_t> g_parallel{0};
+std::atomic<uint64_t> g_cache_cap{0};
+
+DeviceCache& device_cache(int device) { return g_cache[device >= 0 && device < kCacheDevices ? device : 0]; }
 
 uint64_t cache_cap() {
     uint64_t c = g_cache_cap.load();
@@ -69,101 +96,236 @@ uint64_t cache_cap() {
     }
     return c;
 }
+
+// Moves blocks from the cold end of `c` into `out` until at most `keep` bytes
+// stay cached (the caller holds c.mu and frees `out` after unlocking).
+void take_oldest(DeviceCache& c, uint64_t keep, std::vector<CachedBlock>& out) {
+    while (c.bytes > keep && !c.order.empty()) {
+        CachedBlock& b = c.order.front();
+        c.bytes -= b.bytes;
+        c.by_size.erase(b.by_size);
+        out.push_back(b);
+        c.order.pop_front();
+    }
+}
+
+// hipFree waits for the whole device, so no stream needs waiting for.
+void release(DeviceCache& c, std::vector<CachedBlock>& blocks) {
+    if (blocks.empty()) return;
+    for (auto& b : blocks) (void)hipFree(b.ptr);
+    std::lock_guard<std::mutex> lock(c.mu);
+    for (auto& b : blocks)
+        if (b.ready) c.events.push_back(b.ready);
+    blocks.clear();
+}
+
+// An idle event from the device's pool, or a new one (nullptr on failure).
+hipEvent_t take_event(DeviceCache& c) {
+    {
+        std::lock_guard<std::mutex> lock(c.mu);
+        if (!c.events.empty()) {
+            hipEvent_t e = c.events.back();
+            c.events.pop_back();
+            return e;
+        }
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
 } // namespace
 
 DeviceHeap::DeviceHeap() {
     if (hipGetDevice(&device_) != hipSuccess) device_ = 0;
-    std::lock_guard<std::mutex> lock(g_heaps_mu);
-    g_heaps.push_back(this);
+    DeviceCache&                c = device_cache(device_);
+    std::lock_guard<std::mutex> lock(c.mu);
+    ++c.heaps;
+}
+
+void DeviceHeap::set_stream(hipStream_t s) {
+    std::lock_guard<std::mutex> lock(mu_);
+    stream_ = s;
 }
 
 DeviceHeap::Parallel::Parallel() { ++g_parallel; }
 DeviceHeap::Parallel::~Parallel() { --g_parallel; }
 
 void DeviceHeap::settle(int device) {
-    const uint64_t              cap = cache_cap();
-    std::lock_guard<std::mutex> lock(g_heaps_mu);
-    for (DeviceHeap* h : g_heaps) {
-        if (g_cached <= cap) break;
-        if (h->device_ != device) continue;
-        std::lock_guard<std::mutex> hl(h->mu_);
-        const uint64_t              over = g_cached - std::min<uint64_t>(g_cached, cap - cap / 8);
-        h->evict_locked(h->cached_ - std::min<uint64_t>(h->cached_, over));
+    const uint64_t           cap = cache_cap();
+    DeviceCache&             c   = device_cache(device);
+    std::vector<CachedBlock> out;
+    {
+        std::lock_guard<std::mutex> lock(c.mu);
+        if (c.bytes > cap) take_oldest(c, cap - cap / 8, out);
     }
+    release(c, out);
 }
 
 void DeviceHeap::trim_device(int device) {
-    std::lock_guard<std::mutex> lock(g_heaps_mu);
-    for (DeviceHeap* h : g_heaps)
-        if (h->device_ == device) h->trim();
+    DeviceCache&             c = device_cache(device);
+    std::vector<CachedBlock> out;
+    {
+        std::lock_guard<std::mutex> lock(c.mu);
+        take_oldest(c, 0, out);
+    }
+    release(c, out);
+}
+
+void DeviceHeap::trim() { trim_device(device_); }
+
+void DeviceHeap::forget_stream(int device, hipStream_t s) {
+    DeviceCache&                c = device_cache(device);
+    std::lock_guard<std::mutex> lock(c.mu);
+    for (auto& b : c.order)
+        if (b.stream == s) b.stream = nullptr;
 }
 
 void* DeviceHeap::alloc(size_t bytes) {
     const size_t r = round(bytes);
+    DeviceCache& c = device_cache(device_);
+    hipStream_t  s;
     {
         std::lock_guard<std::mutex> lock(mu_);
-        auto                        it = free_.lower_bound(r);
-        if (it != free_.end() && it->first <= r + r / 2) {
-            const auto blk = it->second;
-            void*      p   = blk->ptr;
-            live_[p]       = blk->bytes;
-            cached_ -= blk->bytes;
-            g_cached -= blk->bytes;
-            free_.erase(it);
-            order_.erase(blk);
-            return p;
+        s = stream_;
+        if (r < kSharedMin) {
+            auto it = own_.lower_bound(r);
+            if (it != own_.end() && it->first <= r + r / 2) {
+                void* p  = it->second;
+                live_[p] = it->first;
+                own_bytes_ -= it->first;
+                own_.erase(it);
+                return p;
+            }
         }
     }
-    void*      p = nullptr;
-    hipError_t e = hipMalloc(&p, r);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        trim_device(device_);
-        e = hipMalloc(&p, r);
+    void*       p     = nullptr;
+    size_t      got   = 0;
+    hipStream_t prev  = nullptr;
+    hipEvent_t  ready = nullptr;
+    if (r >= kSharedMin) {
+        std::lock_guard<std::mutex> lock(c.mu);
+        // the first fitting block of this stream, else the first fitting one
+        auto best = c.by_size.end();
+        for (auto it = c.by_size.lower_bound(r); it != c.by_size.end() && it->first <= r + r / 2; ++it) {
+            if (best == c.by_size.end()) best = it;
+            if (it->second->stream == s) {
+                best = it;
+                break;
+            }
+        }
+        if (best != c.by_size.end()) {
+            const CachedBlock b = *best->second;
+            c.order.erase(best->second);
+            c.by_size.erase(best);
+            c.bytes -= b.bytes;
+            p     = b.ptr;
+            got   = b.bytes;
+            prev  = b.stream == s ? nullptr : b.stream; // the same stream orders its own reuse
+            ready = b.ready;
+            if (ready && !prev) {
+                c.events.push_back(ready);
+                ready = nullptr;
+            }
+        }
     }
-    if (e != hipSuccess) throw HipError("hipMalloc(" + std::to_string(r) + "): " + hipGetErrorString(e));
+    if (p && prev) {
+        hipError_t e = hipSuccess;
+        if (ready) {
+            // recorded at the release
+            e = s ? hipStreamWaitEvent(s, ready, 0) : hipEventSynchronize(ready);
+        } else if (hipStreamQuery(prev) != hipSuccess) {
+            (void)hipGetLastError(); // hipErrorNotReady: work still queued on the old stream
+            ready = take_event(c);
+            e     = ready ? hipEventRecord(ready, prev) : hipErrorOutOfMemory;
+            if (e == hipSuccess) e = s ? hipStreamWaitEvent(s, ready, 0) : hipEventSynchronize(ready);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            e = hipStreamSynchronize(prev);
+        }
+        if (ready) {
+            std::lock_guard<std::mutex> lock(c.mu);
+            c.events.push_back(ready);
+        }
+        CGJ_HIP(e);
+    }
+    if (!p) {
+        got          = r;
+        hipError_t e = hipMalloc(&p, r);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            trim_device(device_);
+            e = hipMalloc(&p, r);
+        }
+        if (e != hipSuccess) throw HipError("hipMalloc(" + std::to_string(r) + "): " + hipGetErrorString(e));
+    }
     std::lock_guard<std::mutex> lock(mu_);
-    live_[p] = r;
+    live_[p] = got;
     return p;
 }
 
-void DeviceHeap::free(void* p) {
-    std::lock_guard<std::mutex> lock(mu_);
-    auto                        it = live_.find(p);
-    if (it == live_.end()) return;
-    const size_t bytes = it->second;
-    live_.erase(it);
-    order_.push_back(Cached{bytes, p, {}});
-    auto blk     = std::prev(order_.end());
-    blk->by_size = free_.emplace(bytes, blk);
-    cached_ += bytes;
-    const uint64_t cap   = g_parallel ? 2 * cache_cap() : cache_cap();
-    const uint64_t total = g_cached += bytes;
-    if (total > cap) evict_locked(cached_ - std::min<uint64_t>(cached_, total - (cap - cap / 8)));
-}
-
-void DeviceHeap::evict_locked(size_t keep) {
-    while (cached_ > keep && !order_.empty()) {
-        Cached& c = order_.front();
-        (void)hipFree(c.ptr); // waits for work still using the block
-        cached_ -= c.bytes;
-        g_cached -= c.bytes;
-        free_.erase(c.by_size);
-        order_.pop_front();
+void DeviceHeap::free(void* p, bool evict_first) {
+    size_t      bytes;
+    hipStream_t s;
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto                        it = live_.find(p);
+        if (it == live_.end()) return;
+        bytes = it->second;
+        live_.erase(it);
+        s = stream_;
+        if (bytes < kSharedMin) {
+            // small blocks stay with their heap, bounded by kOwnMax bytes
+            own_.emplace(bytes, p);
+            own_bytes_ += bytes;
+            while (own_bytes_ > kOwnMax && !own_.empty()) {
+                auto big = std::prev(own_.end());
+                own_bytes_ -= big->first;
+                (void)hipFree(big->second);
+                own_.erase(big);
+            }
+            return;
+        }
     }
-}
-
-void DeviceHeap::trim() {
-    std::lock_guard<std::mutex> lock(mu_);
-    evict_locked(0);
+    DeviceCache& c     = device_cache(device_);
+    hipEvent_t   ready = nullptr;
+    if (s && g_parallel) {
+        // other workers may take the block while this stream still runs
+        ready = take_event(c);
+        if (ready && hipEventRecord(ready, s) != hipSuccess) {
+            (void)hipGetLastError();
+            std::lock_guard<std::mutex> lock(c.mu);
+            c.events.push_back(ready);
+            ready = nullptr; // the lazy path at reuse still orders it
+        }
+    }
+    std::vector<CachedBlock> out;
+    {
+        std::lock_guard<std::mutex> lock(c.mu);
+        auto blk =
+            c.order.insert(evict_first ? c.order.begin() : c.order.end(), CachedBlock{bytes, p, s, ready, {}});
+        blk->by_size = c.by_size.emplace(bytes, blk);
+        c.bytes += bytes;
+        const uint64_t cap = g_parallel ? 2 * cache_cap() : cache_cap();
+        if (c.bytes > cap) take_oldest(c, cap - cap / 8, out);
+    }
+    release(c, out);
 }
 
 DeviceHeap::~DeviceHeap() {
+    DeviceCache& c    = device_cache(device_);
+    bool         last = false;
     {
-        std::lock_guard<std::mutex> lock(g_heaps_mu);
-        g_heaps.erase(std::remove(g_heaps.begin(), g_heaps.end(), this), g_heaps.end());
+        std::lock_guard<std::mutex> lock(c.mu);
+        last = --c.heaps == 0;
     }
-    trim();
+    // the last heap of the device leaves nothing cached behind
+    if (last) trim_device(device_);
+    for (auto& kv : own_) (void)hipFree(kv.second);
+    own_.clear();
     for (auto& kv : live_) (void)hipFree(kv.first);
     live_.clear();
 }
@@ -209,6 +371,7 @@ Context::Context(int device) : device_(device), est_(std::make_unique<Estimator>
     keep_freed_pages();
     CGJ_HIP(hipSetDevice(device));
     CGJ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    heap_->set_stream(stream_);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         if (prop.multiProcessorCount > 0) cus_ = uint32_t(prop.multiProcessorCount);
@@ -228,8 +391,14 @@ Context::~Context() {
     if (sample_pin_) (void)hipHostFree(sample_pin_);
     workers_.clear();
     est_.reset(); // its network's buffers go back to the heap
-    heap_->trim();
-    if (stream_) (void)hipStreamDestroy(stream_);
+    // blocks released from now on (tables and results outliving the
+    // context) are idle: the stream is drained before it goes
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    heap_->set_stream(nullptr);
+    if (stream_) {
+        DeviceHeap::forget_stream(device_, stream_);
+        (void)hipStreamDestroy(stream_);
+    }
 }
 
 void Context::prepare_column(DevColumn& c) { prepare_columns({&c}); }
@@ -585,7 +754,8 @@ struct ProbeStep {
 struct Target {
     uint32_t              kind = TARGET_STAGE;
     std::vector<uint32_t> eqs;       // TARGET_STAGE: class of each staged word
-    double                est  = 0;  // estimated tuples
+    double                est  = 0;  // upper bound of the tuples (at least the estimate)
+    double                card = 0;  // the planner's estimate (0 = none)
     DevBuf                stage;     // TARGET_STAGE output
     uint64_t              count = 0; // tuples produced
     uint64_t              slots = 0; // TARGET_STAGE: staging slots used (unused ones have key kEmpty)
@@ -624,6 +794,7 @@ public:
     double   ms_build = 0, ms_probe = 0, ms_probe_kernel = 0, ms_partition = 0, ms_replan = 0;
     uint32_t launches = 0, retries = 0, retries_build = 0, probe_lean = 0, probe_waves = 0, replans = 0;
     uint32_t probe_packed = 0;
+    uint32_t pools_compacted = 0;
     uint32_t restrictions = 0, semijoins = 0;
     uint64_t build_tuples = 0, probe_rows = 0, build_bytes = 0, out_ref_pages = 0, probe_input_pages = 0;
 
@@ -786,6 +957,12 @@ private:
         return grid_waves(nrows, a.xcd_order != 0, per_cu_for(a, nslots));
     }
     const Tuples* partition_driver(uint32_t rel, uint32_t key_eq, uint64_t table_bytes);
+    // Output pools are sized for an upper bound of the result (the largest
+    // input of the set). A result whose pages fill little of its pool gets a
+    // pool of exactly its pages, so results held together (execute_many over
+    // thousands of plans) do not keep the unused slots: 1,850 such c5 results
+    // had asked for 715 GB of pools.
+    void compact_pool(DevResult& res);
     static constexpr uint32_t kMorselRows = 4096;
 };
 
@@ -830,6 +1007,28 @@ uint64_t Executor::grid_waves(uint64_t nrows, bool xcd_order, uint32_t per_cu) c
     uint64_t waves   = std::min<uint64_t>(std::max<uint64_t>(morsels, 1), uint64_t(ctx_.cu_count()) * per_cu);
     uint32_t per     = pipeline_waves_per_block() * (xcd_order ? 8 : 1);
     return (waves + per - 1) / per * per;
+}
+
+void Executor::compact_pool(DevResult& res) {
+    const uint64_t used = res.total_pages;
+    if (res.pool_pages <= 2 * used + 64 || (res.pool_pages - used) * kPageSize < (uint64_t(32) << 20)) return;
+    DevBuf pool(&ctx_.heap(), std::max<uint64_t>(used, 1) * (kPageSize + 16) + 64);
+    // counters_ is free again once the pipeline's counters were read back
+    PipeCounters* pc = counters_.get<PipeCounters>();
+    CGJ_HIP(hipMemsetAsync(&pc->morsel, 0, sizeof(pc->morsel), ctx_.stream()));
+    CGJ_HIP(hipMemsetAsync(&pc->fault, 0, sizeof(pc->fault), ctx_.stream()));
+    CGJ_HIP(launch_compact_pages(res.pool.get<uint8_t>(), res.pool_pages, res.reserved_pages, pool.get<uint8_t>(), used,
+                                 &pc->morsel, &pc->fault, ctx_.stream()));
+    unsigned long long moved = 0, fault = 0;
+    CGJ_HIP(hipMemcpyAsync(&moved, &pc->morsel, sizeof(moved), hipMemcpyDeviceToHost, ctx_.stream()));
+    CGJ_HIP(hipMemcpyAsync(&fault, &pc->fault, sizeof(fault), hipMemcpyDeviceToHost, ctx_.stream()));
+    CGJ_HIP(hipStreamSynchronize(ctx_.stream()));
+    if (fault || moved != used) throw std::runtime_error("result page directory does not match the page count");
+    res.pool           = std::move(pool);
+    res.pool.set_evict_first();
+    res.pool_pages     = used;
+    res.reserved_pages = used;
+    ++pools_compacted;
 }
 
 const Tuples* Executor::partition_driver(uint32_t rel, uint32_t key_eq, uint64_t table_bytes) {
@@ -1087,6 +1286,7 @@ const HashTable* Executor::build(int node, uint32_t key_eq, uint64_t probe_rows)
     t.kind = TARGET_STAGE;
     t.eqs  = eqs;
     t.est  = size_bound(tn.set, tn.card);
+    t.card = tn.card;
     std::vector<int> joins;
     int              leaf;
     walk(node, joins, leaf);
@@ -1597,11 +1797,15 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             a.stage_slot[w] = it->second;
         }
         // Each wave reserves `chunk` slots per atomic; the last reservation of a
-        // wave can stay partly unused (those slots carry key kEmpty).
+        // wave can stay partly unused (those slots carry key kEmpty). The area
+        // holds four times the planner's estimate (at least 64K tuples), never
+        // more than the bound (a larger result re-runs at its exact size), as
+        // the output pool does.
+        const double est = t.card > 0 ? std::min(t.est, std::max(4.0 * t.card, 65536.0)) : t.est;
         uint64_t waves = grid_waves(nrows, xcd_order);
-        double   per   = (probes.empty() ? double(nrows) : std::max(t.est, 1.0)) / double(waves);
+        double   per   = (probes.empty() ? double(nrows) : std::max(est, 1.0)) / double(waves);
         uint64_t chunk = std::min<uint64_t>(4096, std::max<uint64_t>(64, uint64_t(per)));
-        uint64_t cap   = probes.empty() ? nrows : uint64_t(std::min(std::max(t.est * 1.25, 1024.0), 4e9));
+        uint64_t cap   = probes.empty() ? nrows : uint64_t(std::min(std::max(est * 1.25, 1024.0), 4e9));
         cap            = std::max<uint64_t>(cap, 1) + waves * chunk;
         // staged tuples beyond what device memory holds cannot be built: the
         // pipeline stops reserving slots there instead of enumerating them all
@@ -1653,12 +1857,18 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         a.out[i].vc   = a.out[i].type == CGJ_VARCHAR ? a.nvarchar++ : 0;
         bytes_per_row += a.out[i].type == CGJ_INT32 ? 4.2 : (a.out[i].type == CGJ_VARCHAR ? 48.0 : 8.2);
     }
-    // Expected data pages; every wave also ends with a partly filled page per
-    // column and may hold a partly used reservation. So that those stay small
-    // beside the data, the grid gets about as many waves per CU as there are
-    // pages per column per CU (4..the default), and a reservation holds about
-    // one wave's share of the pages (1..kPageChunk).
-    const double   dp      = std::min(t.est, 4e9) * bytes_per_row / 7800.0;
+    // Expected data pages: for four times the planner's estimate (at least
+    // 64K rows), never more than the bound; a larger result overflows the pool
+    // and the pipeline re-runs at its exact size. A pool sized for the bound
+    // alone (the largest input of the set) is mostly unused when the result
+    // is small, and thousands of such results held together do not fit the
+    // device. Every wave also ends with a partly filled page per column and
+    // may hold a partly used reservation. So that those stay small beside the
+    // data, the grid gets about as many waves per CU as there are pages per
+    // column per CU (4..the default), and a reservation holds about one wave's
+    // share of the pages (1..kPageChunk).
+    const double   pool_rows = t.card > 0 ? std::min(t.est, std::max(4.0 * t.card, 65536.0)) : t.est;
+    const double   dp      = std::min(pool_rows, 4e9) * bytes_per_row / 7800.0;
     const uint32_t def_pcu = per_cu_for(a, nslots);
     a.grid_per_cu          = uint32_t(std::min<double>(
         def_pcu, std::max<double>(4.0, dp / (std::max<uint32_t>(a.nout, 1) * double(ctx_.cu_count())))));
@@ -1672,6 +1882,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
     a.abort_pages    = ctx_.mem_bytes() / kPageSize;
     for (int attempt = 0;; ++attempt) {
         res->pool    = DevBuf(&ctx_.heap(), pages * kPageSize + pages * 16 + 64);
+        res->pool.set_evict_first();
         a.pool       = res->pool.get<uint8_t>();
         a.pool_pages = pages;
         a.dir        = reinterpret_cast<uint32_t*>(a.pool + pages * kPageSize);
@@ -1686,6 +1897,7 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
             for (uint32_t i = 0; i < a.nout; ++i) res->total_pages += c.col_pages[i];
             res->pool_pages     = pages;
             res->reserved_pages = std::min<uint64_t>(c.reserved(), pages);
+            compact_pool(*res);
             ms_probe_kernel     = last_ms_;
             probe_lean          = use_lean(a, nslots) ? 1 : 0;
             probe_packed        = c.packed ? 1 : 0;
@@ -1750,6 +1962,7 @@ std::unique_ptr<DevResult> Executor::run(bool count_only) {
     Target t;
     t.kind   = count_only ? TARGET_COUNT : TARGET_PAGES;
     t.est    = size_bound(t_->nodes[t_->root].set, t_->nodes[t_->root].card);
+    t.card   = t_->nodes[t_->root].card;
     t.result = res.get();
     if (!run_chain(t_->root, t)) return res;
     if (count_only) res->nrows = t.count;
@@ -1877,6 +2090,7 @@ std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const Vert
             Target t;
             t.kind   = count_only ? TARGET_COUNT : TARGET_PAGES;
             t.est    = size_bound(tree.nodes[tree.root].set, tree.nodes[tree.root].card);
+            t.card   = tree.nodes[tree.root].card;
             t.result = res.get();
             if (!run_chain(tree.root, t)) return res;
             if (count_only) res->nrows = t.count;
@@ -1928,7 +2142,6 @@ std::vector<std::unique_ptr<DevResult>> Context::execute_many(const std::vector<
         workers_.back()->hints_  = hints_;
     }
     for (uint32_t w = 0; w < workers; ++w) workers_[w]->options = options;
-    heap_->trim(); // this context's cached blocks would sit idle while the workers run
     // Each worker plans its query with this context's estimator (statistics
     // catalog shared under a lock, the network's inference on the worker's own
     // stream) and executes it, so planning runs on as many threads as
@@ -2279,6 +2492,7 @@ std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const st
     stats.ms_probe_partition = ex.ms_partition;
     stats.probe_kernel      = ex.probe_lean;
     stats.probe_packed      = ex.probe_packed;
+    stats.pools_compacted   = ex.pools_compacted;
     stats.probe_waves       = ex.probe_waves;
     stats.build_tuples      = ex.build_tuples;
     stats.build_bytes       = ex.build_bytes;

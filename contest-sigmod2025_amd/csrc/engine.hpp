@@ -47,11 +47,23 @@ public:
     DeviceHeap();
     ~DeviceHeap();
     void* alloc(size_t bytes);
-    void  free(void* p);
+    // `evict_first`: the block goes to the cold end of the cache (released
+    // before any other). Result pools are freed in bulk when their results
+    // go, and would otherwise push out the tables and staging areas that the
+    // next queries reuse.
+    void  free(void* p, bool evict_first = false);
+    // Releases every cached block of this heap's device.
     void  trim();
+    // The stream the owning context runs on: a released block becomes
+    // reusable on another stream once this stream has passed the release.
+    // nullptr (the context is gone, its stream drained): blocks are idle.
+    void  set_stream(hipStream_t s);
+    // Cached blocks released on `s` become idle (the stream is drained and
+    // about to be destroyed).
+    static void forget_stream(int device, hipStream_t s);
     // While worker contexts run queries on one device, releases (each waits
     // for the whole device) are put off until the cached total passes twice
-    // the bound; settle() brings the device's heaps back under the bound
+    // the bound; settle() brings the device's cache back under the bound
     // once the workers are done.
     struct Parallel {
         Parallel();
@@ -61,22 +73,17 @@ public:
 
 private:
     static size_t round(size_t bytes);
-    // Releases the cached blocks of every heap of `device` (this one
-    // included): an allocation that failed retries after this.
+    // Releases the cached blocks of `device`: an allocation that failed
+    // retries after this.
     static void trim_device(int device);
-    // Releases the oldest cached blocks until at most `keep` bytes stay cached.
-    void        evict_locked(size_t keep);
-    struct Cached {
-        size_t                                       bytes;
-        void*                                        ptr;
-        std::multimap<size_t, std::list<Cached>::iterator>::iterator by_size;
-    };
-    int                                                 device_ = 0;
-    std::mutex                                          mu_; // worker contexts trim each other's caches
-    std::list<Cached>                                   order_; // cached blocks, oldest first
-    std::multimap<size_t, std::list<Cached>::iterator> free_;  // the same blocks by size
-    size_t                                              cached_ = 0;
-    std::unordered_map<void*, size_t>                   live_;
+    int                               device_ = 0;
+    std::mutex                        mu_;
+    hipStream_t                       stream_ = nullptr;
+    std::unordered_map<void*, size_t> live_;
+    // released blocks below the shared cache's minimum, for this heap alone
+    static constexpr size_t           kOwnMax = size_t(256) << 20;
+    std::multimap<size_t, void*>      own_;
+    size_t                            own_bytes_ = 0;
 };
 
 // RAII device buffer from a DeviceHeap.
@@ -86,21 +93,27 @@ public:
     DevBuf(DeviceHeap* h, size_t bytes)
         : heap_(h->shared_from_this()), bytes_(bytes), ptr_(bytes ? h->alloc(bytes) : nullptr) {}
     ~DevBuf() { reset(); }
-    DevBuf(DevBuf&& o) noexcept : heap_(std::move(o.heap_)), bytes_(o.bytes_), ptr_(o.ptr_) { o.ptr_ = nullptr; }
+    DevBuf(DevBuf&& o) noexcept
+        : heap_(std::move(o.heap_)), bytes_(o.bytes_), ptr_(o.ptr_), evict_first_(o.evict_first_) {
+        o.ptr_ = nullptr;
+    }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
             reset();
-            heap_  = std::move(o.heap_);
-            bytes_ = o.bytes_;
-            ptr_   = o.ptr_;
-            o.ptr_ = nullptr;
+            heap_        = std::move(o.heap_);
+            bytes_       = o.bytes_;
+            ptr_         = o.ptr_;
+            evict_first_ = o.evict_first_;
+            o.ptr_       = nullptr;
         }
         return *this;
     }
+    // On release the block goes to the cold end of its heap's cache.
+    void set_evict_first() { evict_first_ = true; }
     DevBuf(const DevBuf&)            = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     void reset() {
-        if (ptr_) heap_->free(ptr_);
+        if (ptr_) heap_->free(ptr_, evict_first_);
         ptr_ = nullptr;
     }
     template <class T = void>
@@ -109,8 +122,9 @@ public:
 
 private:
     std::shared_ptr<DeviceHeap> heap_;
-    size_t                      bytes_ = 0;
-    void*                       ptr_   = nullptr;
+    size_t                      bytes_       = 0;
+    void*                       ptr_         = nullptr;
+    bool                        evict_first_ = false;
 };
 
 // One column of a device-resident table.

@@ -276,6 +276,46 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t n16, hipStream_t s) 
     return hipGetLastError();
 }
 
+// Moves the valid pages of a result pool (directory word 3 == 1) into a pool
+// of exactly `dst_pages` slots, each with its directory entry, so a result
+// stops holding the slots its pool was sized for but did not use. One
+// workgroup per source page; destination slots come from a counter (the page
+// order is in the directory keys, not in the slot positions). A page beyond
+// `dst_pages` (a count that does not match the directory) sets *fault.
+__global__ __launch_bounds__(256) void k_compact_pages(const uint8_t* src, uint64_t src_pages, uint64_t reserved,
+                                                       uint8_t* dst, uint64_t dst_pages,
+                                                       unsigned long long* counter, unsigned long long* fault) {
+    __shared__ uint64_t slot;
+    const uint32_t*     sdir = reinterpret_cast<const uint32_t*>(src + src_pages * kPageSize);
+    uint32_t*           ddir = reinterpret_cast<uint32_t*>(dst + dst_pages * kPageSize);
+    for (uint64_t i = blockIdx.x; i < reserved; i += gridDim.x) {
+        const uint4 e = *reinterpret_cast<const uint4*>(sdir + 4 * i);
+        if (e.w != 1) continue; // uniform over the workgroup
+        if (threadIdx.x == 0) slot = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t d = slot;
+        __syncthreads(); // every thread has read `slot` before the next page's atomic
+        if (d >= dst_pages) {
+            if (threadIdx.x == 0) atomicExch(fault, 1ull);
+            continue;
+        }
+        const uint4* s = reinterpret_cast<const uint4*>(src + i * kPageSize);
+        uint4*       o = reinterpret_cast<uint4*>(dst + d * kPageSize);
+        for (uint32_t k = threadIdx.x; k < kPageSize / 16; k += blockDim.x) o[k] = s[k];
+        if (threadIdx.x == 0) *reinterpret_cast<uint4*>(ddir + 4 * d) = e;
+    }
+}
+
+hipError_t launch_compact_pages(const uint8_t* src, uint64_t src_pages, uint64_t reserved, uint8_t* dst,
+                                uint64_t dst_pages, unsigned long long* counter, unsigned long long* fault,
+                                hipStream_t s) {
+    if (reserved == 0) return hipSuccess;
+    const uint64_t wgs = reserved < 8192 ? reserved : 8192;
+    k_compact_pages<<<dim3(uint32_t(wgs)), dim3(256), 0, s>>>(src, src_pages, reserved, dst, dst_pages, counter,
+                                                             fault);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
     uint64_t i      = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     uint64_t stride = uint64_t(gridDim.x) * blockDim.x;

@@ -275,6 +275,13 @@ size_t     prepare_scratch_words(uint64_t npages);
 hipError_t launch_decode(const ScanCol& c, uint64_t nrows, uint64_t* out, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, uint64_t n16, hipStream_t s);
+// Valid pages of a result pool of `src_pages` slots (directory after the
+// slots, `reserved` entries to scan) moved into a pool of `dst_pages` slots;
+// *counter must start at 0, *fault is set when more pages are valid than
+// `dst_pages`.
+hipError_t launch_compact_pages(const uint8_t* src, uint64_t src_pages, uint64_t reserved, uint8_t* dst,
+                                uint64_t dst_pages, unsigned long long* counter, unsigned long long* fault,
+                                hipStream_t s);
 // flags[0] = duplicate keys, flags[2] = an insert passed walk_cap slots
 // holding its own key (0 = no cap) and stopped (the table is incomplete).
 hipError_t launch_ht_insert(const uint64_t* stage, uint64_t n, uint32_t words, uint64_t* ht, uint32_t shift,

@@ -104,6 +104,7 @@ class Stats(C.Structure):
         ("restrictions", C.c_uint32),
         ("semijoins", C.c_uint32),
         ("probe_packed", C.c_uint32),
+        ("pools_compacted", C.c_uint32),
     ]
 
     def as_dict(self):

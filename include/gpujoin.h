@@ -137,6 +137,8 @@ typedef struct {
                                    key's producers (duplicate-free, key only) */
     uint32_t probe_packed;      /* the output pipeline's lean kernel read its hash table packed
                                    (INT32 key | INT32 attribute << 32, 8 bytes per slot) */
+    uint32_t pools_compacted;   /* result page pools moved into a pool of exactly their pages
+                                   (the pool, sized for an upper bound, was mostly unused) */
 } cgj_stats;
 
 /* Execution options of a context (0 = success, -1 = unknown name):
