@@ -466,7 +466,9 @@ def run_c5(args):
             r.free()
         log(f"batched+learned step {step}: {per_step[-1]:.2f} s, {rows} rows")
     s_batched = min(per_step)
-    results["batched_learned"] = {"s_per_step": round(s_batched, 4), "workers": args.workers, "rows_per_step": rows}
+    results["batched_learned"] = {"s_per_step": round(s_batched, 4), "workers": args.workers, "rows_per_step": rows,
+                                  "s_steps": [round(x, 3) for x in per_step],
+                                  "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)}
     if rows != results["sample"]["rows_per_step"] or rows != results["learned"]["rows_per_step"]:
         raise RuntimeError(f"c5: result rows differ between estimators: {rows} (batched + learned), "
                            f"{results['sample']['rows_per_step']} (sampling), {results['learned']['rows_per_step']} (learned)")
@@ -537,6 +539,15 @@ def main():
         args.scale = 1.0
     if args.workers is None:  # the fastest counts measured on one MI355X
         args.workers = 8 if args.workload == "c5" else 6
+    if args.workload in ("c4", "c5"):
+        # One hardware queue per worker stream (plus the calling context's):
+        # with fewer, HIP maps several streams onto one queue and their
+        # kernels run one after another (c5: 2.0-3.5 s per step at HIP's
+        # default of 4 queues, 1.8-2.0 s at 12). Read at HIP's initialisation,
+        # which has not happened yet.
+        want = min(32, args.workers + 2)
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < want:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     if args.workload == "c5":
         return run_c5(args)
     if args.workload == "c4":

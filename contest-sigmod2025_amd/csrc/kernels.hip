@@ -1282,10 +1282,13 @@ __device__ inline uint64_t prefilter_blocks(const PipeArgs& a, uint64_t first, u
     return live;
 }
 
-// The narrow variant is held to 4 waves per SIMD (at most 128 VGPRs): left
-// alone the allocator lands one register above and loses a wave.
+// The narrow variant runs at 3 waves per SIMD (134 VGPRs). Held to 4 (128
+// VGPRs) it spilled 20 bytes per lane to scratch memory: every launch then
+// needs a scratch allocation per queue, which can fail when many worker
+// streams run with device memory nearly full, and the c5 plan set ran 2-3%
+// slower.
 template <int NA>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 4 : 1, 8))) void k_pipeline(PipeArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NA <= 4 ? 3 : 1, 8))) void k_pipeline(PipeArgs a) {
     extern __shared__ uint64_t lds[];
     const uint32_t wib    = wave_in_block();
     const uint32_t stream = blockIdx.x * kWavesPerBlock + wib;

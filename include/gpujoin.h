@@ -224,11 +224,17 @@ uint64_t    cgj_dtable_num_pages(const cgj_dtable* t, uint32_t column);
 /* Runs `num_plans` independent plans over device-resident inputs concurrently:
  * plan i reads the next num_inputs[i] handles of `inputs` (concatenated), and
  * out[i] receives its result (free each with cgj_dresult_free). `workers`
- * host threads (1..16), each with its own HIP stream and device heap, take
- * plans in turn, so one plan's host round trips overlap another's kernels;
- * workers order joins with the max-of-inputs rule. The contest harness calls
- * Contest::execute once per query (reference tests/read_sql.cpp:1254-1256);
- * this is that loop, batched. Results are those of cgj_execute_device. */
+ * host threads (1..16), each with its own HIP stream, take plans in turn, so
+ * one plan's host round trips overlap another's kernels; each worker plans
+ * its queries with the context's estimator (cgj_set_estimator). Released
+ * device blocks of 4 MiB and more are shared by the workers (reuse on another
+ * stream waits for the releasing stream). HIP maps streams onto
+ * GPU_MAX_HW_QUEUES hardware queues per process (default 4): with fewer
+ * queues than workers + 1, workers sharing a queue run their kernels one
+ * after another, so set it before the process first uses HIP. The contest
+ * harness calls Contest::execute once per query (reference
+ * tests/read_sql.cpp:1254-1256); this is that loop, batched. Results are
+ * those of cgj_execute_device. */
 int      cgj_execute_many(cgj_context* ctx, const cgj_plan* plans, uint32_t num_plans, cgj_dtable* const* inputs,
                           const uint32_t* num_inputs, uint32_t workers, cgj_dresult** out);
 
