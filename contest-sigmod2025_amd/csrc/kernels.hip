@@ -1803,16 +1803,30 @@ __device__ inline void lean_wave_init(LeanWave<NS>& w) {
     w.local = 0;
 }
 
+// Launch shapes of the one-probe kernel whose descriptor flags are known at
+// compile time. Every test of a flag costs scalar instructions in each block
+// or probe step, and the scalar unit (one per CU, shared by its four SIMDs)
+// is the kernel's busiest issue port: with the c2 shape compiled in, the
+// probe kernel went from 11.48 to 10.56 ms.
+enum : int {
+    LEAN_ANY    = 0, // flags read from the descriptor
+    LEAN_TWO    = 1, // page driver, no bloom filter, no selection words, 2-word entries read as aligned pairs
+    LEAN_PACKED = 2, // page driver, no bloom filter, no selection words, packed table (its flag still read)
+};
+
 // Runs the one-probe pipeline over rows [r0, end) (multiples of 64 except at
 // the end) of the descriptor at `args_addr` (re-read per 64-row block).
 // `unique` >= 0 overrides the descriptor's duplicate-free flag.
-template <int NO, int NS>
+template <int NO, int NS, int VAR>
 __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end, uint32_t stream, LeanWave<NS>& W,
                                  int32_t unique) {
-    const uint32_t lane = lane_id();
+    constexpr bool kPages = VAR != LEAN_ANY; // the driver is a page-stored table
+    constexpr bool kPlain = VAR != LEAN_ANY; // no bloom filter, no selection words
+    constexpr bool kTwo   = VAR == LEAN_TWO; // 2-word entries, never packed, aligned pairs
+    const uint32_t lane   = lane_id();
     // the packed table is used unless its build flagged an entry that does not fit
-    bool packed;
-    {
+    bool packed = false;
+    if (!kTwo) {
         const LeanArgs& a0 = args_at<LeanArgs>(args_addr);
         packed             = a0.ht_packed && load_invariant_u32(a0.pack_flag) == 0;
     }
@@ -1826,7 +1840,7 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
             uint64_t key  = kNull;
             bool     hole = false;
             uint64_t v[NS];
-            if (a.driver == DRIVER_PAGES) {
+            if (kPages || a.driver == DRIVER_PAGES) {
                 key = decode_block(a.key, blk, active);
 #pragma unroll
                 for (int c = 0; c < NO; ++c) v[c] = a.out[c].src == SRC_SCAN ? decode_block(a.out[c].col, blk, active) : kNull;
@@ -1840,15 +1854,16 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                 for (int c = 0; c < NO; ++c) v[c] = (a.out[c].src == SRC_SCAN && active) ? t[a.out[c].word] : kNull;
             }
             bool           alive = active && !hole && key != kNull;
-            if (a.driver == DRIVER_PAGES && a.sel) alive = alive && ((load_invariant_u64(a.sel + blk) >> lane) & 1);
+            if (!kPlain && a.driver == DRIVER_PAGES && a.sel)
+                alive = alive && ((load_invariant_u64(a.sel + blk) >> lane) & 1);
             const uint64_t h     = hash_key(key);
-            if (alive && a.bloom) {
+            if (!kPlain && alive && a.bloom) {
                 uint64_t h2 = hash_bloom(h);
                 uint64_t bb = bloom_bits(h2);
                 alive       = (a.bloom[bloom_word(h2, a.bloom_log)] & bb) == bb;
             }
             const uint64_t* ht    = packed ? a.ht_packed : a.ht;
-            const uint32_t  words = a.words;
+            const uint32_t  words = kTwo ? 2u : a.words;
             const uint64_t  rmask = a.mask;
             uint64_t        q     = h >> a.shift; // next slot to inspect
             bool            live  = alive;
@@ -1865,7 +1880,7 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                 // halve the dependent hash-table round trips.
                 bool     have = false, searching = live;
                 uint64_t hslot = 0, k1 = 0;
-                const uint64_t  odd   = (a.aligned_pairs || packed) ? 1 : 0; // uniform
+                const uint64_t  odd   = (kTwo || a.aligned_pairs || packed) ? 1 : 0; // uniform
                 while (__ballot(searching)) {
                     if (++walk > walk_limit) {
                         if (lead_lane()) atomicExch(&a.counters->fault, 1ull);
@@ -1879,7 +1894,7 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
                         const uint64_t qb   = (qa & ~rmask) | ((qa + 1) & rmask);
                         const uint64_t qn   = (qb & ~rmask) | ((qb + 1) & rmask);
                         uint64_t       ka, wa, kb, wb;
-                        if (packed) {
+                        if (!kTwo && packed) {
                             // one 8-byte word per slot, always aligned pairs: both
                             // slots come from one 16-byte load (two loads of one
                             // line can each miss the L2 and fetch it twice)
@@ -1962,8 +1977,8 @@ __device__ inline void lean_flush(const LeanArgs& a, LeanWave<NS>& W, uint32_t s
     lean_wave_init(W);
 }
 
-// NO = output columns (0: count only).
-template <int NO>
+// NO = output columns (0: count only); VAR = launch shape (LEAN_*).
+template <int NO, int VAR>
 __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
     (void)karg; // read through kernel_args()
     const LeanArgs& a     = kernel_args<LeanArgs>();
@@ -2002,7 +2017,7 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
         }
         const uint64_t r0  = m * a.morsel_rows;
         const uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
-        lean_rows<NO>(kernarg_addr(), r0, end, stream, W, -1);
+        lean_rows<NO, NS, VAR>(kernarg_addr(), r0, end, stream, W, -1);
         m = m_next;
     }
     lean_flush<NO>(a, W, stream);
@@ -2084,7 +2099,7 @@ __device__ inline void batch_probe_task(uintptr_t job_addr, uint64_t index, Lean
     const LeanArgs& a     = args_at<LeanArgs>(pa);
     const uint64_t  r0    = index * a.morsel_rows;
     const uint64_t  end   = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
-    lean_rows<NO>(pa, r0, end, stream, W, dup ? 0 : 1);
+    lean_rows<NO, NS, LEAN_ANY>(pa, r0, end, stream, W, dup ? 0 : 1);
     lean_flush<NO>(args_at<LeanArgs>(pa), W, stream);
 }
 
@@ -2303,28 +2318,39 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
         l.ht_packed = P.ht_packed;
         l.pack_flag = P.pack_flag;
     }
+    // the launch shape whose flags can be compiled in
+    int var = LEAN_ANY;
+    if (l.driver == DRIVER_PAGES && !l.bloom && !l.sel && !l.xcd_order) {
+        if (l.ht_packed)
+            var = LEAN_PACKED;
+        else if (l.words == 2 && l.aligned_pairs)
+            var = LEAN_TWO;
+    }
     // With the dynamic morsel schedule a grid larger than one resident wave
     // set only adds a tail: cap it at what the kernel's occupancy fits.
-    static const void* const kernels[kLeanOut + 1] = {
-        reinterpret_cast<const void*>(&k_lean<0>), reinterpret_cast<const void*>(&k_lean<1>),
-        reinterpret_cast<const void*>(&k_lean<2>), reinterpret_cast<const void*>(&k_lean<3>),
-        reinterpret_cast<const void*>(&k_lean<4>)};
+#define CGJ_LEAN_KERNELS(V)                                                                                  \
+    {reinterpret_cast<const void*>(&k_lean<0, V>), reinterpret_cast<const void*>(&k_lean<1, V>),            \
+     reinterpret_cast<const void*>(&k_lean<2, V>), reinterpret_cast<const void*>(&k_lean<3, V>),            \
+     reinterpret_cast<const void*>(&k_lean<4, V>)}
+    static const void* const kernels[3][kLeanOut + 1] = {CGJ_LEAN_KERNELS(LEAN_ANY), CGJ_LEAN_KERNELS(LEAN_TWO),
+                                                          CGJ_LEAN_KERNELS(LEAN_PACKED)};
+#undef CGJ_LEAN_KERNELS
     const uint32_t no = l.nout < uint32_t(kLeanOut) ? l.nout : uint32_t(kLeanOut);
     if (!a.xcd_order) {
         // resident workgroups per device and kernel, queried once (the
         // occupancy query costs more host time than a small launch)
         constexpr int              kMaxDev = 16;
-        static std::atomic<uint32_t> resident_cache[kMaxDev][kLeanOut + 1];
+        static std::atomic<uint32_t> resident_cache[kMaxDev][3][kLeanOut + 1];
         int                          dev = 0;
         if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kMaxDev) {
-            uint32_t resident = resident_cache[dev][no].load(std::memory_order_relaxed);
+            uint32_t resident = resident_cache[dev][var][no].load(std::memory_order_relaxed);
             if (resident == 0) {
                 int cus = 0, per_cu = 0;
                 if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-                    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[no], 256, 0) == hipSuccess &&
+                    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[var][no], 256, 0) == hipSuccess &&
                     per_cu > 0 && cus > 0) {
                     resident = uint32_t(per_cu) * uint32_t(cus);
-                    resident_cache[dev][no].store(resident, std::memory_order_relaxed);
+                    resident_cache[dev][var][no].store(resident, std::memory_order_relaxed);
                 }
             }
             if (resident && blocks > resident) blocks = resident;
@@ -2332,12 +2358,23 @@ hipError_t launch_probe1(const PipeArgs& a, uint32_t slots, uint32_t grid_waves,
     }
     // per-wave sums are stored at index `wave` < stride
     if (l.wave_sums && uint64_t(blocks) * kWavesPerBlock > l.wave_sums_stride) return hipErrorInvalidValue;
-    switch (no) {
-    case 0: k_lean<0><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
-    case 1: k_lean<1><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
-    case 2: k_lean<2><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
-    case 3: k_lean<3><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
-    default: k_lean<4><<<dim3(blocks), dim3(256), 0, s>>>(l); break;
+    const dim3 g(blocks), b(256);
+    switch (var * 8 + int(no)) {
+    case LEAN_ANY * 8 + 0: k_lean<0, LEAN_ANY><<<g, b, 0, s>>>(l); break;
+    case LEAN_ANY * 8 + 1: k_lean<1, LEAN_ANY><<<g, b, 0, s>>>(l); break;
+    case LEAN_ANY * 8 + 2: k_lean<2, LEAN_ANY><<<g, b, 0, s>>>(l); break;
+    case LEAN_ANY * 8 + 3: k_lean<3, LEAN_ANY><<<g, b, 0, s>>>(l); break;
+    case LEAN_ANY * 8 + 4: k_lean<4, LEAN_ANY><<<g, b, 0, s>>>(l); break;
+    case LEAN_TWO * 8 + 0: k_lean<0, LEAN_TWO><<<g, b, 0, s>>>(l); break;
+    case LEAN_TWO * 8 + 1: k_lean<1, LEAN_TWO><<<g, b, 0, s>>>(l); break;
+    case LEAN_TWO * 8 + 2: k_lean<2, LEAN_TWO><<<g, b, 0, s>>>(l); break;
+    case LEAN_TWO * 8 + 3: k_lean<3, LEAN_TWO><<<g, b, 0, s>>>(l); break;
+    case LEAN_TWO * 8 + 4: k_lean<4, LEAN_TWO><<<g, b, 0, s>>>(l); break;
+    case LEAN_PACKED * 8 + 0: k_lean<0, LEAN_PACKED><<<g, b, 0, s>>>(l); break;
+    case LEAN_PACKED * 8 + 1: k_lean<1, LEAN_PACKED><<<g, b, 0, s>>>(l); break;
+    case LEAN_PACKED * 8 + 2: k_lean<2, LEAN_PACKED><<<g, b, 0, s>>>(l); break;
+    case LEAN_PACKED * 8 + 3: k_lean<3, LEAN_PACKED><<<g, b, 0, s>>>(l); break;
+    default: k_lean<4, LEAN_PACKED><<<g, b, 0, s>>>(l); break;
     }
     return hipGetLastError();
 }
