@@ -14,13 +14,13 @@ from helpers import check_table
 pytestmark = pytest.mark.gpu
 
 SEED = 0x5EED00C7
-NB, NP = 100_000, 4_000_000
+NB, NP = 100_000, 8_000_000
 
 
 def gens():
     build = [gj.GenColumn(type=INT64, kind=gj.GEN_UNIQUE_KEY, stream=1, null_stream=11, null_frac=0.05, domain=NB),
              gj.GenColumn(type=FP64, kind=gj.GEN_UNIT_F64, stream=2, null_stream=12, null_frac=0.05)]
-    # 0.05% of the probe keys hit the build side: a ~2,000-row result from a 4M-row scan
+    # 0.05% of the probe keys hit the build side: a ~3,500-row result from an 8M-row scan
     # (the foreign keys follow the build keys' stream)
     probe = [gj.GenColumn(type=INT64, kind=gj.GEN_FK_ZIPF, stream=1, null_stream=13, null_frac=0.05, domain=NB,
                           zipf_s=0.5, miss_frac=0.9995),
@@ -52,7 +52,7 @@ def host_plan():
 
 
 def test_selective_join_moves_its_pool(ctx):
-    print("selective join 100K x 4M through host pages", flush=True)
+    print("selective join 100K x 8M through host pages", flush=True)
     plan = host_plan()
     exp = check_table(oracle.execute(plan), TYPES)
     got = ctx.execute(plan)
@@ -63,7 +63,7 @@ def test_selective_join_moves_its_pool(ctx):
 
 
 def test_moved_pools_in_execute_many(ctx):
-    print("selective join 100K x 4M, device tables, execute_many", flush=True)
+    print("selective join 100K x 8M, device tables, execute_many", flush=True)
     exp = check_table(oracle.execute(host_plan()), TYPES)
     g, p = gens()
     tabs = [ctx.generate(SEED, NB, g), ctx.generate(SEED, NP, p)]
