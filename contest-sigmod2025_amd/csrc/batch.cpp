@@ -196,6 +196,9 @@ void Batch::layout() {
         L.mask              = cap - 1;
         L.shift             = 64 - J.cap_log;
         L.words             = J.words;
+        // 2-word entries (16-byte aligned) are read as aligned pairs; the
+        // region is the whole table, so the pair's second slot wraps with it
+        L.aligned_pairs     = J.words == 2 ? 1u : 0u;
         L.bloom             = blm;
         L.bloom_log         = blog;
         L.nout              = J.nout;

@@ -2099,7 +2099,11 @@ __device__ inline void batch_probe_task(uintptr_t job_addr, uint64_t index, Lean
     const LeanArgs& a     = args_at<LeanArgs>(pa);
     const uint64_t  r0    = index * a.morsel_rows;
     const uint64_t  end   = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
-    lean_rows<NO, NS, LEAN_ANY>(pa, r0, end, stream, W, dup ? 0 : 1);
+    // jobs without a bloom filter on 2-word tables take the compiled-in shape
+    if (a.driver == DRIVER_PAGES && !a.bloom && !a.sel && !a.ht_packed && a.words == 2 && a.aligned_pairs)
+        lean_rows<NO, NS, LEAN_TWO>(pa, r0, end, stream, W, dup ? 0 : 1);
+    else
+        lean_rows<NO, NS, LEAN_ANY>(pa, r0, end, stream, W, dup ? 0 : 1);
     lean_flush<NO>(args_at<LeanArgs>(pa), W, stream);
 }
 
