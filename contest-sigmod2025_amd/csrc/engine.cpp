@@ -11,6 +11,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -2607,7 +2608,8 @@ void Context::download_column(const DevColumn& c, cgj_result_column& out) {
     }
 }
 
-void Context::download_pool(const PoolView& r, cgj_result& out) {
+void Context::download_pool(const PoolView& view, cgj_result& out) {
+    PoolView r      = view;
     out.num_rows    = r.nrows;
     out.num_columns = uint32_t(r.types.size());
     out.columns     = new cgj_result_column[r.types.size()];
@@ -2617,12 +2619,32 @@ void Context::download_pool(const PoolView& r, cgj_result& out) {
         out.columns[c].pages     = new void*[1];
     }
     if (r.total_pages == 0) return;
+    // Reserved pool pages that hold nothing (the unused part of each wave's
+    // last page reservation) would cross the link with the rest: a pool with
+    // more than 1/8 of them is first moved into a dense one on the device,
+    // at device bandwidth, so only written pages are copied to the host.
+    DevBuf                            dense, moved;
+    std::array<unsigned long long, 2> mv{0, 0};
+    if (r.reserved_pages > r.total_pages + r.total_pages / 8 + 256) {
+        dense = DevBuf(heap_.get(), r.total_pages * (kPageSize + 16) + 64);
+        moved = DevBuf(heap_.get(), 2 * sizeof(unsigned long long));
+        auto* m = moved.get<unsigned long long>();
+        CGJ_HIP(hipMemsetAsync(m, 0, 2 * sizeof(unsigned long long), stream_));
+        CGJ_HIP(launch_compact_pages(r.pool, r.pool_pages, r.reserved_pages, dense.get<uint8_t>(), r.total_pages, m,
+                                     m + 1, stream_));
+        CGJ_HIP(hipMemcpyAsync(mv.data(), m, sizeof(mv), hipMemcpyDeviceToHost, stream_));
+        r.pool           = dense.get<uint8_t>();
+        r.pool_pages     = r.total_pages;
+        r.reserved_pages = r.total_pages;
+    }
     // The directory has one entry per reserved pool page: (column, wave
     // stream, sequence, valid). Page order per column: stream, then sequence.
     std::vector<uint32_t> dir(r.reserved_pages * 4);
     CGJ_HIP(hipMemcpyAsync(dir.data(), r.pool + r.pool_pages * kPageSize, dir.size() * sizeof(uint32_t),
                            hipMemcpyDeviceToHost, stream_));
     CGJ_HIP(hipStreamSynchronize(stream_));
+    if (dense.get() && (mv[1] || mv[0] != r.total_pages))
+        throw std::runtime_error("page directory does not match the page count");
     std::vector<uint64_t> order;
     order.reserve(r.total_pages);
     for (uint64_t i = 0; i < r.reserved_pages; ++i)
