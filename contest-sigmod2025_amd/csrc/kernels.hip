@@ -1814,12 +1814,29 @@ enum : int {
     LEAN_PACKED = 2, // page driver, no bloom filter, no selection words, packed table (its flag still read)
 };
 
+// Probe state of the lanes that one probe step left searching, per wave in
+// LDS: key, next slot and the scanned output values. Once 64 have gathered
+// they are probed as a block of their own. Without it a wave steps as long as
+// the longest probe sequence of its 64 lanes (at load factor 0.35 and two
+// slots per step: 1.15 steps per lane on average, 3.0 for the longest of 64);
+// with it a block of fresh rows takes one step, and the 12% of lanes that
+// need more finish in full blocks of deferred lanes (about 1.45 steps per 64
+// rows in all). Unique tables only: a lane then stops at its first match.
+template <int NS>
+struct LeanQueue {
+    static constexpr uint32_t kCap = 128; // fewer than 64 held plus up to 64 added
+    uint64_t key[kCap];
+    uint64_t v[NS][kCap];
+    uint32_t slot[kCap];
+};
+
 // Runs the one-probe pipeline over rows [r0, end) (multiples of 64 except at
 // the end) of the descriptor at `args_addr` (re-read per 64-row block).
-// `unique` >= 0 overrides the descriptor's duplicate-free flag.
-template <int NO, int NS, int VAR>
+// `unique` >= 0 overrides the descriptor's duplicate-free flag. `Q` (DQ only)
+// is the wave's deferral queue; it is empty again when the call returns.
+template <int NO, int NS, int VAR, bool DQ = false>
 __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end, uint32_t stream, LeanWave<NS>& W,
-                                 int32_t unique) {
+                                 int32_t unique, LeanQueue<NS>* Q = nullptr) {
     constexpr bool kPages = VAR != LEAN_ANY; // the driver is a page-stored table
     constexpr bool kPlain = VAR != LEAN_ANY; // no bloom filter, no selection words
     constexpr bool kTwo   = VAR == LEAN_TWO; // 2-word entries, never packed, aligned pairs
@@ -1830,133 +1847,193 @@ __device__ inline void lean_rows(uintptr_t args_addr, uint64_t r0, uint64_t end,
         const LeanArgs& a0 = args_at<LeanArgs>(args_addr);
         packed             = a0.ht_packed && load_invariant_u32(a0.pack_flag) == 0;
     }
-    {
-        for (uint64_t rb = r0; rb < end && !W.ws.stop; rb += 64) {
-            const LeanArgs& a      = args_at<LeanArgs>(args_addr); // re-read per block (scalar cache)
-            const uint64_t  row    = rb + lane;
-            const bool      active = row < end;
-            const uint64_t  blk    = rb / 64;
-            // probe key and the scanned output columns: all loads issued up front
-            uint64_t key  = kNull;
-            bool     hole = false;
-            uint64_t v[NS];
-            if (kPages || a.driver == DRIVER_PAGES) {
-                key = decode_block(a.key, blk, active);
-#pragma unroll
-                for (int c = 0; c < NO; ++c) v[c] = a.out[c].src == SRC_SCAN ? decode_block(a.out[c].col, blk, active) : kNull;
-            } else {
-                const uint64_t* t = a.tuples + row * a.tuple_words;
-                if (active) {
-                    hole = t[0] == kEmpty;
-                    key  = t[a.key_word];
-                }
-#pragma unroll
-                for (int c = 0; c < NO; ++c) v[c] = (a.out[c].src == SRC_SCAN && active) ? t[a.out[c].word] : kNull;
+    // Walks the searching lanes until each has decided (a match or an empty
+    // slot), at most `max_steps` steps. Each step inspects two consecutive
+    // slots of every searching lane, which halves the dependent hash-table
+    // round trips. While the region holds an empty slot no lane takes more
+    // than rmask + 2 steps over all its rounds, so a block takes at most 64
+    // times that; past it the table is corrupt or full, and the block stops
+    // with the fault flag set instead of walking on.
+    auto walk_lanes = [&](const LeanArgs& a, uint64_t key, uint64_t& q, bool& live, bool& have, uint64_t& hslot,
+                          uint64_t& k1, uint64_t& walk, uint32_t max_steps) {
+        const uint64_t* ht         = packed ? a.ht_packed : a.ht;
+        const uint32_t  words      = kTwo ? 2u : a.words;
+        const uint64_t  rmask      = a.mask;
+        const uint64_t  walk_limit = 64 * (rmask + 2);
+        const uint64_t  odd        = (kTwo || a.aligned_pairs || packed) ? 1 : 0; // uniform
+        bool            searching  = live;
+        have                       = false;
+        hslot                      = 0;
+        k1                         = 0;
+        uint32_t steps             = 0;
+        while (__ballot(searching) && steps < max_steps) {
+            ++steps;
+            if (++walk > walk_limit) {
+                if (lead_lane()) atomicExch(&a.counters->fault, 1ull);
+                live = false;
+                break;
             }
-            bool           alive = active && !hole && key != kNull;
-            if (!kPlain && a.driver == DRIVER_PAGES && a.sel)
-                alive = alive && ((load_invariant_u64(a.sel + blk) >> lane) & 1);
-            const uint64_t h     = hash_key(key);
-            if (!kPlain && alive && a.bloom) {
-                uint64_t h2 = hash_bloom(h);
-                uint64_t bb = bloom_bits(h2);
-                alive       = (a.bloom[bloom_word(h2, a.bloom_log)] & bb) == bb;
-            }
-            const uint64_t* ht    = packed ? a.ht_packed : a.ht;
-            const uint32_t  words = kTwo ? 2u : a.words;
-            const uint64_t  rmask = a.mask;
-            uint64_t        q     = h >> a.shift; // next slot to inspect
-            bool            live  = alive;
-            // While the region holds an empty slot, no lane takes more than
-            // rmask + 2 steps over all its rounds, so the wave takes at most 64
-            // times that; past it the table is corrupt or full, and the block
-            // stops with the fault flag set instead of walking on.
-            const uint64_t walk_limit = 64 * (rmask + 2);
-            uint64_t       walk       = 0;
-            for (;;) {
-                // Each step inspects two consecutive slots of every searching
-                // lane; a lane stops at a match or an empty slot. The wave steps
-                // as long as its longest probe sequence, so two slots per step
-                // halve the dependent hash-table round trips.
-                bool     have = false, searching = live;
-                uint64_t hslot = 0, k1 = 0;
-                const uint64_t  odd   = (kTwo || a.aligned_pairs || packed) ? 1 : 0; // uniform
-                while (__ballot(searching)) {
-                    if (++walk > walk_limit) {
-                        if (lead_lane()) atomicExch(&a.counters->fault, 1ull);
-                        live = false;
-                        break;
-                    }
-                    if (searching) {
-                        // slots qa (ignored when `skip`) and qb, then continue at qn
-                        const uint64_t qa   = q & ~odd;
-                        const bool     skip = qa != q;
-                        const uint64_t qb   = (qa & ~rmask) | ((qa + 1) & rmask);
-                        const uint64_t qn   = (qb & ~rmask) | ((qb + 1) & rmask);
-                        uint64_t       ka, wa, kb, wb;
-                        if (!kTwo && packed) {
-                            // one 8-byte word per slot, always aligned pairs: both
-                            // slots come from one 16-byte load (two loads of one
-                            // line can each miss the L2 and fetch it twice)
-                            const ulonglong2 v  = *reinterpret_cast<const ulonglong2*>(ht + qa);
-                            const uint64_t   ea = v.x, eb = v.y;
-                            ka = ea == kEmpty ? kEmpty : (ea & 0xffffffffull);
-                            wa = ea >> 32;
-                            kb = eb == kEmpty ? kEmpty : (eb & 0xffffffffull);
-                            wb = eb >> 32;
-                        } else {
-                            ht_entry2(ht, words, qa, ka, wa);
-                            ht_entry2(ht, words, qb, kb, wb);
-                        }
-                        // the first deciding slot wins: a match, or an empty
-                        // slot that ends the walk. Selects, not an if-chain:
-                        // the chain's exec-mask bookkeeping cost more scalar
-                        // instructions than the whole step.
-                        const bool ma = !skip & (ka == key);
-                        const bool ea = !skip & (ka == kEmpty);
-                        const bool mb = !ma & !ea & (kb == key);
-                        const bool eb = !ma & !ea & !mb & (kb == kEmpty);
-                        have          = ma | mb;
-                        hslot         = ma ? qa : qb;
-                        k1            = ma ? wa : wb;
-                        q             = ma ? qb : qn;
-                        live          = live & !(ea | eb);
-                        searching     = live & !have;
-                    }
+            if (searching) {
+                // slots qa (ignored when `skip`) and qb, then continue at qn
+                const uint64_t qa   = q & ~odd;
+                const bool     skip = qa != q;
+                const uint64_t qb   = (qa & ~rmask) | ((qa + 1) & rmask);
+                const uint64_t qn   = (qb & ~rmask) | ((qb + 1) & rmask);
+                uint64_t       ka, wa, kb, wb;
+                if (!kTwo && packed) {
+                    // one 8-byte word per slot, always aligned pairs: both
+                    // slots come from one 16-byte load (two loads of one
+                    // line can each miss the L2 and fetch it twice)
+                    const ulonglong2 e2 = *reinterpret_cast<const ulonglong2*>(ht + qa);
+                    const uint64_t   ea = e2.x, eb = e2.y;
+                    ka = ea == kEmpty ? kEmpty : (ea & 0xffffffffull);
+                    wa = ea >> 32;
+                    kb = eb == kEmpty ? kEmpty : (eb & 0xffffffffull);
+                    wb = eb >> 32;
+                } else {
+                    ht_entry2(ht, words, qa, ka, wa);
+                    ht_entry2(ht, words, qb, kb, wb);
                 }
-                if (unique > 0 || (unique < 0 && a.unique)) live = false; // a lane without a match has stopped
-                const uint64_t* he = ht + hslot * words;
-                const uint64_t  hm = __ballot(have);
-                if (hm) {
-                    const uint32_t cnt = __popcll(hm);
-                    W.local += cnt;
-                    if (NO > 0) {
-                        const uint32_t rank = __popcll(hm & mask_lt());
-                        // forward permute: have-lanes to their rank, the others to cnt.. (a permutation)
-                        const uint32_t dst = have ? rank : cnt + __popcll(~hm & mask_lt());
-                        uint32_t       vbm = 0;
-                        uint64_t       o[NS];
-#pragma unroll
-                        for (int c = 0; c < NO; ++c) {
-                            const uint32_t src = a.out[c].src, w = a.out[c].word;
-                            o[c] = src == SRC_KEY ? key : src == SRC_SCAN ? v[c] : (!have ? kNull : w == 1 ? k1 : he[w]);
-                            if (have && o[c] != kNull) vbm |= 1u << c;
-                        }
-                        const uint32_t recv = uint32_t(__builtin_amdgcn_ds_permute(int(dst << 2), int(vbm)));
-#pragma unroll
-                        for (int c = 0; c < NO; ++c) {
-                            const bool     valid = (vbm >> c) & 1u;
-                            const uint64_t vm    = __ballot(valid);
-                            const uint64_t pat   = __ballot((recv >> c) & 1u);
-                            lean_append(a, W.ws, W.st[c], W.bits[c], a.out[c].type == T_INT32 ? 4u : 8u, o[c], valid,
-                                        rank, cnt, vm, pat, uint32_t(c), stream);
-                        }
-                    }
-                }
-                if (!__ballot(live) || W.ws.stop) break;
+                // the first deciding slot wins: a match, or an empty
+                // slot that ends the walk. Selects, not an if-chain:
+                // the chain's exec-mask bookkeeping cost more scalar
+                // instructions than the whole step.
+                const bool ma = !skip & (ka == key);
+                const bool ea = !skip & (ka == kEmpty);
+                const bool mb = !ma & !ea & (kb == key);
+                const bool eb = !ma & !ea & !mb & (kb == kEmpty);
+                have          = ma | mb;
+                hslot         = ma ? qa : qb;
+                k1            = ma ? wa : wb;
+                q             = ma ? qb : qn;
+                live          = live & !(ea | eb);
+                searching     = live & !have;
             }
         }
+    };
+    // Appends the matching lanes' output rows to the wave's open pages.
+    auto emit = [&](const LeanArgs& a, uint64_t key, const uint64_t (&v)[NS], bool have, uint64_t hslot,
+                    uint64_t k1) {
+        const uint64_t  hm = __ballot(have);
+        if (!hm) return;
+        const uint64_t* ht  = packed ? a.ht_packed : a.ht;
+        const uint64_t* he  = ht + hslot * (kTwo ? 2u : a.words);
+        const uint32_t  cnt = __popcll(hm);
+        W.local += cnt;
+        if (NO > 0) {
+            const uint32_t rank = __popcll(hm & mask_lt());
+            // forward permute: have-lanes to their rank, the others to cnt.. (a permutation)
+            const uint32_t dst = have ? rank : cnt + __popcll(~hm & mask_lt());
+            uint32_t       vbm = 0;
+            uint64_t       o[NS];
+#pragma unroll
+            for (int c = 0; c < NO; ++c) {
+                const uint32_t src = a.out[c].src, w = a.out[c].word;
+                o[c] = src == SRC_KEY ? key : src == SRC_SCAN ? v[c] : (!have ? kNull : w == 1 ? k1 : he[w]);
+                if (have && o[c] != kNull) vbm |= 1u << c;
+            }
+            const uint32_t recv = uint32_t(__builtin_amdgcn_ds_permute(int(dst << 2), int(vbm)));
+#pragma unroll
+            for (int c = 0; c < NO; ++c) {
+                const bool     valid = (vbm >> c) & 1u;
+                const uint64_t vm    = __ballot(valid);
+                const uint64_t pat   = __ballot((recv >> c) & 1u);
+                lean_append(a, W.ws, W.st[c], W.bits[c], a.out[c].type == T_INT32 ? 4u : 8u, o[c], valid, rank,
+                            cnt, vm, pat, uint32_t(c), stream);
+            }
+        }
+    };
+    // Probes the top `n` (1..64) entries of the queue to the end as one block.
+    uint32_t nq    = 0; // entries in the queue (wave-uniform, < 64 between blocks)
+    auto     drain = [&](uint32_t n) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const LeanArgs& a   = args_at<LeanArgs>(args_addr);
+        const bool      act = lane < n;
+        const uint32_t  e   = nq - n + lane; // in [nq - n, nq) for the active lanes
+        uint64_t        key = kNull, q = 0;
+        uint64_t        v[NS];
+#pragma unroll
+        for (int c = 0; c < NS; ++c) v[c] = kNull;
+        if (act) {
+            key = Q->key[e];
+            q   = Q->slot[e];
+#pragma unroll
+            for (int c = 0; c < NO; ++c)
+                if (a.out[c].src == SRC_SCAN) v[c] = Q->v[c][e];
+        }
+        bool     live = act, have;
+        uint64_t hslot, k1, walk = 0;
+        walk_lanes(a, key, q, live, have, hslot, k1, walk, ~0u);
+        emit(a, key, v, have, hslot, k1);
+        nq -= n;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    };
+    for (uint64_t rb = r0; rb < end && !W.ws.stop; rb += 64) {
+        const LeanArgs& a      = args_at<LeanArgs>(args_addr); // re-read per block (scalar cache)
+        const uint64_t  row    = rb + lane;
+        const bool      active = row < end;
+        const uint64_t  blk    = rb / 64;
+        // probe key and the scanned output columns: all loads issued up front
+        uint64_t key  = kNull;
+        bool     hole = false;
+        uint64_t v[NS];
+#pragma unroll
+        for (int c = 0; c < NS; ++c) v[c] = kNull;
+        if (kPages || a.driver == DRIVER_PAGES) {
+            key = decode_block(a.key, blk, active);
+#pragma unroll
+            for (int c = 0; c < NO; ++c) v[c] = a.out[c].src == SRC_SCAN ? decode_block(a.out[c].col, blk, active) : kNull;
+        } else {
+            const uint64_t* t = a.tuples + row * a.tuple_words;
+            if (active) {
+                hole = t[0] == kEmpty;
+                key  = t[a.key_word];
+            }
+#pragma unroll
+            for (int c = 0; c < NO; ++c) v[c] = (a.out[c].src == SRC_SCAN && active) ? t[a.out[c].word] : kNull;
+        }
+        bool alive = active && !hole && key != kNull;
+        if (!kPlain && a.driver == DRIVER_PAGES && a.sel) alive = alive && ((load_invariant_u64(a.sel + blk) >> lane) & 1);
+        const uint64_t h = hash_key(key);
+        if (!kPlain && alive && a.bloom) {
+            uint64_t h2 = hash_bloom(h);
+            uint64_t bb = bloom_bits(h2);
+            alive       = (a.bloom[bloom_word(h2, a.bloom_log)] & bb) == bb;
+        }
+        const bool uniq = unique > 0 || (unique < 0 && a.unique);
+        // deferral keeps slots as 32-bit values: tables of at most 2^32 slots
+        const bool defer = DQ && uniq && a.shift >= 32;
+        uint64_t   q     = h >> a.shift; // next slot to inspect
+        bool       live  = alive;
+        uint64_t   walk  = 0;
+        for (;;) {
+            bool     have;
+            uint64_t hslot, k1;
+            walk_lanes(a, key, q, live, have, hslot, k1, walk, defer ? 1u : ~0u);
+            if (DQ && defer) {
+                // lanes still searching after one step join the queue
+                const bool     left = live & !have;
+                const uint64_t lm   = __ballot(left);
+                if (lm) {
+                    const uint32_t at = nq + __popcll(lm & mask_lt()); // nq < 64: at < 128
+                    if (left) {
+                        Q->key[at]  = key;
+                        Q->slot[at] = uint32_t(q);
+#pragma unroll
+                        for (int c = 0; c < NO; ++c)
+                            if (a.out[c].src == SRC_SCAN) Q->v[c][at] = v[c];
+                    }
+                    nq += __popcll(lm);
+                }
+            }
+            if (uniq) live = false; // a lane without a match has stopped
+            emit(a, key, v, have, hslot, k1);
+            if (!__ballot(live) || W.ws.stop) break;
+        }
+        if (DQ && nq >= 64) drain(64);
     }
+    if (DQ)
+        while (nq > 0 && !W.ws.stop) drain(nq < 64 ? nq : 64u);
 }
 
 // Closes the wave's open pages, adds its counts to the counters and retires
@@ -1988,6 +2065,13 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
     LeanWave<NS>   W;
     lean_wave_init(W);
+    // the compiled-in shapes defer long probe walks (LeanQueue)
+    constexpr bool DQ = VAR != LEAN_ANY;
+    LeanQueue<NS>* Q  = nullptr;
+    if constexpr (DQ) {
+        __shared__ LeanQueue<NS> queues[kWavesPerBlock];
+        Q = &queues[wib];
+    }
     if (stream == 0 && a.ht_packed && load_invariant_u32(a.pack_flag) == 0 && lead_lane()) a.counters->packed = 1;
     if (a.wave_sums && stream == 0 && lead_lane()) a.counters->waves = nwaves;
     if (NO > 0 && a.static_chunk) {
@@ -2017,7 +2101,7 @@ __global__ __launch_bounds__(256) void k_lean(LeanArgs karg) {
         }
         const uint64_t r0  = m * a.morsel_rows;
         const uint64_t end = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
-        lean_rows<NO, NS, VAR>(kernarg_addr(), r0, end, stream, W, -1);
+        lean_rows<NO, NS, VAR, DQ>(kernarg_addr(), r0, end, stream, W, -1, Q);
         m = m_next;
     }
     lean_flush<NO>(a, W, stream);
