@@ -828,8 +828,13 @@ private:
     // samples ahead of its last synchronisation (queue_refilter), and the
     // loop applies the hits afterwards (finish_refilter), so the re-filter
     // adds no round trip of its own.
+    // A table that gets a bloom filter is probed with the samples as well
+    // (`for_bloom`), even where nothing is left to plan: a pipeline probes it
+    // without its filter when most of the driving relation's sampled keys
+    // found a match (the filter would reject few rows).
     struct PendingRefilter {
         bool                                       armed   = false;
+        bool                                       for_bloom = false;
         uint32_t                                   key     = 0;
         uint64_t                                   grouped = 0;       // relations covered by tables after the build
         const HashTable*                           table   = nullptr; // table the queued probe reads (null: none)
@@ -1754,9 +1759,27 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
         P.grouped   = ht->grouped ? 1 : 0;
         P.pwords    = ht->words - 1;
         P.payload   = ht->grouped ? ht->payload.get<uint64_t>() : nullptr;
-        P.bloom     = ht->has_bloom ? ht->bloom.get<uint64_t>() : nullptr;
-        P.bloom_log = ht->bloom_log;
         P.early     = ks->second < scanned ? 1 : 0;
+        // A bloom filter pays only when most probe keys miss: a table that
+        // found at least half of the driving relation's sampled non-NULL keys
+        // is probed without it (c1 and the c4 joins: foreign keys into
+        // unfiltered build sides; the pre-filtered c3 dimensions keep theirs).
+        bool bloom = ht->has_bloom;
+        if (bloom && P.early && d.rel >= 0 && samples_) {
+            const auto it = samples_->pass[uint32_t(d.rel)].find(ht);
+            const auto kv = samples_->keys[uint32_t(d.rel)].find(probes[i].key_eq);
+            if (it != samples_->pass[uint32_t(d.rel)].end() && kv != samples_->keys[uint32_t(d.rel)].end()) {
+                uint32_t keys = 0, hits = 0;
+                for (size_t j = 0; j < kv->second.size() && j < 64; ++j)
+                    if (kv->second[j] != kNull) {
+                        ++keys;
+                        hits += uint32_t((it->second >> j) & 1u);
+                    }
+                if (keys >= 16 && 2 * hits >= keys) bloom = false;
+            }
+        }
+        P.bloom     = bloom ? ht->bloom.get<uint64_t>() : nullptr;
+        P.bloom_log = ht->bloom_log;
         if (probes.size() == 1 && (t.kind == TARGET_PAGES || t.kind == TARGET_COUNT) && packable(*ht)) {
             if (!ht->packed.get()) {
                 ht->packed    = DevBuf(&ctx_.heap(), ht->cap * sizeof(uint64_t));
@@ -1997,7 +2020,7 @@ bool Executor::next_build(int node, int& x, uint32_t& key, uint64_t& probe_rows)
 void Executor::queue_refilter(const HashTable* ht) {
     rf_.table = nullptr;
     rf_.todo.clear();
-    if (!rf_.armed || !samples_) return;
+    if (!samples_ || !(rf_.armed || (rf_.for_bloom && ht->has_bloom))) return;
     // relations still planned on their own that produce the key class
     size_t n = 0;
     for (uint32_t r = 0; r < q_.rels.size(); ++r) {
@@ -2111,8 +2134,9 @@ std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const Vert
                 ++ntables;
             }
         const uint32_t left = uint32_t(q_.rels.size()) - uint32_t(__builtin_popcountll(covered));
-        rf_.armed   = samples_ && (left >= 2 || ntables >= 2);
-        rf_.key     = key;
+        rf_.armed     = samples_ && (left >= 2 || ntables >= 2);
+        rf_.for_bloom = samples_ != nullptr;
+        rf_.key       = key;
         rf_.grouped = covered;
         rf_.table   = nullptr;
         const HashTable* ht = build(x, key, prows);
@@ -2123,7 +2147,8 @@ std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const Vert
             if ((g.set & set) != g.set) keep.push_back(g);
         if (!semi) keep.push_back(Group{set, key, ht});
         groups_ = std::move(keep);
-        if (rf_.armed) finish_refilter(ht);
+        if (rf_.armed || (rf_.for_bloom && ht->has_bloom)) finish_refilter(ht);
+        rf_.for_bloom = false;
         rf_.armed = false;
     }
 }

@@ -3,7 +3,10 @@ upper bound of the result (the largest input of the joined set); a result that
 fills little of it is moved into a pool of exactly its pages, so results held
 together (execute_many over many plans) do not keep the unused slots. Rows stay
 identical to the oracle's, through host pages, device tables and
-execute_many, and the moved pool's directory still orders every column."""
+execute_many, and the moved pool's directory still orders every column.
+
+These joins are planned with the max rule over exact sizes: the sampler would
+see that few probe keys match and size the pool close to the result."""
 import pytest
 
 import gpujoin as gj
@@ -42,6 +45,13 @@ def plan_for(build=None, probe=None):
 
 
 TYPES = [FP64, INT64, INT64]
+
+
+@pytest.fixture(autouse=True)
+def max_rule(ctx):
+    ctx.set_estimator(gj.EST_MAX)
+    yield
+    ctx.set_estimator(gj.EST_SAMPLE)
 
 
 def host_plan():
