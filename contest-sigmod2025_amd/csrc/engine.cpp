@@ -2135,7 +2135,9 @@ std::unique_ptr<DevResult> Executor::run_incremental(bool count_only, const Vert
             }
         const uint32_t left = uint32_t(q_.rels.size()) - uint32_t(__builtin_popcountll(covered));
         rf_.armed     = samples_ && (left >= 2 || ntables >= 2);
-        rf_.for_bloom = samples_ != nullptr;
+        // (a filter's cost grows with the probe side: below 1M probe rows
+        // the extra sample probe costs more than skipping the filter saves)
+        rf_.for_bloom = samples_ != nullptr && prows >= (uint64_t(1) << 20);
         rf_.key       = key;
         rf_.grouped = covered;
         rf_.table   = nullptr;
