@@ -796,6 +796,7 @@ public:
     uint32_t launches = 0, retries = 0, retries_build = 0, probe_lean = 0, probe_waves = 0, replans = 0;
     uint32_t probe_packed = 0;
     uint32_t pools_compacted = 0;
+    uint32_t blooms_skipped  = 0;
     uint32_t restrictions = 0, semijoins = 0;
     uint64_t build_tuples = 0, probe_rows = 0, build_bytes = 0, out_ref_pages = 0, probe_input_pages = 0;
 
@@ -1775,7 +1776,10 @@ void Executor::pipeline(Driver d, std::vector<ProbeStep> probes, Target& t) {
                         ++keys;
                         hits += uint32_t((it->second >> j) & 1u);
                     }
-                if (keys >= 16 && 2 * hits >= keys) bloom = false;
+                if (keys >= 16 && 2 * hits >= keys) {
+                    bloom = false;
+                    ++blooms_skipped;
+                }
             }
         }
         P.bloom     = bloom ? ht->bloom.get<uint64_t>() : nullptr;
@@ -2523,6 +2527,7 @@ std::unique_ptr<DevResult> Context::run_planned(const PlannedQuery& pq, const st
     stats.probe_kernel      = ex.probe_lean;
     stats.probe_packed      = ex.probe_packed;
     stats.pools_compacted   = ex.pools_compacted;
+    stats.blooms_skipped    = ex.blooms_skipped;
     stats.probe_waves       = ex.probe_waves;
     stats.build_tuples      = ex.build_tuples;
     stats.build_bytes       = ex.build_bytes;

@@ -105,6 +105,7 @@ class Stats(C.Structure):
         ("semijoins", C.c_uint32),
         ("probe_packed", C.c_uint32),
         ("pools_compacted", C.c_uint32),
+        ("blooms_skipped", C.c_uint32),
     ]
 
     def as_dict(self):

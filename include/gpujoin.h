@@ -139,6 +139,8 @@ typedef struct {
                                    (INT32 key | INT32 attribute << 32, 8 bytes per slot) */
     uint32_t pools_compacted;   /* result page pools moved into a pool of exactly their pages
                                    (the pool, sized for an upper bound, was mostly unused) */
+    uint32_t blooms_skipped;    /* probes that left out their table's bloom filter: at least
+                                   half of the driving relation's sampled keys matched */
 } cgj_stats;
 
 /* Execution options of a context (0 = success, -1 = unknown name):
