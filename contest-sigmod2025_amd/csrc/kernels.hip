@@ -2174,7 +2174,7 @@ __device__ inline void batch_build_task(uintptr_t job_addr, uint64_t index) {
 // directory keys (column, task, sequence) stay unique; a download orders the
 // pages by task, i.e. by probe row range.
 template <int NO, int NS>
-__device__ inline void batch_probe_task(uintptr_t job_addr, uint64_t index, LeanWave<NS>& W) {
+__device__ inline void batch_probe_task(uintptr_t job_addr, uint64_t index, LeanWave<NS>& W, LeanQueue<NS>* Q) {
     const uint32_t  stream = uint32_t(index);
     const BatchJob& J    = args_at<BatchJob>(job_addr);
     BatchSync*      sync = J.sync;
@@ -2185,7 +2185,7 @@ __device__ inline void batch_probe_task(uintptr_t job_addr, uint64_t index, Lean
     const uint64_t  end   = r0 + a.morsel_rows < a.nrows ? r0 + a.morsel_rows : a.nrows;
     // jobs without a bloom filter on 2-word tables take the compiled-in shape
     if (a.driver == DRIVER_PAGES && !a.bloom && !a.sel && !a.ht_packed && a.words == 2 && a.aligned_pairs)
-        lean_rows<NO, NS, LEAN_TWO>(pa, r0, end, stream, W, dup ? 0 : 1);
+        lean_rows<NO, NS, LEAN_TWO, true>(pa, r0, end, stream, W, dup ? 0 : 1, Q);
     else
         lean_rows<NO, NS, LEAN_ANY>(pa, r0, end, stream, W, dup ? 0 : 1);
     lean_flush<NO>(args_at<LeanArgs>(pa), W, stream);
@@ -2215,6 +2215,9 @@ __global__ __launch_bounds__(256) void k_batch_probe(const BatchJob* jobs, const
     constexpr int NS = NO > 0 ? NO : 1;
     LeanWave<NS>  W;
     lean_wave_init(W);
+    // deferral queue of the compiled-in shape (empty between tasks)
+    __shared__ LeanQueue<NS> queues[kWavesPerBlock];
+    LeanQueue<NS>* Q = &queues[wave_in_block()];
     for (;;) {
         const unsigned long long t = wave_fetch_add(queue, 1ull);
         if (t >= ntasks) break;
@@ -2222,7 +2225,7 @@ __global__ __launch_bounds__(256) void k_batch_probe(const BatchJob* jobs, const
         const uintptr_t ja  = uintptr_t(jobs + tk.x);
         const uint64_t  idx = uint64_t(tk.z) | (uint64_t(tk.w) << 32);
         const uint64_t  t0  = wall_clock64();
-        batch_probe_task<NO>(ja, idx, W);
+        batch_probe_task<NO>(ja, idx, W, Q);
         const uint64_t dt   = wall_clock64() - t0;
         BatchSync*     sync = args_at<BatchJob>(ja).sync;
         wave_fetch_add(&sync->probe_clocks, (unsigned long long)dt);
